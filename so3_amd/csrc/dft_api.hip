@@ -1,0 +1,60 @@
+// dft_api.hip -- so3x_dft: the library's batched DFT kernel on contiguous
+// sequences, exported so that its numerics can be tested in isolation.
+#include <algorithm>
+#include <string>
+
+#include "fft_plan.hpp"
+#include "kernels.hpp"
+#include "so3/so3x.h"
+
+namespace so3g {
+namespace {
+
+template <int DIR>
+__global__ __launch_bounds__(256) void k_dft(FftDesc fd, int K, int ld, double2 *__restrict__ X,
+                                             int howmany) {
+  extern __shared__ double2 lds[];
+  double2 *A = lds, *B = lds + K * ld;
+  const int n = fd.n;
+  const int64_t r0 = (int64_t)blockIdx.x * K;
+  for (int idx = threadIdx.x; idx < K * n; idx += blockDim.x) {
+    const int s = idx / n, k = idx - s * n;
+    A[s * ld + k] = (r0 + s < howmany) ? X[(r0 + s) * n + k] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  const double2 *R = fft_run<DIR>(A, B, K, ld, fd);
+  for (int idx = threadIdx.x; idx < K * n; idx += blockDim.x) {
+    const int s = idx / n, k = idx - s * n;
+    if (r0 + s < howmany) X[(r0 + s) * n + k] = R[s * ld + k];
+  }
+}
+
+}  // namespace
+}  // namespace so3g
+
+
+extern "C" int so3x_dft(void *x, int n, int howmany, int dir, void *stream) {
+  using namespace so3g;
+  try {
+    if (n < 1 || howmany < 0 || (dir != 1 && dir != -1)) throw HipError("bad so3x_dft arguments");
+    const FftDesc fd = fft_plan(n);
+    const int ld = fft_ld(fd);
+    const int K = std::max(1, std::min(8, lds_limit() / (2 * ld * (int)sizeof(double2))));
+    const size_t sh = 2 * (size_t)K * ld * sizeof(double2);
+    if (sh > (size_t)lds_limit()) throw HipError("DFT length exceeds the LDS budget");
+    const unsigned grid = (unsigned)((howmany + K - 1) / K);
+    if (grid == 0) return 0;
+    if (dir < 0) {
+      if (sh > 65536) SO3_HIP(hipFuncSetAttribute((const void *)k_dft<-1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+      k_dft<-1><<<grid, 256, sh, (hipStream_t)stream>>>(fd, K, ld, (double2 *)x, howmany);
+    } else {
+      if (sh > 65536) SO3_HIP(hipFuncSetAttribute((const void *)k_dft<+1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+      k_dft<+1><<<grid, 256, sh, (hipStream_t)stream>>>(fd, K, ld, (double2 *)x, howmany);
+    }
+    SO3_LAUNCH_CHECK();
+    return 0;
+  } catch (const std::exception &e) {
+    fprintf(stderr, "so3x_dft: %s\n", e.what());
+    return -1;
+  }
+}

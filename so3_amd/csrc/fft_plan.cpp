@@ -1,0 +1,130 @@
+// fft_plan.cpp -- factorisation, twiddle and Bluestein tables for the
+// device FFT (fft_device.hpp).  Tables are computed in long double on the
+// host and rounded once to FP64.
+#include "fft_plan.hpp"
+
+#include <cmath>
+#include <complex>
+#include <map>
+#include <mutex>
+#include <vector>
+
+namespace so3g {
+
+namespace {
+
+using cld = std::complex<long double>;
+const long double kTwoPi = 6.283185307179586476925286766559005768L;
+
+// exp(-2 pi i k / n) with the angle reduced to [0, 2pi) exactly in integers
+cld root(long long k, long long n) {
+  k %= n;
+  if (k < 0) k += n;
+  const long double a = kTwoPi * (long double)k / (long double)n;
+  return cld(cosl(a), -sinl(a));
+}
+
+// in-place radix-2 FFT in long double (host precomputation only)
+void host_fft_pow2(std::vector<cld> &x, int sign) {
+  const size_t n = x.size();
+  for (size_t i = 1, j = 0; i < n; ++i) {
+    size_t bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) std::swap(x[i], x[j]);
+  }
+  for (size_t len = 2; len <= n; len <<= 1) {
+    for (size_t i = 0; i < n; i += len)
+      for (size_t k = 0; k < len / 2; ++k) {
+        cld w = root((long long)k, (long long)len);
+        if (sign > 0) w = std::conj(w);
+        cld u = x[i + k], v = x[i + k + len / 2] * w;
+        x[i + k] = u + v;
+        x[i + k + len / 2] = u - v;
+      }
+  }
+}
+
+bool factor(int n, std::vector<int> &rad) {
+  rad.clear();
+  int m = n;
+  while (m % 8 == 0) { rad.push_back(8); m /= 8; }
+  while (m % 4 == 0) { rad.push_back(4); m /= 4; }
+  while (m % 2 == 0) { rad.push_back(2); m /= 2; }
+  for (int p : {3, 5, 7, 11, 13})
+    while (m % p == 0) { rad.push_back(p); m /= p; }
+  return m == 1 && (int)rad.size() <= kMaxPass;
+}
+
+double2 *upload(const std::vector<cld> &v) {
+  std::vector<double2> h(v.size());
+  for (size_t i = 0; i < v.size(); ++i) h[i] = make_double2((double)v[i].real(), (double)v[i].imag());
+  double2 *d = nullptr;
+  SO3_HIP(hipMalloc(&d, sizeof(double2) * (h.size() ? h.size() : 1)));
+  SO3_HIP(hipMemcpy(d, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice));
+  return d;
+}
+
+FftDesc build(int n) {
+  FftDesc d{};
+  d.n = n;
+  std::vector<int> rad;
+  if (n <= 1) {
+    d.len = 1;
+    d.npass = 0;
+    d.tw = upload(std::vector<cld>{cld(1, 0)});
+    return d;
+  }
+  if (factor(n, rad)) {
+    d.blue = 0;
+    d.len = n;
+  } else {
+    int M = 1;
+    while (M < 2 * n - 1) M *= 2;
+    d.blue = 1;
+    d.len = M;
+    factor(M, rad);
+    std::vector<cld> chirp(n), bf(M, cld(0, 0)), bb(M, cld(0, 0));
+    for (long long j = 0; j < n; ++j) {
+      // exp(-pi i j^2 / n) = root(j^2 mod 2n, 2n)
+      chirp[j] = root((j * j) % (2LL * n), 2LL * n);
+    }
+    for (int j = 0; j < n; ++j) {
+      bf[j] = std::conj(chirp[j]);
+      bb[j] = chirp[j];
+      if (j) {
+        bf[M - j] = std::conj(chirp[j]);
+        bb[M - j] = chirp[j];
+      }
+    }
+    host_fft_pow2(bf, -1);
+    host_fft_pow2(bb, -1);
+    d.chirp = upload(chirp);
+    d.bhat_f = upload(bf);
+    d.bhat_b = upload(bb);
+  }
+  d.npass = (int)rad.size();
+  for (int i = 0; i < d.npass; ++i) d.rad[i] = rad[i];
+  std::vector<cld> tw(d.len);
+  for (int k = 0; k < d.len; ++k) tw[k] = root(k, d.len);
+  d.tw = upload(tw);
+  return d;
+}
+
+}  // namespace
+
+FftDesc fft_plan(int n) {
+  static std::mutex mu;
+  static std::map<std::pair<int, int>, FftDesc> cache;
+  int dev = 0;
+  SO3_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> g(mu);
+  auto key = std::make_pair(dev, n);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  FftDesc d = build(n);
+  cache[key] = d;
+  return d;
+}
+
+}  // namespace so3g

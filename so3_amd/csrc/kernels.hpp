@@ -1,0 +1,55 @@
+// kernels.hpp -- geometry of one transform and the kernel entry points.
+#pragma once
+#include <cstdint>
+
+#include "dl_tables.hpp"
+#include "fft_device.hpp"
+
+namespace so3g {
+
+// Shape of one transform as the kernels see it.
+//   n slots: complex n in [-(N-1), N-1] at slot n mod (2N-1); real n in [0, N-1]
+//   m slots: m in [-(L-1), L-1] at slot m mod nA (MW_SS slot L is the unused
+//            Nyquist column)
+struct Geo {
+  int L, N, L0;
+  int nA, nB, nG;  // alpha / beta samples, n slots
+  int Pb;          // beta DFT length: 2L-1 (MW) or 2L (MW_SS)
+  int ss;          // MW_SS sampling
+  int real;        // real signal: n >= 0 only, c2r / r2c over gamma
+  int NS;          // n signs handled per |n|: 2 complex, 1 real
+  int nmodeL;      // keep only l == |n| (direct transforms with N_MODE_L)
+};
+
+// Per-n data: base[n + N - 1] = storage index of (l=0, m=0, n) in the flmn
+// layout (index(l,m,n) = base + l*l + l + m in both storages; negative for
+// compact storage), or kNoBlock when block n is not part of the transform.
+constexpr int64_t kNoBlock = INT64_MIN;
+struct NBase {
+  const int64_t *base;
+};
+
+// Workspace layouts (double2):
+//   X1 rows  : [n_slot][m_slot][m'], m' < L          (inverse contraction out)
+//   Gp       : [|n|][|m|][m'][2*NS] sign combos       (forward contraction in)
+//   X2       : [n_slot][b][a or m_slot]               (beta / alpha stages)
+void launch_contract_inv(const Geo &g, const double2 *flmn, NBase nb, const DlTables &dl,
+                         double2 *X1, hipStream_t st);
+void launch_contract_fwd(const Geo &g, const double2 *Gp, NBase nb, const DlTables &dl,
+                         double2 *flmn, hipStream_t st);
+
+void launch_inv_beta(const Geo &g, const double2 *X1, double2 *X2, NBase nb, const double2 *phase,
+                     hipStream_t st);
+void launch_alpha(const Geo &g, double2 *X2, int dir, hipStream_t st);
+void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, hipStream_t st);
+void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, hipStream_t st);
+void launch_fwd_beta(const Geo &g, const double2 *X2, double2 *Gp, NBase nb, const double2 *phase,
+                     const double2 *wr, int Pw, hipStream_t st);
+// zero the flmn entries a forward transform must write but does not compute
+void launch_fwd_zero(const Geo &g, double2 *flmn, NBase nb, int direct, int64_t flmn_size,
+                     int storage, int order, hipStream_t st);
+
+// LDS budget per workgroup, bytes
+int lds_limit();
+
+}  // namespace so3g

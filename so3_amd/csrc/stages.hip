@@ -1,0 +1,372 @@
+// stages.hip -- DFT stages of the Wigner transform (beta, alpha, gamma).
+//
+// Every stage is one launch of LDS-staged batched DFTs (fft_device.hpp); a
+// workgroup owns K sequences.  Reference counterparts:
+//   inverse beta : phase exp(i m' pi/(2L-1)), m' < 0 by F(-m') = (-1)^{m+n} F(m'),
+//                  DFT over m' -> b (so3_core.c:1148-1206; SSHT inverse)
+//   alpha        : DFT over m <-> a (so3_core.c:1168-1206, 1335-1364)
+//   inverse gamma: backward DFT over n (so3_core.c:137-158, 216; c2r :564-576)
+//   forward gamma: forward DFT over g times 2 pi/(2N-1) (so3_core.c:340-370,
+//                  r2c :779-812)
+//   forward beta : extension F(2L-2-b) = (-1)^{m+n} F(b), DFT over b, /(2L-1),
+//                  phase exp(-i m'' pi/(2L-1)), then the quadrature-weight
+//                  convolution G(m') = 2 pi sum_k F(k) w(m'-k) as a product in
+//                  the conjugate domain of length Pw >= 4L-3
+//                  (so3_core.c:1366-1492), folded to m' >= 0.
+#include <algorithm>
+
+#include "fft_plan.hpp"
+#include "kernels.hpp"
+
+namespace so3g {
+
+namespace {
+
+constexpr double kTwoPi = 6.283185307179586476925286766559005768;
+constexpr int kThreads = 256;
+
+__device__ inline int n_of_slot(const Geo &g, int slot) {
+  if (g.real) return slot;
+  return slot < g.N ? slot : slot - g.nG;
+}
+__device__ inline int m_of_slot(const Geo &g, int slot) { return slot <= g.L - 1 ? slot : slot - g.nA; }
+
+// ---------------------------------------------------------------- inverse beta
+__global__ __launch_bounds__(kThreads) void k_inv_beta(Geo g, FftDesc fd, int K, int ld,
+                                                       const double2 *__restrict__ X1,
+                                                       double2 *__restrict__ X2,
+                                                       const int64_t *__restrict__ nbase,
+                                                       const double2 *__restrict__ phase) {
+  extern __shared__ double2 lds[];
+  double2 *A = lds, *B = lds + K * ld;
+  const int chunks = (g.nA + K - 1) / K;
+  const int nslot = blockIdx.x / chunks;
+  const int ms0 = (blockIdx.x % chunks) * K;
+  const int n = n_of_slot(g, nslot);
+  const bool active = nbase[n + g.N - 1] != kNoBlock;
+  const int Pb = g.Pb, L = g.L;
+  for (int idx = threadIdx.x; idx < K * Pb; idx += blockDim.x) {
+    const int s = idx / Pb, k = idx - s * Pb;
+    const int mslot = ms0 + s;
+    double2 v = make_double2(0.0, 0.0);
+    if (active && mslot < g.nA) {
+      const int m = m_of_slot(g, mslot);
+      const int mp = k <= L - 1 ? k : k - Pb;
+      if (m >= -(L - 1) && mp >= -(L - 1)) {
+        v = X1[((int64_t)nslot * g.nA + mslot) * L + (mp < 0 ? -mp : mp)];
+        if (mp < 0 && ((m + n) & 1)) v = make_double2(-v.x, -v.y);
+        v = cmul(v, phase[k]);
+      }
+    }
+    A[s * ld + k] = v;
+  }
+  __syncthreads();
+  const double2 *R = fft_run<+1>(A, B, K, ld, fd);
+  for (int idx = threadIdx.x; idx < K * g.nB; idx += blockDim.x) {
+    const int b = idx / K, s = idx - b * K;
+    if (ms0 + s < g.nA) X2[((int64_t)nslot * g.nB + b) * g.nA + ms0 + s] = R[s * ld + b];
+  }
+}
+
+// ---------------------------------------------------------------- rows (alpha)
+template <int DIR>
+__global__ __launch_bounds__(kThreads) void k_rows(FftDesc fd, int K, int ld, double2 *__restrict__ X,
+                                                   int64_t nrows, int n, double scale) {
+  extern __shared__ double2 lds[];
+  double2 *A = lds, *B = lds + K * ld;
+  const int64_t r0 = (int64_t)blockIdx.x * K;
+  for (int idx = threadIdx.x; idx < K * n; idx += blockDim.x) {
+    const int s = idx / n, k = idx - s * n;
+    A[s * ld + k] = (r0 + s < nrows) ? X[(r0 + s) * n + k] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  const double2 *R = fft_run<DIR>(A, B, K, ld, fd);
+  for (int idx = threadIdx.x; idx < K * n; idx += blockDim.x) {
+    const int s = idx / n, k = idx - s * n;
+    if (r0 + s < nrows) X[(r0 + s) * n + k] = cscale(R[s * ld + k], scale);
+  }
+}
+
+// ---------------------------------------------------------------- inverse gamma
+__global__ __launch_bounds__(kThreads) void k_inv_gamma(Geo g, FftDesc fd, int K, int ld,
+                                                        const double2 *__restrict__ X2,
+                                                        double2 *__restrict__ fc,
+                                                        double *__restrict__ fr) {
+  extern __shared__ double2 lds[];
+  double2 *A = lds, *B = lds + K * ld;
+  const int64_t S = (int64_t)g.nB * g.nA;
+  const int64_t p0 = (int64_t)blockIdx.x * K;
+  const int nG = 2 * g.N - 1;
+  for (int idx = threadIdx.x; idx < K * nG; idx += blockDim.x) {
+    const int k = idx / K, s = idx - k * K;
+    const int64_t p = p0 + s;
+    double2 v = make_double2(0.0, 0.0);
+    if (p < S) {
+      if (!g.real) {
+        v = X2[k * S + p];
+      } else if (k == 0) {
+        v = make_double2(X2[p].x, 0.0);
+      } else if (k < g.N) {
+        v = X2[k * S + p];
+      } else {
+        v = cconj(X2[(nG - k) * S + p]);
+      }
+    }
+    A[s * ld + k] = v;
+  }
+  __syncthreads();
+  const double2 *R = fft_run<+1>(A, B, K, ld, fd);
+  for (int idx = threadIdx.x; idx < K * nG; idx += blockDim.x) {
+    const int k = idx / K, s = idx - k * K;
+    const int64_t p = p0 + s;
+    if (p >= S) continue;
+    if (g.real)
+      fr[k * S + p] = R[s * ld + k].x;
+    else
+      fc[k * S + p] = R[s * ld + k];
+  }
+}
+
+// ---------------------------------------------------------------- forward gamma
+__global__ __launch_bounds__(kThreads) void k_fwd_gamma(Geo g, FftDesc fd, int K, int ld,
+                                                        const double2 *__restrict__ fc,
+                                                        const double *__restrict__ fr,
+                                                        double2 *__restrict__ X2) {
+  extern __shared__ double2 lds[];
+  double2 *A = lds, *B = lds + K * ld;
+  const int64_t S = (int64_t)g.nB * g.nA;
+  const int64_t p0 = (int64_t)blockIdx.x * K;
+  const int nG = 2 * g.N - 1;
+  for (int idx = threadIdx.x; idx < K * nG; idx += blockDim.x) {
+    const int k = idx / K, s = idx - k * K;
+    const int64_t p = p0 + s;
+    double2 v = make_double2(0.0, 0.0);
+    if (p < S) v = g.real ? make_double2(fr[k * S + p], 0.0) : fc[k * S + p];
+    A[s * ld + k] = v;
+  }
+  __syncthreads();
+  const double2 *R = fft_run<-1>(A, B, K, ld, fd);
+  const double sc = kTwoPi / (double)nG;
+  const int nout = g.real ? g.N : nG;
+  for (int idx = threadIdx.x; idx < K * nout; idx += blockDim.x) {
+    const int k = idx / K, s = idx - k * K;
+    const int64_t p = p0 + s;
+    if (p < S) X2[k * S + p] = cscale(R[s * ld + k], sc);
+  }
+}
+
+// ---------------------------------------------------------------- forward beta
+__global__ __launch_bounds__(kThreads) void k_fwd_beta(Geo g, FftDesc fb, FftDesc fw, int K, int ld,
+                                                       const double2 *__restrict__ X2,
+                                                       double2 *__restrict__ Gp,
+                                                       const int64_t *__restrict__ nbase,
+                                                       const double2 *__restrict__ phase,
+                                                       const double2 *__restrict__ wr, int Pw) {
+  extern __shared__ double2 lds[];
+  double2 *A = lds, *B = lds + K * ld;
+  const int chunks = (g.nA + K - 1) / K;
+  const int nslot = blockIdx.x / chunks;
+  const int ms0 = (blockIdx.x % chunks) * K;
+  const int n = n_of_slot(g, nslot);
+  if (nbase[n + g.N - 1] == kNoBlock) return;  // block not transformed
+  const int Pb = g.Pb, L = g.L;
+  for (int idx = threadIdx.x; idx < K * Pb; idx += blockDim.x) {
+    const int k = idx / K, s = idx - k * K;
+    const int mslot = ms0 + s;
+    double2 v = make_double2(0.0, 0.0);
+    if (mslot < g.nA) {
+      const int m = m_of_slot(g, mslot);
+      int b = k;
+      bool neg = false;
+      if (!g.ss) {
+        if (b >= L) { b = 2 * L - 2 - b; neg = ((m + n) & 1); }
+      } else {
+        if (b > L) { b = 2 * L - b; neg = ((m + n) & 1); }
+      }
+      v = X2[((int64_t)nslot * g.nB + b) * g.nA + mslot];
+      if (neg) v = make_double2(-v.x, -v.y);
+    }
+    A[s * ld + k] = v;
+  }
+  __syncthreads();
+  double2 *R = fft_run<-1>(A, B, K, ld, fb);
+  double2 *O = (R == A) ? B : A;
+  // F~(m'') = DFT/Pb * phase, placed at m'' mod Pw (zero elsewhere)
+  for (int idx = threadIdx.x; idx < K * Pw; idx += blockDim.x) {
+    const int s = idx / Pw, r = idx - s * Pw;
+    const int mpp = r <= L - 1 ? r : r - Pw;
+    double2 v = make_double2(0.0, 0.0);
+    if (mpp >= -(L - 1)) v = cmul(R[s * ld + (mpp < 0 ? mpp + Pb : mpp)], phase[mpp + L - 1]);
+    O[s * ld + r] = v;
+  }
+  __syncthreads();
+  double2 *R2 = fft_run<+1>(O, R, K, ld, fw);
+  for (int idx = threadIdx.x; idx < K * Pw; idx += blockDim.x) {
+    const int s = idx / Pw, r = idx - s * Pw;
+    R2[s * ld + r] = cmul(R2[s * ld + r], wr[r]);
+  }
+  __syncthreads();
+  double2 *R3 = fft_run<-1>(R2, (R2 == O) ? R : O, K, ld, fw);
+  const double sc = kTwoPi / (double)Pw;
+  const int C = 2 * g.NS;
+  const int an = n < 0 ? -n : n;
+  for (int idx = threadIdx.x; idx < K * L; idx += blockDim.x) {
+    const int s = idx / L, mp = idx - s * L;
+    const int mslot = ms0 + s;
+    if (mslot >= g.nA) continue;
+    const int m = m_of_slot(g, mslot);
+    if (m < -(L - 1)) continue;  // MW_SS Nyquist column
+    double2 G = cscale(R3[s * ld + mp], sc);
+    if (mp > 0) {
+      double2 Gn = cscale(R3[s * ld + (Pw - mp)], sc);
+      if ((m + n) & 1) G = csub(G, Gn);
+      else G = cadd(G, Gn);
+    }
+    const int am = m < 0 ? -m : m;
+    const int sm = m < 0, sn = n < 0;
+    double2 *dst = Gp + (((int64_t)an * L + am) * L + mp) * C;
+    // m = 0 or n = 0 serve both sign combinations
+    for (int cm = 0; cm < 2; ++cm) {
+      if (cm != sm && am != 0) continue;
+      for (int cn = 0; cn < g.NS; ++cn) {
+        if (cn != sn && an != 0) continue;
+        // mixed-sign pairs carry D^l_{m',-|m|} = (-1)^{l+m'} D^l_{m',|m|}: the
+        // (-1)^{m'} part is folded here, (-1)^l in the contraction
+        dst[cm + 2 * cn] = ((cm != cn) && (mp & 1)) ? make_double2(-G.x, -G.y) : G;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- forward zero fill
+// via SSHT (direct == 0): active blocks get zeros for l < max(L0,|n|)
+//   (padded: from l = 0; compact: from l = |n|), as the per-n SSHT output.
+// direct (direct == 1): every stored (l >= |n|) entry the contraction does not
+//   write is zero (so3_core.c:1548-1553).
+__global__ void k_fwd_zero(Geo g, double2 *__restrict__ flmn, const int64_t *__restrict__ allbase,
+                           const int64_t *__restrict__ nbase, int direct, int compact) {
+  const int nidx = blockIdx.y;  // n + N - 1 (complex) or n (real)
+  const int n = g.real ? nidx : nidx - (g.N - 1);
+  const int an = n < 0 ? -n : n;
+  const int l = blockIdx.x;
+  if (l >= g.L) return;
+  const bool active = nbase[n + g.N - 1] != kNoBlock;
+  const int64_t base = allbase[n + g.N - 1];
+  bool zero;
+  if (!direct) {
+    if (!active) return;
+    const int l0e = max(g.L0, an);
+    if (compact && l < an) return;
+    zero = l < l0e;
+  } else {
+    if (l < an) return;
+    zero = !active || l < g.L0 || (g.nmodeL && l != an);
+  }
+  if (!zero) return;
+  for (int m = -l + (int)threadIdx.x; m <= l; m += blockDim.x)
+    flmn[base + (int64_t)l * l + l + m] = make_double2(0.0, 0.0);
+}
+
+int choose_k(int ld, int want) {
+  const int per = 2 * ld * (int)sizeof(double2);
+  int k = lds_limit() / per;
+  if (k < 1) throw HipError("FFT length " + std::to_string(ld) + " exceeds the LDS budget");
+  return std::min(k, want);
+}
+
+}  // namespace
+
+int lds_limit() {
+  static int lim = [] {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    hipDeviceProp_t pr;
+    if (hipGetDeviceProperties(&pr, dev) != hipSuccess || pr.sharedMemPerBlock == 0) return 65536;
+    return (int)pr.sharedMemPerBlock;
+  }();
+  return lim;
+}
+
+template <typename Kern>
+static void set_lds(Kern k, size_t bytes) {
+  if (bytes > 65536) SO3_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
+void launch_inv_beta(const Geo &g, const double2 *X1, double2 *X2, NBase nb, const double2 *phase,
+                     hipStream_t st) {
+  const FftDesc fd = fft_plan(g.Pb);
+  const int ld = fft_ld(fd);
+  const int K = choose_k(ld, 8);
+  const size_t sh = 2 * (size_t)K * ld * sizeof(double2);
+  set_lds(k_inv_beta, sh);
+  const int chunks = (g.nA + K - 1) / K;
+  k_inv_beta<<<g.nG * chunks, kThreads, sh, st>>>(g, fd, K, ld, X1, X2, nb.base, phase);
+  SO3_LAUNCH_CHECK();
+}
+
+void launch_alpha(const Geo &g, double2 *X2, int dir, hipStream_t st) {
+  const FftDesc fd = fft_plan(g.nA);
+  const int ld = fft_ld(fd);
+  const int K = choose_k(ld, 8);
+  const size_t sh = 2 * (size_t)K * ld * sizeof(double2);
+  const int64_t nrows = (int64_t)g.nG * g.nB;
+  const unsigned grid = (unsigned)((nrows + K - 1) / K);
+  if (dir > 0) {
+    set_lds(k_rows<+1>, sh);
+    k_rows<+1><<<grid, kThreads, sh, st>>>(fd, K, ld, X2, nrows, g.nA, 1.0);
+  } else {
+    set_lds(k_rows<-1>, sh);
+    k_rows<-1><<<grid, kThreads, sh, st>>>(fd, K, ld, X2, nrows, g.nA, 1.0 / g.nA);
+  }
+  SO3_LAUNCH_CHECK();
+}
+
+void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, hipStream_t st) {
+  const FftDesc fd = fft_plan(2 * g.N - 1);
+  const int ld = fft_ld(fd);
+  const int K = choose_k(ld, 16);
+  const size_t sh = 2 * (size_t)K * ld * sizeof(double2);
+  set_lds(k_inv_gamma, sh);
+  const int64_t S = (int64_t)g.nB * g.nA;
+  k_inv_gamma<<<(unsigned)((S + K - 1) / K), kThreads, sh, st>>>(
+      g, fd, K, ld, X2, g.real ? nullptr : (double2 *)f, g.real ? (double *)f : nullptr);
+  SO3_LAUNCH_CHECK();
+}
+
+void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, hipStream_t st) {
+  const FftDesc fd = fft_plan(2 * g.N - 1);
+  const int ld = fft_ld(fd);
+  const int K = choose_k(ld, 16);
+  const size_t sh = 2 * (size_t)K * ld * sizeof(double2);
+  set_lds(k_fwd_gamma, sh);
+  const int64_t S = (int64_t)g.nB * g.nA;
+  k_fwd_gamma<<<(unsigned)((S + K - 1) / K), kThreads, sh, st>>>(
+      g, fd, K, ld, g.real ? nullptr : (const double2 *)f, g.real ? (const double *)f : nullptr, X2);
+  SO3_LAUNCH_CHECK();
+}
+
+void launch_fwd_beta(const Geo &g, const double2 *X2, double2 *Gp, NBase nb, const double2 *phase,
+                     const double2 *wr, int Pw, hipStream_t st) {
+  const FftDesc fb = fft_plan(g.Pb);
+  const FftDesc fw = fft_plan(Pw);
+  const int ld = std::max(fft_ld(fb), fft_ld(fw));
+  const int K = choose_k(ld, 8);
+  const size_t sh = 2 * (size_t)K * ld * sizeof(double2);
+  set_lds(k_fwd_beta, sh);
+  const int chunks = (g.nA + K - 1) / K;
+  k_fwd_beta<<<g.nG * chunks, kThreads, sh, st>>>(g, fb, fw, K, ld, X2, Gp, nb.base, phase, wr, Pw);
+  SO3_LAUNCH_CHECK();
+}
+
+void launch_fwd_zero(const Geo &g, double2 *flmn, NBase nb, int direct, int64_t /*flmn_size*/,
+                     int storage, int /*order*/, hipStream_t st) {
+  (void)storage;
+  // nb.base is the activity-masked table; the unmasked one follows it
+  const int nn = g.real ? g.N : 2 * g.N - 1;
+  dim3 grid(g.L, nn);
+  k_fwd_zero<<<grid, 64, 0, st>>>(g, flmn, nb.base + (2 * g.N - 1), nb.base, direct,
+                                  storage == 1);
+  SO3_LAUNCH_CHECK();
+}
+
+}  // namespace so3g

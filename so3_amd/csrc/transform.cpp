@@ -1,0 +1,371 @@
+// transform.cpp -- plans, workspace and the stage sequence of the inverse
+// and forward transforms.
+//
+// Stage order (reference so3_core.c, via-SSHT and direct forms):
+//   inverse: contraction (l) -> beta DFT (m') -> alpha DFT (m) -> gamma DFT (n)
+//   forward: gamma DFT (g) -> alpha DFT (a) -> beta extension, DFT and weights
+//            -> contraction (m')
+// The Delta tables and DFT plans are cached per device; one workspace arena
+// per device serves every transform (calls on the plain path are
+// serialised).
+#include "transform.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "fft_plan.hpp"
+#include "so3/so3_sampling.h"
+
+namespace so3g {
+
+namespace {
+
+using cld = std::complex<long double>;
+const long double kPiL = 3.141592653589793238462643383279502884L;
+
+struct Plan {
+  Geo g{};
+  int Pw = 0;
+  double2 *phase_inv = nullptr;  // Pb entries
+  double2 *phase_fwd = nullptr;  // 2L-1 entries, includes 1/Pb
+  double2 *wr = nullptr;         // Pw entries
+  int64_t *nbase = nullptr;      // 2*(2N-1): masked, then unmasked
+  size_t x1 = 0, x2 = 0;         // workspace sizes (double2)
+};
+
+int smooth_at_least(int n) {
+  for (int m = n;; ++m) {
+    int k = m;
+    for (int p : {2, 3, 5, 7}) while (k % p == 0) k /= p;
+    if (k == 1) return m;
+  }
+}
+
+double2 *upload(const std::vector<double2> &h) {
+  double2 *d = nullptr;
+  SO3_HIP(hipMalloc(&d, sizeof(double2) * std::max<size_t>(1, h.size())));
+  SO3_HIP(hipMemcpy(d, h.data(), sizeof(double2) * h.size(), hipMemcpyHostToDevice));
+  return d;
+}
+
+bool n_active_ssht(int n, int N, int mode) {
+  if (mode == SO3_N_MODE_EVEN && (n % 2)) return false;
+  if (mode == SO3_N_MODE_ODD && !(n % 2)) return false;
+  if (mode == SO3_N_MODE_MAXIMUM && std::abs(n) < N - 1) return false;
+  return true;
+}
+
+Geo make_geo(const so3_parameters_t *p, Method method) {
+  Geo g{};
+  g.L = p->L;
+  g.N = p->N;
+  g.L0 = p->L0;
+  g.ss = (method == kViaSsht && p->sampling_scheme == SO3_SAMPLING_MW_SS) ? 1 : 0;
+  g.real = p->reality ? 1 : 0;
+  g.nA = g.ss ? 2 * g.L : 2 * g.L - 1;
+  g.nB = g.ss ? g.L + 1 : g.L;
+  g.nG = g.real ? g.N : 2 * g.N - 1;
+  g.Pb = g.ss ? 2 * g.L : 2 * g.L - 1;
+  g.NS = g.real ? 1 : 2;
+  g.nmodeL = (method == kDirect && p->n_mode == SO3_N_MODE_L) ? 1 : 0;
+  return g;
+}
+
+using Key = std::tuple<int, int, int, int, int, int, int, int, int, int>;
+
+Key key_of(const so3_parameters_t *p, Method method, int dev) {
+  const int ss = (method == kViaSsht && p->sampling_scheme == SO3_SAMPLING_MW_SS);
+  return Key(dev, p->L, p->N, p->L0, ss, p->reality ? 1 : 0, (int)p->storage, (int)p->n_order,
+             (int)p->n_mode, (int)method);
+}
+
+Plan build_plan(const so3_parameters_t *p, Method method) {
+  Plan P;
+  Geo &g = P.g;
+  g = make_geo(p, method);
+  const int L = g.L, N = g.N;
+  // inverse phase exp(i m' pi/(2L-1)) at k (m' = k or k - Pb); MW_SS none
+  std::vector<double2> ph(g.Pb);
+  for (int k = 0; k < g.Pb; ++k) {
+    const int mp = k <= L - 1 ? k : k - g.Pb;
+    if (g.ss) {
+      ph[k] = make_double2(1.0, 0.0);
+    } else {
+      const long double a = kPiL * mp / (2.0L * L - 1.0L);
+      ph[k] = make_double2((double)cosl(a), (double)sinl(a));
+    }
+  }
+  P.phase_inv = upload(ph);
+  // forward: exp(-i m'' pi/(2L-1)) / Pb for m'' in [-(L-1), L-1]
+  std::vector<double2> pf(2 * L - 1);
+  for (int mpp = -(L - 1); mpp <= L - 1; ++mpp) {
+    if (g.ss) {
+      pf[mpp + L - 1] = make_double2(1.0 / g.Pb, 0.0);
+    } else {
+      const long double a = -kPiL * mpp / (2.0L * L - 1.0L);
+      pf[mpp + L - 1] = make_double2((double)(cosl(a) / g.Pb), (double)(sinl(a) / g.Pb));
+    }
+  }
+  P.phase_fwd = upload(pf);
+  // weights w(p) = int_0^pi exp(-i p b) sin b db (so3_sampling_weight) at p mod Pw,
+  // backward DFT of length Pw >= 4L-3 (any such length is exact)
+  P.Pw = smooth_at_least(4 * L - 3);
+  {
+    const int Pw = P.Pw;
+    std::vector<cld> w(Pw, cld(0, 0));
+    for (int q = -2 * (L - 1); q <= 2 * (L - 1); ++q) {
+      cld v(0, 0);
+      if (q == 1) v = cld(0, -kPiL / 2);
+      else if (q == -1) v = cld(0, kPiL / 2);
+      else if (q % 2 == 0) v = cld(2.0L / (1.0L - (long double)q * q), 0);
+      w[((q % Pw) + Pw) % Pw] = v;
+    }
+    // wr[r] = sum_q w[q] exp(+2 pi i q r / Pw); w has 2L-1 non-zero even taps + 2
+    std::vector<double2> wr(Pw);
+    for (int r = 0; r < Pw; ++r) {
+      cld acc(0, 0);
+      for (int q = -2 * (L - 1); q <= 2 * (L - 1); ++q) {
+        const cld wq = w[((q % Pw) + Pw) % Pw];
+        if (wq == cld(0, 0)) continue;
+        long long k = ((long long)q * r) % Pw;
+        if (k < 0) k += Pw;
+        const long double a = 2.0L * kPiL * k / Pw;
+        acc += wq * cld(cosl(a), sinl(a));
+      }
+      wr[r] = make_double2((double)acc.real(), (double)acc.imag());
+    }
+    P.wr = upload(wr);
+  }
+  // per-n storage bases
+  std::vector<int64_t> nb(2 * (2 * N - 1), kNoBlock);
+  for (int n = -(N - 1); n <= N - 1; ++n) {
+    if (g.real && n < 0) continue;
+    const int an = std::abs(n);
+    int ind = 0;
+    if (an < L) {
+      if (g.real) so3_sampling_elmn2ind_real(&ind, an, -an, n, p);
+      else so3_sampling_elmn2ind(&ind, an, -an, n, p);
+    } else {
+      // |n| >= L: block holds nothing (only padded storage reaches here)
+      if (g.real) so3_sampling_elmn2ind_real(&ind, 0, 0, n, p);
+      else so3_sampling_elmn2ind(&ind, 0, 0, n, p);
+      ind += an * an;
+    }
+    const int64_t base = (int64_t)ind - (int64_t)an * an;
+    nb[(2 * N - 1) + n + N - 1] = base;
+    bool act = n_active_ssht(n, N, p->n_mode);
+    if (an >= L) act = false;
+    nb[n + N - 1] = act ? base : kNoBlock;
+  }
+  {
+    int64_t *d = nullptr;
+    SO3_HIP(hipMalloc(&d, sizeof(int64_t) * nb.size()));
+    SO3_HIP(hipMemcpy(d, nb.data(), sizeof(int64_t) * nb.size(), hipMemcpyHostToDevice));
+    P.nbase = d;
+  }
+  P.x1 = std::max<size_t>((size_t)g.nG * g.nA * L, (size_t)N * L * L * 2 * g.NS);
+  P.x2 = (size_t)g.nG * g.nB * g.nA;
+  // DFT plans are built now so that a timed call does not build them
+  fft_plan(g.Pb);
+  fft_plan(g.nA);
+  fft_plan(2 * N - 1);
+  fft_plan(P.Pw);
+  return P;
+}
+
+std::mutex g_mu;
+std::map<Key, Plan> g_plans;
+
+// ---- stage profiler
+bool g_timing = false;
+struct Pending {
+  int stage;
+  hipEvent_t a, b;
+};
+std::vector<Pending> g_pending;
+const char *kStageNames[] = {"contract_inv", "beta_inv", "alpha_inv", "gamma_inv",
+                             "gamma_fwd",    "alpha_fwd", "beta_fwd", "zero_fwd",
+                             "contract_fwd"};
+constexpr int kNStages = 9;
+double g_ms[kNStages];
+int g_cnt[kNStages];
+
+template <typename F>
+void timed(int stage, hipStream_t st, F &&launch) {
+  if (!g_timing) {
+    launch();
+    return;
+  }
+  Pending p{stage, nullptr, nullptr};
+  SO3_HIP(hipEventCreate(&p.a));
+  SO3_HIP(hipEventCreate(&p.b));
+  SO3_HIP(hipEventRecord(p.a, st));
+  launch();
+  SO3_HIP(hipEventRecord(p.b, st));
+  g_pending.push_back(p);
+}
+
+void resolve_pending() {
+  for (auto &p : g_pending) {
+    SO3_HIP(hipEventSynchronize(p.b));
+    float ms = 0.f;
+    SO3_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+    g_ms[p.stage] += ms;
+    g_cnt[p.stage] += 1;
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  g_pending.clear();
+}
+
+struct Arena {
+  double2 *buf = nullptr;
+  size_t cap = 0;  // double2 elements
+};
+std::map<int, Arena> g_arena;
+
+const Plan &get_plan(const so3_parameters_t *p, Method method) {
+  int dev = 0;
+  SO3_HIP(hipGetDevice(&dev));
+  const Key k = key_of(p, method, dev);
+  auto it = g_plans.find(k);
+  if (it != g_plans.end()) return it->second;
+  return g_plans.emplace(k, build_plan(p, method)).first->second;
+}
+
+double2 *arena(size_t elems) {
+  int dev = 0;
+  SO3_HIP(hipGetDevice(&dev));
+  Arena &a = g_arena[dev];
+  if (a.cap < elems) {
+    if (a.buf) {
+      SO3_HIP(hipDeviceSynchronize());
+      SO3_HIP(hipFree(a.buf));
+    }
+    a.buf = nullptr;
+    a.cap = 0;
+    SO3_HIP(hipMalloc(&a.buf, sizeof(double2) * elems));
+    a.cap = elems;
+  }
+  return a.buf;
+}
+
+}  // namespace
+
+void validate(const so3_parameters_t *p, Method method) {
+  if (!p) throw HipError("parameters pointer is NULL");
+  if (p->L < 1) throw HipError("L must be positive");
+  if (p->N < 1) throw HipError("N must be positive");
+  if (p->L0 < 0 || p->L0 >= p->L) throw HipError("L0 must be in [0, L)");
+  if (p->sampling_scheme != SO3_SAMPLING_MW && p->sampling_scheme != SO3_SAMPLING_MW_SS)
+    throw HipError("Invalid sampling scheme.");
+  if (p->storage != SO3_STORAGE_PADDED && p->storage != SO3_STORAGE_COMPACT)
+    throw HipError("Invalid storage method.");
+  if (p->n_order != SO3_N_ORDER_ZERO_FIRST && p->n_order != SO3_N_ORDER_NEGATIVE_FIRST)
+    throw HipError("Invalid n-order.");
+  if ((int)p->n_mode < 0 || (int)p->n_mode >= (int)SO3_N_MODE_SIZE) throw HipError("Invalid n-mode.");
+  if ((int)p->dl_method != 0 && (int)p->dl_method != 1) throw HipError("Invalid dl method");
+  if (p->storage == SO3_STORAGE_COMPACT && p->N > p->L)
+    throw HipError("compact storage needs N <= L");
+  if (method == kViaSsht && p->steerable)
+    throw HipError("steerable signals are not supported by this build");
+  if ((size_t)(2 * p->N - 1) * (2 * p->L) * (p->L + 1) > (size_t)1 << 40)
+    throw HipError("transform too large");
+}
+
+bool forward_writes_all(const so3_parameters_t *p, Method method) {
+  if (method == kViaSsht) return p->n_mode == SO3_N_MODE_ALL || p->n_mode == SO3_N_MODE_L;
+  return p->storage == SO3_STORAGE_COMPACT;  // direct leaves padded l < |n| alone
+}
+
+size_t workspace_bytes(const so3_parameters_t *p, Method method) {
+  validate(p, method);
+  std::lock_guard<std::mutex> lk(g_mu);
+  const Plan &P = get_plan(p, method);
+  return sizeof(double2) * (P.x1 + P.x2);
+}
+
+void prepare(const so3_parameters_t *p, Method method) {
+  validate(p, method);
+  std::lock_guard<std::mutex> lk(g_mu);
+  get_plan(p, method);
+  dl_tables(p->L, (int)p->dl_method);
+}
+
+void run_inverse(const so3_parameters_t *p, Method method, const void *flmn, void *f, hipStream_t st) {
+  validate(p, method);
+  std::lock_guard<std::mutex> lk(g_mu);
+  const Plan &P = get_plan(p, method);
+  const DlTables &dl = dl_tables(p->L, (int)p->dl_method);
+  double2 *ws = arena(P.x1 + P.x2);
+  double2 *X1 = ws, *X2 = ws + P.x1;
+  const Geo &g = P.g;
+  NBase nb{P.nbase};
+  timed(0, st, [&] { launch_contract_inv(g, (const double2 *)flmn, nb, dl, X1, st); });
+  timed(1, st, [&] { launch_inv_beta(g, X1, X2, nb, P.phase_inv, st); });
+  timed(2, st, [&] { launch_alpha(g, X2, +1, st); });
+  timed(3, st, [&] { launch_inv_gamma(g, X2, f, st); });
+}
+
+void run_forward(const so3_parameters_t *p, Method method, const void *f, void *flmn, hipStream_t st) {
+  validate(p, method);
+  std::lock_guard<std::mutex> lk(g_mu);
+  const Plan &P = get_plan(p, method);
+  const DlTables &dl = dl_tables(p->L, (int)p->dl_method);
+  double2 *ws = arena(P.x1 + P.x2);
+  double2 *Gp = ws, *X2 = ws + P.x1;
+  const Geo &g = P.g;
+  NBase nb{P.nbase};
+  timed(4, st, [&] { launch_fwd_gamma(g, f, X2, st); });
+  timed(5, st, [&] { launch_alpha(g, X2, -1, st); });
+  timed(6, st, [&] { launch_fwd_beta(g, X2, Gp, nb, P.phase_fwd, P.wr, P.Pw, st); });
+  timed(7, st, [&] {
+    launch_fwd_zero(g, (double2 *)flmn, nb, method == kDirect, 0, (int)p->storage, (int)p->n_order, st);
+  });
+  timed(8, st, [&] { launch_contract_fwd(g, Gp, nb, dl, (double2 *)flmn, st); });
+}
+
+void set_stage_timing(bool on) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_timing = on;
+}
+
+void reset_stage_times() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  resolve_pending();
+  for (int i = 0; i < kNStages; ++i) g_ms[i] = 0.0, g_cnt[i] = 0;
+}
+
+int stage_times(const char **names, double *ms, int *count, int cap) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  resolve_pending();
+  const int n = std::min(cap, kNStages);
+  for (int i = 0; i < n; ++i) {
+    if (names) names[i] = kStageNames[i];
+    if (ms) ms[i] = g_ms[i];
+    if (count) count[i] = g_cnt[i];
+  }
+  return kNStages;
+}
+
+void release_all() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  (void)hipDeviceSynchronize();
+  for (auto &kv : g_plans) {
+    (void)hipFree(kv.second.phase_inv);
+    (void)hipFree(kv.second.phase_fwd);
+    (void)hipFree(kv.second.wr);
+    (void)hipFree(kv.second.nbase);
+  }
+  g_plans.clear();
+  for (auto &kv : g_arena) (void)hipFree(kv.second.buf);
+  g_arena.clear();
+  dl_tables_clear();
+}
+
+}  // namespace so3g

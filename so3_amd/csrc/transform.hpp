@@ -1,0 +1,32 @@
+// transform.hpp -- host orchestration of one Wigner transform on the GPU.
+#pragma once
+#include <cstddef>
+
+#include "kernels.hpp"
+#include "so3/so3_types.h"
+
+namespace so3g {
+
+enum Method { kViaSsht = 0, kDirect = 1 };
+
+// Throws HipError on invalid parameters or device failures.
+void validate(const so3_parameters_t *p, Method method);
+void run_inverse(const so3_parameters_t *p, Method method, const void *flmn_dev, void *f_dev,
+                 hipStream_t st);
+void run_forward(const so3_parameters_t *p, Method method, const void *f_dev, void *flmn_dev,
+                 hipStream_t st);
+void prepare(const so3_parameters_t *p, Method method);
+size_t workspace_bytes(const so3_parameters_t *p, Method method);
+void release_all();
+
+// Stage profiler: when enabled, every stage launch is bracketed by HIP
+// events on its stream; stage_times() resolves them (synchronising) and
+// returns accumulated milliseconds and launch counts per stage name.
+void set_stage_timing(bool on);
+int stage_times(const char **names, double *ms, int *count, int cap);
+void reset_stage_times();
+
+// true when every stored flmn entry is written by run_forward
+bool forward_writes_all(const so3_parameters_t *p, Method method);
+
+}  // namespace so3g

@@ -1,0 +1,46 @@
+"""The library's batched DFT kernel (so3x_dft) against numpy.fft."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import so3_amd.api as A
+from so3_amd._lib import lib
+
+pytestmark = pytest.mark.gpu
+
+LENGTHS = [1, 2, 3, 4, 5, 7, 8, 9, 11, 13, 15, 16, 17, 31, 61, 64, 73, 127, 255, 511, 512,
+           767, 1021, 1023, 1024, 1537, 2047, 4096]
+
+
+def _hip():
+    h = ctypes.CDLL("libamdhip64.so")
+    h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    h.hipFree.argtypes = [ctypes.c_void_p]
+    h.hipDeviceSynchronize.argtypes = []
+    return h
+
+
+@pytest.mark.parametrize("n", LENGTHS)
+@pytest.mark.parametrize("direction", [-1, 1])
+def test_dft(n, direction):
+    lib()
+    h = _hip()
+    rng = np.random.default_rng(n)
+    howmany = 5
+    x = rng.standard_normal((howmany, n)) + 1j * rng.standard_normal((howmany, n))
+    x = np.ascontiguousarray(x)
+    d = ctypes.c_void_p()
+    assert h.hipMalloc(ctypes.byref(d), x.nbytes) == 0
+    try:
+        assert h.hipMemcpy(d, x.ctypes.data, x.nbytes, 1) == 0
+        A.dft_device(d.value, n, howmany, direction)
+        assert h.hipDeviceSynchronize() == 0
+        y = np.empty_like(x)
+        assert h.hipMemcpy(y.ctypes.data, d, x.nbytes, 2) == 0
+    finally:
+        h.hipFree(d)
+    ref = np.fft.fft(x, axis=1) if direction < 0 else np.fft.ifft(x, axis=1) * n
+    err = np.abs(y - ref).max() / np.abs(ref).max()
+    assert err < 5e-15 * max(1.0, np.log2(n)), err
