@@ -9,11 +9,15 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <string>
 
+#include "conv.hpp"
 #include "fft_plan.hpp"
+#include "so3/so3_adjoint.h"
+#include "so3/so3_conv.h"
 #include "so3/so3_core.h"
 #include "so3/so3_sampling.h"
 #include "so3/so3x.h"
@@ -119,6 +123,35 @@ void host_forward(const char *fn, void *flmn, const void *f, const so3_parameter
   }
 }
 
+// a device buffer that is either the caller's device pointer or a staged copy
+struct DevBuf {
+  void *d = nullptr;
+  bool owned = false;
+  DevBuf(const void *host, size_t bytes, bool copy_in) {
+    if (is_device_ptr(host)) {
+      d = const_cast<void *>(host);
+      return;
+    }
+    SO3_HIP(hipMalloc(&d, bytes ? bytes : 16));
+    owned = true;
+    if (copy_in && bytes) SO3_HIP(hipMemcpy(d, host, bytes, hipMemcpyHostToDevice));
+  }
+  void copy_out(void *host, size_t bytes) {
+    if (owned && bytes) SO3_HIP(hipMemcpy(host, d, bytes, hipMemcpyDeviceToHost));
+  }
+  ~DevBuf() {
+    if (owned) (void)hipFree(d);
+  }
+};
+
+void check_conv_params(const so3_parameters_t *p) {
+  if (p->storage != SO3_STORAGE_PADDED && p->storage != SO3_STORAGE_COMPACT)
+    throw HipError("Invalid storage method.");
+  if (p->n_order != SO3_N_ORDER_ZERO_FIRST && p->n_order != SO3_N_ORDER_NEGATIVE_FIRST)
+    throw HipError("Invalid n-order.");
+  if ((int)p->n_mode < 0 || (int)p->n_mode >= (int)SO3_N_MODE_SIZE) throw HipError("Invalid n-mode.");
+}
+
 int status(const std::exception &e) {
   t_err = e.what();
   return -1;
@@ -173,6 +206,134 @@ void so3_core_forward_direct_real(std::complex<double> *flmn, const double *f,
   so3_parameters_t q = *parameters;
   q.reality = 1;
   host_forward(__func__, flmn, f, &q, so3g::kDirect);
+}
+
+// ---------------------------------------------------------------- adjoints
+static void host_adjoint(const char *fn, void *out, const void *in, const so3_parameters_t *p,
+                         bool of_inverse) {
+  try {
+    so3g::validate(p, so3g::kDirect);
+    const size_t nf = f_bytes(p, so3g::kDirect), nl = flmn_bytes(p);
+    if (of_inverse) {  // f -> flmn
+      DevBuf df(in, nf, true), dl(out, nl, true);
+      so3g::run_adjoint_inverse(p, df.d, dl.d, nullptr);
+      SO3_HIP(hipDeviceSynchronize());
+      dl.copy_out(out, nl);
+    } else {  // flmn -> f
+      DevBuf dl(in, nl, true), df(out, nf, false);
+      so3g::run_adjoint_forward(p, dl.d, df.d, nullptr);
+      SO3_HIP(hipDeviceSynchronize());
+      df.copy_out(out, nf);
+    }
+  } catch (const std::exception &e) {
+    fatal(fn, e.what());
+  }
+}
+
+void so3_adjoint_inverse_direct(std::complex<double> *flmn, const std::complex<double> *f,
+                                const so3_parameters_t *parameters) {
+  so3_parameters_t q = *parameters;
+  q.reality = 0;
+  host_adjoint(__func__, flmn, f, &q, true);
+}
+void so3_adjoint_forward_direct(std::complex<double> *f, const std::complex<double> *flmn,
+                                const so3_parameters_t *parameters) {
+  so3_parameters_t q = *parameters;
+  q.reality = 0;
+  host_adjoint(__func__, f, flmn, &q, false);
+}
+void so3_adjoint_inverse_direct_real(std::complex<double> *flmn, const double *f,
+                                     const so3_parameters_t *parameters) {
+  so3_parameters_t q = *parameters;
+  q.reality = 1;
+  host_adjoint(__func__, flmn, f, &q, true);
+}
+void so3_adjoint_forward_direct_real(double *f, const std::complex<double> *flmn,
+                                     const so3_parameters_t *parameters) {
+  so3_parameters_t q = *parameters;
+  q.reality = 1;
+  host_adjoint(__func__, f, flmn, &q, false);
+}
+
+// ---------------------------------------------------------------- conv
+void so3_conv_s2toso3_harmonic_convolution(std::complex<double> *hlmn, const so3_parameters_t *hp,
+                                           const std::complex<double> *flm,
+                                           const std::complex<double> *glm) {
+  try {
+    check_conv_params(hp);
+    const size_t nl = (size_t)hp->L * hp->L * 16, nh = flmn_bytes(hp);
+    DevBuf f(flm, nl, true), g(glm, nl, true), h(hlmn, nh, false);
+    so3g::s2toso3(hp, f.d, g.d, h.d, nullptr);
+    h.copy_out(hlmn, nh);
+  } catch (const std::exception &e) {
+    fatal(__func__, e.what());
+  }
+}
+
+void so3_conv_harmonic_convolution(std::complex<double> *hlmn, const so3_parameters_t *hp,
+                                   const std::complex<double> *flmn, const so3_parameters_t *fp,
+                                   const std::complex<double> *glmn, const so3_parameters_t *gp) {
+  try {
+    check_conv_params(hp);
+    check_conv_params(fp);
+    check_conv_params(gp);
+    const size_t nh = flmn_bytes(hp), nf = flmn_bytes(fp), ng = flmn_bytes(gp);
+    DevBuf f(flmn, nf, true), g(glmn, ng, true), h(hlmn, nh, true);
+    so3g::harmonic_convolution(hp, fp, gp, f.d, g.d, h.d, nullptr);
+    h.copy_out(hlmn, nh);
+  } catch (const std::exception &e) {
+    fatal(__func__, e.what());
+  }
+}
+
+so3_parameters_t so3_conv_get_parameters_of_convolved_lmn(const so3_parameters_t *fp,
+                                                          const so3_parameters_t *gp) {
+  // so3_conv.c:104-125, including h.N = h.L
+  if (fp->sampling_scheme != gp->sampling_scheme) fatal(__func__, "f and g must have the same sampling_scheme");
+  if (fp->n_order != gp->n_order) fatal(__func__, "f and g must have the same n_order");
+  if (fp->storage != gp->storage) fatal(__func__, "f and g must have the same storage");
+  if (fp->n_mode != gp->n_mode) fatal(__func__, "f and g must have the same n_mode");
+  if (fp->reality != gp->reality) fatal(__func__, "f and g must have the same reality");
+  so3_parameters_t h = *fp;
+  h.L = std::min(fp->L, gp->L);
+  h.N = h.L;
+  h.L0 = std::max(fp->L0, gp->L0);
+  return h;
+}
+
+void so3_conv_get_parameters_of_convolved_lmn_void(so3_parameters_t *hp, const so3_parameters_t *fp,
+                                                   const so3_parameters_t *gp) {
+  *hp = so3_conv_get_parameters_of_convolved_lmn(fp, gp);
+}
+
+void so3_conv_convolution(std::complex<double> *h, const so3_parameters_t *hp,
+                          const std::complex<double> *f, const so3_parameters_t *fp,
+                          const std::complex<double> *g, const so3_parameters_t *gp) {
+  // so3_conv.c:154-184: forward_direct(f), forward_direct(g), harmonic
+  // product, inverse_direct(h), all on the GPU (temporaries are freed).
+  try {
+    so3_parameters_t fq = *fp, gq = *gp, hq = *hp;
+    fq.reality = gq.reality = hq.reality = 0;
+    so3g::validate(&fq, so3g::kDirect);
+    so3g::validate(&gq, so3g::kDirect);
+    so3g::validate(&hq, so3g::kDirect);
+    const size_t ff = f_bytes(&fq, so3g::kDirect), fg = f_bytes(&gq, so3g::kDirect),
+                 fh = f_bytes(&hq, so3g::kDirect);
+    DevBuf df(f, ff, true), dg(g, fg, true), dh(h, fh, false);
+    DevBuf fl(nullptr, flmn_bytes(fp), false), gl(nullptr, flmn_bytes(gp), false),
+        hl(nullptr, flmn_bytes(hp), false);
+    SO3_HIP(hipMemset(fl.d, 0, flmn_bytes(fp)));
+    SO3_HIP(hipMemset(gl.d, 0, flmn_bytes(gp)));
+    SO3_HIP(hipMemset(hl.d, 0, flmn_bytes(hp)));
+    so3g::run_forward(&fq, so3g::kDirect, df.d, fl.d, nullptr);
+    so3g::run_forward(&gq, so3g::kDirect, dg.d, gl.d, nullptr);
+    so3g::harmonic_convolution(hp, fp, gp, fl.d, gl.d, hl.d, nullptr);
+    so3g::run_inverse(&hq, so3g::kDirect, hl.d, dh.d, nullptr);
+    SO3_HIP(hipDeviceSynchronize());
+    dh.copy_out(h, fh);
+  } catch (const std::exception &e) {
+    fatal(__func__, e.what());
+  }
 }
 
 const char *so3x_last_error(void) { return t_err.c_str(); }

@@ -7,19 +7,21 @@
 // Forward (so3_core.c:1589-1617, ssht_core_mw_lb_forward_sov_conv_sym at :410):
 //   f_lmn = i^{m-n} sum_{0 <= m' <= l} D^l_{m'm} D^l_{m'n} G'(m,n,m').
 //
-// Both kernels use one wavefront per workgroup and treat the four sign
-// combinations (+-m, +-n) of a (|m|, |n|) pair together: with
-//   D^l_{m',-m} = (-1)^{l+m'} D^l_{m',m},
-// one product D^l_{m'|m|} D^l_{m'|n|} feeds four complex FMAs.  The factor
-// (-1)^l of the mixed-sign pairs is applied per l (l unrolled by two), the
-// factor (-1)^{m'} once at the end (inverse) or folded into G' by the
-// forward beta stage.
-//   inverse: lanes run over m' (64 per wave), the loop over l accumulates in
-//            registers; f_lmn is wave-uniform and read with scalar loads.
-//   forward: lanes run over l, the loop over m' accumulates in registers;
-//            G' is wave-uniform and read with scalar loads.
-// Delta columns come from the two layouts of dl_tables.hpp so that every
-// vector load is a contiguous 512-byte row.
+// The four sign combinations (+-m, +-n) of a pair (|m|, |n|) are treated
+// together: with D^l_{m',-m} = (-1)^{l+m'} D^l_{m',m}, one product
+// D^l_{m'|m|} D^l_{m'|n|} feeds four complex FMAs.  The factor (-1)^l of the
+// mixed-sign combinations sits in the packed coefficients (inverse) or is
+// applied at the store (forward); (-1)^{m'} is applied at the store
+// (inverse) or folded into G' by the forward beta stage.
+//
+// Work decomposition (both directions): a workgroup of four wavefronts owns
+// a tile of 4 |m| x 4 |n| pairs and one 64-wide block of the lane index
+// (m' for the inverse, l for the forward); each wavefront owns 2 x 2 pairs.
+// Per step of the reduction loop (l for the inverse, m' for the forward)
+// the workgroup stages the 8 Delta columns of the tile (4 |m| + 4 |n|, one
+// 512-byte row each) in LDS, prefetched one step ahead, and every wavefront
+// reads the coefficients of its pairs as one 64-byte scalar load per pair
+// from a pair-contiguous stream, so f_lmn / G' never occupy vector registers.
 #include "kernels.hpp"
 
 namespace so3g {
@@ -27,103 +29,131 @@ namespace so3g {
 namespace {
 
 typedef const __attribute__((address_space(4))) double cdouble;
-typedef const __attribute__((address_space(4))) int64_t cint64;
 
 constexpr double kInv8Pi2 = 0.012665147955292221430953884060716700591;  // 1/(8 pi^2)
+constexpr int kTile = 4;     // |m| and |n| per workgroup tile
+constexpr int kWaves = 4;    // wavefronts per workgroup (2 x 2 pairs each)
+constexpr int kCols = 2 * kTile;
 
-template <int A, int B, int NS, bool ODD>
-__device__ __forceinline__ void inv_step(int l, const Geo &g, const double *__restrict__ Dinv,
-                                         int64_t offl, int mp, int m0, int n0, cdouble *fl,
-                                         const int64_t (&bas)[B][NS], double2 (&acc)[A][B][2 * NS]) {
-  constexpr int C = 2 * NS;
-  const int rowlen = (l + 64) & ~63;
-  const double *Dl = Dinv + offl + mp;
-  double dm[A], dn[B];
-#pragma unroll
-  for (int i = 0; i < A; ++i) dm[i] = Dl[(int64_t)min(m0 + i, l) * rowlen];
-  const double cl = (2.0 * l + 1.0) * kInv8Pi2;
-#pragma unroll
-  for (int j = 0; j < B; ++j) dn[j] = cl * Dl[(int64_t)min(n0 + j, l) * rowlen];
-  const int64_t ll = (int64_t)l * l + l;
-#pragma unroll
-  for (int i = 0; i < A; ++i) {
-    const int mi = m0 + i;
-    if (mi > l) continue;
-#pragma unroll
-    for (int j = 0; j < B; ++j) {
-      const int nj = n0 + j;
-      if (nj > l || nj >= g.N) continue;
-      if (g.nmodeL && nj != l) continue;
-      const double w = dm[i] * dn[j];
-#pragma unroll
-      for (int c = 0; c < C; ++c) {
-        const int sm = c & 1, sn = c >> 1;
-        const int64_t b = bas[j][sn];
-        if (b == kNoBlock) continue;
-        const int64_t q = 2 * (b + ll + (sm ? -mi : mi));
-        const double2 v = make_double2(fl[q], fl[q + 1]);
-        if (ODD && (sm != sn)) {
-          acc[i][j][c].x -= w * v.x;
-          acc[i][j][c].y -= w * v.y;
-        } else {
-          acc[i][j][c].x += w * v.x;
-          acc[i][j][c].y += w * v.y;
-        }
+__device__ __forceinline__ double2 ld2(cdouble *p, int64_t i) { return make_double2(p[2 * i], p[2 * i + 1]); }
+
+// ------------------------------------------------------------------ packing
+// Gi[(off(an,am) + (l - l0(an,am))) * C + c], l0 = max(an & ~3, am & ~3):
+// c_l f_{l, +-am, +-an} with the (-1)^l of mixed signs folded in, zero where
+// the coefficient is not part of the transform.
+__global__ void k_pack_inv(Geo g, const double2 *__restrict__ flmn, const int64_t *__restrict__ nbase,
+                           const int64_t *__restrict__ poff, double2 *__restrict__ Gi) {
+  const int an = blockIdx.y;
+  const int am = blockIdx.x;
+  const int C = 2 * g.NS;
+  const int l0 = max(an & ~3, am & ~3);
+  const int64_t off = poff[(int64_t)an * g.L + am];
+  for (int t = threadIdx.x; t < (g.L - l0) * C; t += blockDim.x) {
+    const int l = l0 + t / C, c = t % C;
+    const int sm = c & 1, sn = c >> 1;
+    const int m = sm ? -am : am, n = sn ? -an : an;
+    double2 v = make_double2(0.0, 0.0);
+    const bool ok = an < g.N && l >= g.L0 && l >= am && l >= an && (!g.nmodeL || l == an);
+    if (ok) {
+      const int64_t b = nbase[n + g.N - 1];
+      if (b != kNoBlock) {
+        v = flmn[b + (int64_t)l * l + l + m];
+        double s = g.clmode == 1 ? (2.0 * l + 1.0) * kInv8Pi2 : 1.0;
+        if ((sm != sn) && (l & 1)) s = -s;
+        v = cscale(v, s);
       }
     }
+    Gi[(off + (l - l0)) * C + c] = v;
   }
 }
 
-template <int A, int B, int NS>
-__global__ __launch_bounds__(64) void k_contract_inv(Geo g, const double2 *__restrict__ flmn,
-                                                     const int64_t *__restrict__ nbase,
-                                                     const double *__restrict__ Dinv,
-                                                     const int64_t *__restrict__ inv_off,
-                                                     double2 *__restrict__ X1) {
+// ------------------------------------------------------------------ inverse
+template <int NS>
+__global__ __launch_bounds__(256) void k_contract_inv(Geo g, const double2 *__restrict__ Gi,
+                                                      const int64_t *__restrict__ poff,
+                                                      const double *__restrict__ Dinv,
+                                                      const int64_t *__restrict__ inv_off,
+                                                      double2 *__restrict__ X1) {
   constexpr int C = 2 * NS;
-  const int lane = threadIdx.x;
+  __shared__ double2 sD[2][kCols][32];  // [buffer][column][m' pairs] = 64 doubles per column
+  const int tid = threadIdx.x;
+  // the wavefront index is uniform: tell the compiler, so that everything
+  // derived from it lives in scalar registers and feeds scalar loads
+  const int lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int mpb = (g.L + 63) >> 6;
   const int mb = blockIdx.x % mpb;
   const int mt = blockIdx.x / mpb;
-  const int m0 = mt * A, n0 = blockIdx.y * B;
+  const int m0 = mt * kTile, n0 = blockIdx.y * kTile;
   const int mp = mb * 64 + lane;
-  cdouble *fl = (cdouble *)flmn;
-  cint64 *nbs = (cint64 *)nbase;
-  cint64 *ioff = (cint64 *)inv_off;
-  int64_t bas[B][NS];
+  // this wavefront's 2 x 2 pairs
+  const int am0 = m0 + 2 * (w & 1), an0 = n0 + 2 * (w >> 1);
+  cdouble *G = (cdouble *)Gi;
+  const int lt = max(m0, n0);  // first l of the tile's coefficient streams
+  int64_t gofs[2][2];
 #pragma unroll
-  for (int j = 0; j < B; ++j) {
-    const int nj = n0 + j;
-    bas[j][0] = nj < g.N ? nbs[nj + g.N - 1] : kNoBlock;
-    if (NS == 2) bas[j][NS - 1] = nj < g.N ? nbs[-nj + g.N - 1] : kNoBlock;
-  }
-  double2 acc[A][B][C];
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-  for (int i = 0; i < A; ++i)
+    for (int b = 0; b < 2; ++b) {
+      const int am = am0 + a, an = an0 + b;
+      gofs[a][b] = (am < g.L && an < g.N) ? poff[(int64_t)an * g.L + am] : -1;
+    }
+  double2 acc[2][2][C];
 #pragma unroll
-    for (int j = 0; j < B; ++j)
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-      for (int c = 0; c < C; ++c) acc[i][j][c] = make_double2(0.0, 0.0);
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int c = 0; c < C; ++c) acc[a][b][c] = make_double2(0.0, 0.0);
 
-  int l = max(max(g.L0, mb * 64), max(m0, n0));
-  if (l < g.L && (l & 1)) {
-    inv_step<A, B, NS, true>(l, g, Dinv, ioff[l], mp, m0, n0, fl, bas, acc);
-    ++l;
+  // staging map: thread -> (column, two consecutive m')
+  const int col = tid >> 5, pr = tid & 31;
+  const int colv = col < kTile ? m0 + col : n0 + (col - kTile);
+  auto fetch = [&](int l) -> double2 {
+    const int rowlen = (l + 64) & ~63;
+    const int cc = min(colv, l);
+    const double *src = Dinv + inv_off[l] + (int64_t)cc * rowlen + mb * 64 + 2 * pr;
+    return *(const double2 *)src;
+  };
+  const int lbeg = max(max(g.L0, mb * 64), lt);
+  double2 nxt = make_double2(0.0, 0.0);
+  if (lbeg < g.L) nxt = fetch(lbeg);
+  int buf = 0;
+  for (int l = lbeg; l < g.L; ++l) {
+    sD[buf][col][pr] = nxt;
+    __syncthreads();
+    if (l + 1 < g.L) nxt = fetch(l + 1);
+    const double *col_d = (const double *)&sD[buf][0][0];
+    double dm[2], dn[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) dm[a] = col_d[(2 * (w & 1) + a) * 64 + lane];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) dn[b] = col_d[(kTile + 2 * (w >> 1) + b) * 64 + lane];
+    const int64_t li = l - lt;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        if (gofs[a][b] < 0) continue;
+        const double wgt = dm[a] * dn[b];
+        const int64_t q = (gofs[a][b] + li) * C;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          const double2 v = ld2(G, q + c);
+          acc[a][b][c].x += wgt * v.x;
+          acc[a][b][c].y += wgt * v.y;
+        }
+      }
+    buf ^= 1;
   }
-  for (; l + 1 < g.L; l += 2) {
-    inv_step<A, B, NS, false>(l, g, Dinv, ioff[l], mp, m0, n0, fl, bas, acc);
-    inv_step<A, B, NS, true>(l + 1, g, Dinv, ioff[l + 1], mp, m0, n0, fl, bas, acc);
-  }
-  if (l < g.L) inv_step<A, B, NS, false>(l, g, Dinv, ioff[l], mp, m0, n0, fl, bas, acc);
 
   if (mp >= g.L) return;
 #pragma unroll
-  for (int i = 0; i < A; ++i) {
-    const int mi = m0 + i;
+  for (int a = 0; a < 2; ++a) {
+    const int mi = am0 + a;
     if (mi >= g.L) continue;
 #pragma unroll
-    for (int j = 0; j < B; ++j) {
-      const int nj = n0 + j;
+    for (int b = 0; b < 2; ++b) {
+      const int nj = an0 + b;
       if (nj >= g.N) continue;
 #pragma unroll
       for (int c = 0; c < C; ++c) {
@@ -131,7 +161,7 @@ __global__ __launch_bounds__(64) void k_contract_inv(Geo g, const double2 *__res
         if (sm && mi == 0) continue;
         if (sn && nj == 0) continue;
         const int m = sm ? -mi : mi, n = sn ? -nj : nj;
-        double2 v = acc[i][j][c];
+        double2 v = acc[a][b][c];
         if ((sm != sn) && (mp & 1)) v = make_double2(-v.x, -v.y);
         v = cmul_ipow(v, n - m);
         const int nslot = g.real ? n : (n + g.nG) % g.nG;
@@ -142,76 +172,93 @@ __global__ __launch_bounds__(64) void k_contract_inv(Geo g, const double2 *__res
   }
 }
 
-template <int A, int B, int NS>
-__global__ __launch_bounds__(64) void k_contract_fwd(Geo g, const double2 *__restrict__ Gp,
-                                                     const int64_t *__restrict__ nbase,
-                                                     const double *__restrict__ Dfwd,
-                                                     const int64_t *__restrict__ fwd_off,
-                                                     double2 *__restrict__ flmn) {
+// ------------------------------------------------------------------ forward
+template <int NS>
+__global__ __launch_bounds__(256) void k_contract_fwd(Geo g, const double2 *__restrict__ Gp,
+                                                      const int64_t *__restrict__ nbase,
+                                                      const double *__restrict__ Dfwd,
+                                                      const int64_t *__restrict__ fwd_off,
+                                                      double2 *__restrict__ flmn) {
   constexpr int C = 2 * NS;
-  const int lane = threadIdx.x;
+  __shared__ double2 sD[2][kCols][32];
+  const int tid = threadIdx.x;
+  // the wavefront index is uniform: tell the compiler, so that everything
+  // derived from it lives in scalar registers and feeds scalar loads
+  const int lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lbn = (g.L + 63) >> 6;
   const int lb = blockIdx.x % lbn;
   const int mt = blockIdx.x / lbn;
-  const int m0 = mt * A, n0 = blockIdx.y * B;
+  const int m0 = mt * kTile, n0 = blockIdx.y * kTile;
   const int lo = lb * 64;
+  if (max(m0, n0) > lo + 63) return;  // no l of this block reaches the tile
   const int l = lo + lane;
-  const int lc = min(l, g.L - 1);
+  const int am0 = m0 + 2 * (w & 1), an0 = n0 + 2 * (w >> 1);
   cdouble *G = (cdouble *)Gp;
-  cint64 *foff = (cint64 *)fwd_off;
-  double2 acc[A][B][C];
+  double2 acc[2][2][C];
 #pragma unroll
-  for (int i = 0; i < A; ++i)
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int j = 0; j < B; ++j)
+    for (int b = 0; b < 2; ++b)
 #pragma unroll
-      for (int c = 0; c < C; ++c) acc[i][j][c] = make_double2(0.0, 0.0);
+      for (int c = 0; c < C; ++c) acc[a][b][c] = make_double2(0.0, 0.0);
+  bool pv[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) pv[a][b] = (am0 + a < g.L) && (an0 + b < g.N) && (an0 + b < g.L);
 
+  const int col = tid >> 5, pr = tid & 31;
+  const int colv = col < kTile ? m0 + col : n0 + (col - kTile);
+  const bool colok = colv < g.L && (col < kTile || colv < g.N);
+  // Delta^l_{m', colv} for l = lo + 2 pr + {0, 1}
+  auto fetch = [&](int mp) -> double2 {
+    const int mx = max(mp, colv);
+    if (!colok || mx > lo + 63) return make_double2(0.0, 0.0);
+    const int64_t base = fwd_off[(int64_t)mp * g.L + colv] - (mx & ~63);
+    const int l0 = lo + 2 * pr;
+    const int la = min(l0, g.L - 1), lbb = min(l0 + 1, g.L - 1);
+    return make_double2(Dfwd[base + la], Dfwd[base + lbb]);
+  };
   const int mpmax = min(g.L - 1, lo + 63);
+  double2 nxt = fetch(0);
+  int buf = 0;
   for (int mp = 0; mp <= mpmax; ++mp) {
-    double dm[A], dn[B];
+    sD[buf][col][pr] = nxt;
+    __syncthreads();
+    if (mp + 1 <= mpmax) nxt = fetch(mp + 1);
+    const double *col_d = (const double *)&sD[buf][0][0];
+    double dm[2], dn[2];
 #pragma unroll
-    for (int i = 0; i < A; ++i) {
-      const int c = m0 + i;
-      const int mx = max(mp, c);
-      dm[i] = (c < g.L && mx <= lo + 63) ? Dfwd[foff[(int64_t)mp * g.L + c] + (lc - (mx & ~63))] : 0.0;
-    }
+    for (int a = 0; a < 2; ++a) dm[a] = col_d[(2 * (w & 1) + a) * 64 + lane];
 #pragma unroll
-    for (int j = 0; j < B; ++j) {
-      const int c = n0 + j;
-      const int mx = max(mp, c);
-      dn[j] = (c < g.N && c < g.L && mx <= lo + 63) ? Dfwd[foff[(int64_t)mp * g.L + c] + (lc - (mx & ~63))] : 0.0;
-    }
+    for (int b = 0; b < 2; ++b) dn[b] = col_d[(kTile + 2 * (w >> 1) + b) * 64 + lane];
 #pragma unroll
-    for (int i = 0; i < A; ++i) {
-      const int mi = m0 + i;
-      if (mi >= g.L) continue;
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-      for (int j = 0; j < B; ++j) {
-        const int nj = n0 + j;
-        if (nj >= g.N) continue;
-        const double w = dm[i] * dn[j];
-        const int64_t gb = (((int64_t)nj * g.L + mi) * g.L + mp) * C;
+      for (int b = 0; b < 2; ++b) {
+        if (!pv[a][b]) continue;
+        const double wgt = dm[a] * dn[b];
+        const int64_t q = ((((int64_t)(an0 + b) * g.L + (am0 + a)) * g.L) + mp) * C;
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-          const double2 v = make_double2(G[2 * (gb + c)], G[2 * (gb + c) + 1]);
-          acc[i][j][c].x += w * v.x;
-          acc[i][j][c].y += w * v.y;
+          const double2 v = ld2(G, q + c);
+          acc[a][b][c].x += wgt * v.x;
+          acc[a][b][c].y += wgt * v.y;
         }
       }
-    }
+    buf ^= 1;
   }
-  if (l >= g.L) return;
-  cint64 *nbs = (cint64 *)nbase;
+
+  if (l >= g.L || l < g.L0) return;
   const int64_t ll = (int64_t)l * l + l;
 #pragma unroll
-  for (int i = 0; i < A; ++i) {
-    const int mi = m0 + i;
-    if (mi >= g.L || mi > l) continue;
+  for (int a = 0; a < 2; ++a) {
+    const int mi = am0 + a;
+    if (!(mi < g.L) || mi > l) continue;
 #pragma unroll
-    for (int j = 0; j < B; ++j) {
-      const int nj = n0 + j;
-      if (nj >= g.N || nj > l || l < g.L0) continue;
+    for (int b = 0; b < 2; ++b) {
+      const int nj = an0 + b;
+      if (!pv[a][b] || nj > l) continue;
       if (g.nmodeL && nj != l) continue;
 #pragma unroll
       for (int c = 0; c < C; ++c) {
@@ -219,39 +266,51 @@ __global__ __launch_bounds__(64) void k_contract_fwd(Geo g, const double2 *__res
         if (sm && mi == 0) continue;
         if (sn && nj == 0) continue;
         const int m = sm ? -mi : mi, n = sn ? -nj : nj;
-        const int64_t b = nbs[n + g.N - 1];
-        if (b == kNoBlock) continue;
-        double2 v = acc[i][j][c];
+        const int64_t bb = nbase[n + g.N - 1];
+        if (bb == kNoBlock) continue;
+        double2 v = acc[a][b][c];
         if ((sm != sn) && (l & 1)) v = make_double2(-v.x, -v.y);
-        flmn[b + ll + m] = cmul_ipow(v, m - n);
+        if (g.clmode == 2) v = cscale(v, (2.0 * l + 1.0) * kInv8Pi2);
+        flmn[bb + ll + m] = cmul_ipow(v, m - n);
       }
     }
   }
 }
 
-constexpr int kA = 2, kB = 4;
-
 }  // namespace
 
+int64_t pack_inv_size(const Geo &g, int64_t *poff_host) {
+  int64_t acc = 0;
+  for (int an = 0; an < g.N; ++an)
+    for (int am = 0; am < g.L; ++am) {
+      if (poff_host) poff_host[(int64_t)an * g.L + am] = acc;
+      const int l0 = std::max(an & ~3, am & ~3);
+      acc += std::max(0, g.L - l0);
+    }
+  return acc * 2 * g.NS;  // double2 elements
+}
+
 void launch_contract_inv(const Geo &g, const double2 *flmn, NBase nb, const DlTables &dl,
-                         double2 *X1, hipStream_t st) {
+                         const int64_t *poff, double2 *Gi, double2 *X1, hipStream_t st) {
+  k_pack_inv<<<dim3(g.L, g.N), 256, 0, st>>>(g, flmn, nb.base, poff, Gi);
+  SO3_LAUNCH_CHECK();
   const int mpb = (g.L + 63) / 64;
-  dim3 grid(mpb * ((g.L + kA - 1) / kA), (g.N + kB - 1) / kB);
+  dim3 grid(mpb * ((g.L + kTile - 1) / kTile), (g.N + kTile - 1) / kTile);
   if (g.NS == 2)
-    k_contract_inv<kA, kB, 2><<<grid, 64, 0, st>>>(g, flmn, nb.base, dl.inv, dl.inv_off, X1);
+    k_contract_inv<2><<<grid, 256, 0, st>>>(g, Gi, poff, dl.inv, dl.inv_off, X1);
   else
-    k_contract_inv<kA, kB, 1><<<grid, 64, 0, st>>>(g, flmn, nb.base, dl.inv, dl.inv_off, X1);
+    k_contract_inv<1><<<grid, 256, 0, st>>>(g, Gi, poff, dl.inv, dl.inv_off, X1);
   SO3_LAUNCH_CHECK();
 }
 
 void launch_contract_fwd(const Geo &g, const double2 *Gp, NBase nb, const DlTables &dl,
                          double2 *flmn, hipStream_t st) {
   const int lbn = (g.L + 63) / 64;
-  dim3 grid(lbn * ((g.L + kA - 1) / kA), (g.N + kB - 1) / kB);
+  dim3 grid(lbn * ((g.L + kTile - 1) / kTile), (g.N + kTile - 1) / kTile);
   if (g.NS == 2)
-    k_contract_fwd<kA, kB, 2><<<grid, 64, 0, st>>>(g, Gp, nb.base, dl.fwd, dl.fwd_off, flmn);
+    k_contract_fwd<2><<<grid, 256, 0, st>>>(g, Gp, nb.base, dl.fwd, dl.fwd_off, flmn);
   else
-    k_contract_fwd<kA, kB, 1><<<grid, 64, 0, st>>>(g, Gp, nb.base, dl.fwd, dl.fwd_off, flmn);
+    k_contract_fwd<1><<<grid, 256, 0, st>>>(g, Gp, nb.base, dl.fwd, dl.fwd_off, flmn);
   SO3_LAUNCH_CHECK();
 }
 

@@ -14,7 +14,8 @@ template <int DIR>
 __global__ __launch_bounds__(256) void k_dft(FftDesc fd, int K, int ld, double2 *__restrict__ X,
                                              int howmany) {
   extern __shared__ double2 lds[];
-  double2 *A = lds, *B = lds + K * ld;
+  double2 *A = lds, *tw = lds + K * ld;
+  fft_load_twiddles(tw, fd);
   const int n = fd.n;
   const int64_t r0 = (int64_t)blockIdx.x * K;
   for (int idx = threadIdx.x; idx < K * n; idx += blockDim.x) {
@@ -22,10 +23,10 @@ __global__ __launch_bounds__(256) void k_dft(FftDesc fd, int K, int ld, double2 
     A[s * ld + k] = (r0 + s < howmany) ? X[(r0 + s) * n + k] : make_double2(0.0, 0.0);
   }
   __syncthreads();
-  const double2 *R = fft_run<DIR>(A, B, K, ld, fd);
+  fft_run<DIR>(A, K, ld, fd, tw);
   for (int idx = threadIdx.x; idx < K * n; idx += blockDim.x) {
     const int s = idx / n, k = idx - s * n;
-    if (r0 + s < howmany) X[(r0 + s) * n + k] = R[s * ld + k];
+    if (r0 + s < howmany) X[(r0 + s) * n + k] = A[s * ld + k];
   }
 }
 
@@ -39,8 +40,11 @@ extern "C" int so3x_dft(void *x, int n, int howmany, int dir, void *stream) {
     if (n < 1 || howmany < 0 || (dir != 1 && dir != -1)) throw HipError("bad so3x_dft arguments");
     const FftDesc fd = fft_plan(n);
     const int ld = fft_ld(fd);
-    const int K = std::max(1, std::min(8, lds_limit() / (2 * ld * (int)sizeof(double2))));
-    const size_t sh = 2 * (size_t)K * ld * sizeof(double2);
+    if (fd.len > kFftThreads * kFftElems) throw HipError("DFT length exceeds the register budget");
+    const int tb = fd.len * (int)sizeof(double2);
+    int K = std::max(1, std::min(8, kFftThreads * kFftElems / fd.len));
+    while (K > 1 && (size_t)K * ld * sizeof(double2) + tb > (size_t)lds_limit()) --K;
+    const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
     if (sh > (size_t)lds_limit()) throw HipError("DFT length exceeds the LDS budget");
     const unsigned grid = (unsigned)((howmany + K - 1) / K);
     if (grid == 0) return 0;

@@ -19,6 +19,9 @@ struct Geo {
   int real;        // real signal: n >= 0 only, c2r / r2c over gamma
   int NS;          // n signs handled per |n|: 2 complex, 1 real
   int nmodeL;      // keep only l == |n| (direct transforms with N_MODE_L)
+  int clmode;      // (2l+1)/(8 pi^2) factor: 1 in the inverse contraction
+                   // (synthesis), 2 in the forward one (adjoint of the
+                   // inverse), 0 nowhere (forward, adjoint of the forward)
 };
 
 // Per-n data: base[n + N - 1] = storage index of (l=0, m=0, n) in the flmn
@@ -33,18 +36,29 @@ struct NBase {
 //   X1 rows  : [n_slot][m_slot][m'], m' < L          (inverse contraction out)
 //   Gp       : [|n|][|m|][m'][2*NS] sign combos       (forward contraction in)
 //   X2       : [n_slot][b][a or m_slot]               (beta / alpha stages)
+// Packed inverse coefficient streams: pair (|n|, |m|) at poff[|n| L + |m|]
+// (in units of 2*NS double2), covering l from max(|n| & ~3, |m| & ~3).
+// Returns the packed size in double2 and fills poff_host when non-null.
+int64_t pack_inv_size(const Geo &g, int64_t *poff_host);
 void launch_contract_inv(const Geo &g, const double2 *flmn, NBase nb, const DlTables &dl,
-                         double2 *X1, hipStream_t st);
+                         const int64_t *poff, double2 *Gi, double2 *X1, hipStream_t st);
 void launch_contract_fwd(const Geo &g, const double2 *Gp, NBase nb, const DlTables &dl,
                          double2 *flmn, hipStream_t st);
 
 void launch_inv_beta(const Geo &g, const double2 *X1, double2 *X2, NBase nb, const double2 *phase,
                      hipStream_t st);
-void launch_alpha(const Geo &g, double2 *X2, int dir, hipStream_t st);
-void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, hipStream_t st);
-void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, hipStream_t st);
+void launch_alpha(const Geo &g, double2 *X2, int dir, double scale, hipStream_t st);
+void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, double scale, hipStream_t st);
+void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, double scale, hipStream_t st);
+// adjoint != 0: zero-fill instead of the beta extension and no weights (the
+// beta stage of the adjoint of the inverse)
 void launch_fwd_beta(const Geo &g, const double2 *X2, double2 *Gp, NBase nb, const double2 *phase,
-                     const double2 *wr, int Pw, hipStream_t st);
+                     const double2 *wr, int Pw, int adjoint, hipStream_t st);
+// beta stage of the adjoint of the forward transform (so3_adjoint.c:730-905):
+// m' < 0 by symmetry, weight convolution, phase exp(+i m' pi/(2L-1)),
+// backward DFT over m' / (2L-1), adjoint of the extension
+void launch_adjf_beta(const Geo &g, const double2 *X1, double2 *X2, NBase nb, const double2 *phase,
+                      const double2 *wr, int Pw, hipStream_t st);
 // zero the flmn entries a forward transform must write but does not compute
 void launch_fwd_zero(const Geo &g, double2 *flmn, NBase nb, int direct, int64_t flmn_size,
                      int storage, int order, hipStream_t st);

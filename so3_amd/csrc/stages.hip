@@ -23,7 +23,7 @@ namespace so3g {
 namespace {
 
 constexpr double kTwoPi = 6.283185307179586476925286766559005768;
-constexpr int kThreads = 256;
+constexpr int kThreads = kFftThreads;
 
 __device__ inline int n_of_slot(const Geo &g, int slot) {
   if (g.real) return slot;
@@ -38,7 +38,8 @@ __global__ __launch_bounds__(kThreads) void k_inv_beta(Geo g, FftDesc fd, int K,
                                                        const int64_t *__restrict__ nbase,
                                                        const double2 *__restrict__ phase) {
   extern __shared__ double2 lds[];
-  double2 *A = lds, *B = lds + K * ld;
+  double2 *A = lds, *tw = lds + K * ld;
+  fft_load_twiddles(tw, fd);
   const int chunks = (g.nA + K - 1) / K;
   const int nslot = blockIdx.x / chunks;
   const int ms0 = (blockIdx.x % chunks) * K;
@@ -61,10 +62,10 @@ __global__ __launch_bounds__(kThreads) void k_inv_beta(Geo g, FftDesc fd, int K,
     A[s * ld + k] = v;
   }
   __syncthreads();
-  const double2 *R = fft_run<+1>(A, B, K, ld, fd);
+  fft_run<+1>(A, K, ld, fd, tw);
   for (int idx = threadIdx.x; idx < K * g.nB; idx += blockDim.x) {
     const int b = idx / K, s = idx - b * K;
-    if (ms0 + s < g.nA) X2[((int64_t)nslot * g.nB + b) * g.nA + ms0 + s] = R[s * ld + b];
+    if (ms0 + s < g.nA) X2[((int64_t)nslot * g.nB + b) * g.nA + ms0 + s] = A[s * ld + b];
   }
 }
 
@@ -73,17 +74,18 @@ template <int DIR>
 __global__ __launch_bounds__(kThreads) void k_rows(FftDesc fd, int K, int ld, double2 *__restrict__ X,
                                                    int64_t nrows, int n, double scale) {
   extern __shared__ double2 lds[];
-  double2 *A = lds, *B = lds + K * ld;
+  double2 *A = lds, *tw = lds + K * ld;
+  fft_load_twiddles(tw, fd);
   const int64_t r0 = (int64_t)blockIdx.x * K;
   for (int idx = threadIdx.x; idx < K * n; idx += blockDim.x) {
     const int s = idx / n, k = idx - s * n;
     A[s * ld + k] = (r0 + s < nrows) ? X[(r0 + s) * n + k] : make_double2(0.0, 0.0);
   }
   __syncthreads();
-  const double2 *R = fft_run<DIR>(A, B, K, ld, fd);
+  fft_run<DIR>(A, K, ld, fd, tw);
   for (int idx = threadIdx.x; idx < K * n; idx += blockDim.x) {
     const int s = idx / n, k = idx - s * n;
-    if (r0 + s < nrows) X[(r0 + s) * n + k] = cscale(R[s * ld + k], scale);
+    if (r0 + s < nrows) X[(r0 + s) * n + k] = cscale(A[s * ld + k], scale);
   }
 }
 
@@ -91,9 +93,10 @@ __global__ __launch_bounds__(kThreads) void k_rows(FftDesc fd, int K, int ld, do
 __global__ __launch_bounds__(kThreads) void k_inv_gamma(Geo g, FftDesc fd, int K, int ld,
                                                         const double2 *__restrict__ X2,
                                                         double2 *__restrict__ fc,
-                                                        double *__restrict__ fr) {
+                                                        double *__restrict__ fr, double scale) {
   extern __shared__ double2 lds[];
-  double2 *A = lds, *B = lds + K * ld;
+  double2 *A = lds, *tw = lds + K * ld;
+  fft_load_twiddles(tw, fd);
   const int64_t S = (int64_t)g.nB * g.nA;
   const int64_t p0 = (int64_t)blockIdx.x * K;
   const int nG = 2 * g.N - 1;
@@ -115,15 +118,15 @@ __global__ __launch_bounds__(kThreads) void k_inv_gamma(Geo g, FftDesc fd, int K
     A[s * ld + k] = v;
   }
   __syncthreads();
-  const double2 *R = fft_run<+1>(A, B, K, ld, fd);
+  fft_run<+1>(A, K, ld, fd, tw);
   for (int idx = threadIdx.x; idx < K * nG; idx += blockDim.x) {
     const int k = idx / K, s = idx - k * K;
     const int64_t p = p0 + s;
     if (p >= S) continue;
     if (g.real)
-      fr[k * S + p] = R[s * ld + k].x;
+      fr[k * S + p] = A[s * ld + k].x * scale;
     else
-      fc[k * S + p] = R[s * ld + k];
+      fc[k * S + p] = cscale(A[s * ld + k], scale);
   }
 }
 
@@ -131,9 +134,10 @@ __global__ __launch_bounds__(kThreads) void k_inv_gamma(Geo g, FftDesc fd, int K
 __global__ __launch_bounds__(kThreads) void k_fwd_gamma(Geo g, FftDesc fd, int K, int ld,
                                                         const double2 *__restrict__ fc,
                                                         const double *__restrict__ fr,
-                                                        double2 *__restrict__ X2) {
+                                                        double2 *__restrict__ X2, double sc) {
   extern __shared__ double2 lds[];
-  double2 *A = lds, *B = lds + K * ld;
+  double2 *A = lds, *tw = lds + K * ld;
+  fft_load_twiddles(tw, fd);
   const int64_t S = (int64_t)g.nB * g.nA;
   const int64_t p0 = (int64_t)blockIdx.x * K;
   const int nG = 2 * g.N - 1;
@@ -145,13 +149,12 @@ __global__ __launch_bounds__(kThreads) void k_fwd_gamma(Geo g, FftDesc fd, int K
     A[s * ld + k] = v;
   }
   __syncthreads();
-  const double2 *R = fft_run<-1>(A, B, K, ld, fd);
-  const double sc = kTwoPi / (double)nG;
+  fft_run<-1>(A, K, ld, fd, tw);
   const int nout = g.real ? g.N : nG;
   for (int idx = threadIdx.x; idx < K * nout; idx += blockDim.x) {
     const int k = idx / K, s = idx - k * K;
     const int64_t p = p0 + s;
-    if (p < S) X2[k * S + p] = cscale(R[s * ld + k], sc);
+    if (p < S) X2[k * S + p] = cscale(A[s * ld + k], sc);
   }
 }
 
@@ -161,14 +164,17 @@ __global__ __launch_bounds__(kThreads) void k_fwd_beta(Geo g, FftDesc fb, FftDes
                                                        double2 *__restrict__ Gp,
                                                        const int64_t *__restrict__ nbase,
                                                        const double2 *__restrict__ phase,
-                                                       const double2 *__restrict__ wr, int Pw) {
+                                                       const double2 *__restrict__ wr, int Pw,
+                                                       int adjoint) {
   extern __shared__ double2 lds[];
-  double2 *A = lds, *B = lds + K * ld;
+  double2 *A = lds, *twb = lds + K * ld, *tww = twb + fb.len;
   const int chunks = (g.nA + K - 1) / K;
   const int nslot = blockIdx.x / chunks;
   const int ms0 = (blockIdx.x % chunks) * K;
   const int n = n_of_slot(g, nslot);
   if (nbase[n + g.N - 1] == kNoBlock) return;  // block not transformed
+  fft_load_twiddles(twb, fb);
+  fft_load_twiddles(tww, fw);
   const int Pb = g.Pb, L = g.L;
   for (int idx = threadIdx.x; idx < K * Pb; idx += blockDim.x) {
     const int k = idx / K, s = idx - k * K;
@@ -177,37 +183,57 @@ __global__ __launch_bounds__(kThreads) void k_fwd_beta(Geo g, FftDesc fb, FftDes
     if (mslot < g.nA) {
       const int m = m_of_slot(g, mslot);
       int b = k;
-      bool neg = false;
+      bool neg = false, zero = false;
       if (!g.ss) {
-        if (b >= L) { b = 2 * L - 2 - b; neg = ((m + n) & 1); }
+        if (b >= L) { b = 2 * L - 2 - b; neg = ((m + n) & 1); zero = adjoint; }
       } else {
-        if (b > L) { b = 2 * L - b; neg = ((m + n) & 1); }
+        if (b > L) { b = 2 * L - b; neg = ((m + n) & 1); zero = adjoint; }
       }
-      v = X2[((int64_t)nslot * g.nB + b) * g.nA + mslot];
+      v = zero ? make_double2(0.0, 0.0) : X2[((int64_t)nslot * g.nB + b) * g.nA + mslot];
       if (neg) v = make_double2(-v.x, -v.y);
     }
     A[s * ld + k] = v;
   }
   __syncthreads();
-  double2 *R = fft_run<-1>(A, B, K, ld, fb);
-  double2 *O = (R == A) ? B : A;
-  // F~(m'') = DFT/Pb * phase, placed at m'' mod Pw (zero elsewhere)
-  for (int idx = threadIdx.x; idx < K * Pw; idx += blockDim.x) {
-    const int s = idx / Pw, r = idx - s * Pw;
-    const int mpp = r <= L - 1 ? r : r - Pw;
-    double2 v = make_double2(0.0, 0.0);
-    if (mpp >= -(L - 1)) v = cmul(R[s * ld + (mpp < 0 ? mpp + Pb : mpp)], phase[mpp + L - 1]);
-    O[s * ld + r] = v;
+  fft_run<-1>(A, K, ld, fb, twb);
+  // F~(m'') = DFT/Pb * phase, placed at m'' mod Pw (zero elsewhere); the
+  // remap runs through registers because it widens the sequence in place
+  {
+    constexpr int kMaxPer = 16;
+    double2 v[kMaxPer];
+#pragma unroll
+    for (int b = 0; b < kMaxPer; ++b) {
+      const int idx = threadIdx.x + b * kThreads;
+      v[b] = make_double2(0.0, 0.0);
+      if (idx < K * Pw) {
+        const int s = idx / Pw, r = idx - s * Pw;
+        const int mpp = r <= L - 1 ? r : r - Pw;
+        if (mpp >= -(L - 1)) v[b] = cmul(A[s * ld + (mpp < 0 ? mpp + Pb : mpp)], phase[mpp + L - 1]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < kMaxPer; ++b) {
+      const int idx = threadIdx.x + b * kThreads;
+      if (idx < K * Pw) {
+        const int s = idx / Pw, r = idx - s * Pw;
+        A[s * ld + r] = v[b];
+      }
+    }
   }
   __syncthreads();
-  double2 *R2 = fft_run<+1>(O, R, K, ld, fw);
-  for (int idx = threadIdx.x; idx < K * Pw; idx += blockDim.x) {
-    const int s = idx / Pw, r = idx - s * Pw;
-    R2[s * ld + r] = cmul(R2[s * ld + r], wr[r]);
+  double sc = 1.0;
+  if (!adjoint) {
+    fft_run<+1>(A, K, ld, fw, tww);
+    for (int idx = threadIdx.x; idx < K * Pw; idx += blockDim.x) {
+      const int s = idx / Pw, r = idx - s * Pw;
+      A[s * ld + r] = cmul(A[s * ld + r], wr[r]);
+    }
+    __syncthreads();
+    fft_run<-1>(A, K, ld, fw, tww);
+    sc = kTwoPi / (double)Pw;
   }
-  __syncthreads();
-  double2 *R3 = fft_run<-1>(R2, (R2 == O) ? R : O, K, ld, fw);
-  const double sc = kTwoPi / (double)Pw;
+  const double2 *R3 = A;
   const int C = 2 * g.NS;
   const int an = n < 0 ? -n : n;
   for (int idx = threadIdx.x; idx < K * L; idx += blockDim.x) {
@@ -235,6 +261,88 @@ __global__ __launch_bounds__(kThreads) void k_fwd_beta(Geo g, FftDesc fb, FftDes
         dst[cm + 2 * cn] = ((cm != cn) && (mp & 1)) ? make_double2(-G.x, -G.y) : G;
       }
     }
+  }
+}
+
+// ---------------------------------------------------------------- adjoint of the forward: beta
+__global__ __launch_bounds__(kThreads) void k_adjf_beta(Geo g, FftDesc fb, FftDesc fw, int K, int ld,
+                                                        const double2 *__restrict__ X1,
+                                                        double2 *__restrict__ X2,
+                                                        const int64_t *__restrict__ nbase,
+                                                        const double2 *__restrict__ phase,
+                                                        const double2 *__restrict__ wr, int Pw) {
+  extern __shared__ double2 lds[];
+  double2 *A = lds, *twb = lds + K * ld, *tww = twb + fb.len;
+  const int chunks = (g.nA + K - 1) / K;
+  const int nslot = blockIdx.x / chunks;
+  const int ms0 = (blockIdx.x % chunks) * K;
+  const int n = n_of_slot(g, nslot);
+  const bool active = nbase[n + g.N - 1] != kNoBlock;
+  const int Pb = g.Pb, L = g.L;
+  fft_load_twiddles(twb, fb);
+  fft_load_twiddles(tww, fw);
+  // G(m') for |m'| <= L-1 at m' mod Pw, F(-m') = (-1)^{m+n} F(m')
+  for (int idx = threadIdx.x; idx < K * Pw; idx += blockDim.x) {
+    const int s = idx / Pw, r = idx - s * Pw;
+    const int mslot = ms0 + s;
+    const int mp = r <= L - 1 ? r : r - Pw;
+    double2 v = make_double2(0.0, 0.0);
+    if (active && mslot < g.nA && mp >= -(L - 1)) {
+      const int m = m_of_slot(g, mslot);
+      v = X1[((int64_t)nslot * g.nA + mslot) * L + (mp < 0 ? -mp : mp)];
+      if (mp < 0 && ((m + n) & 1)) v = make_double2(-v.x, -v.y);
+    }
+    A[s * ld + r] = v;
+  }
+  __syncthreads();
+  fft_run<+1>(A, K, ld, fw, tww);
+  for (int idx = threadIdx.x; idx < K * Pw; idx += blockDim.x) {
+    const int s = idx / Pw, r = idx - s * Pw;
+    A[s * ld + r] = cmul(A[s * ld + r], wr[r]);
+  }
+  __syncthreads();
+  fft_run<-1>(A, K, ld, fw, tww);
+  // x 4 pi^2 / Pw, phase exp(+i m' pi/(2L-1)) / Pb, to m' mod Pb (in place
+  // through registers: the sequence shrinks from Pw to Pb)
+  {
+    constexpr int kMaxPer = 16;
+    const double sc = 2.0 * kTwoPi * 3.14159265358979323846264338327950288 / (double)Pw;
+    double2 v[kMaxPer];
+#pragma unroll
+    for (int b = 0; b < kMaxPer; ++b) {
+      const int idx = threadIdx.x + b * kThreads;
+      v[b] = make_double2(0.0, 0.0);
+      if (idx < K * Pb) {
+        const int s = idx / Pb, k = idx - s * Pb;
+        const int mp = k <= L - 1 ? k : k - Pb;
+        if (mp >= -(L - 1))
+          v[b] = cscale(cmul(A[s * ld + (mp < 0 ? mp + Pw : mp)], phase[mp + L - 1]), sc);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < kMaxPer; ++b) {
+      const int idx = threadIdx.x + b * kThreads;
+      if (idx < K * Pb) {
+        const int s = idx / Pb, k = idx - s * Pb;
+        A[s * ld + k] = v[b];
+      }
+    }
+  }
+  __syncthreads();
+  fft_run<+1>(A, K, ld, fb, twb);
+  // adjoint of the extension: out(b) = E(b) + (-1)^{m+n} E(2L-2-b), b < L-1
+  for (int idx = threadIdx.x; idx < K * g.nB; idx += blockDim.x) {
+    const int b = idx / K, s = idx - b * K;
+    const int mslot = ms0 + s;
+    if (mslot >= g.nA) continue;
+    const int m = m_of_slot(g, mslot);
+    double2 v = A[s * ld + b];
+    if (b < L - 1) {
+      const double2 e = A[s * ld + (2 * L - 2 - b)];
+      v = ((m + n) & 1) ? csub(v, e) : cadd(v, e);
+    }
+    X2[((int64_t)nslot * g.nB + b) * g.nA + mslot] = v;
   }
 }
 
@@ -267,11 +375,16 @@ __global__ void k_fwd_zero(Geo g, double2 *__restrict__ flmn, const int64_t *__r
     flmn[base + (int64_t)l * l + l + m] = make_double2(0.0, 0.0);
 }
 
-int choose_k(int ld, int want) {
-  const int per = 2 * ld * (int)sizeof(double2);
-  int k = lds_limit() / per;
-  if (k < 1) throw HipError("FFT length " + std::to_string(ld) + " exceeds the LDS budget");
-  return std::min(k, want);
+// sequences per workgroup: every pass must fit kFftElems values per thread,
+// and the LDS (data + twiddle tables) should leave room for two workgroups
+// per CU when it can
+int choose_k(int ld, int len, int tables, int want) {
+  int k = std::min(want, kThreads * kFftElems / len);
+  const int budget = std::max(lds_limit() / 2, (int)(ld * sizeof(double2)) + tables);
+  while (k > 1 && (size_t)k * ld * sizeof(double2) + tables > (size_t)budget) --k;
+  if (k < 1 || (size_t)k * ld * sizeof(double2) + tables > (size_t)lds_limit() || len > kThreads * kFftElems)
+    throw HipError("DFT length " + std::to_string(len) + " exceeds the LDS / register budget");
+  return k;
 }
 
 }  // namespace
@@ -296,65 +409,84 @@ void launch_inv_beta(const Geo &g, const double2 *X1, double2 *X2, NBase nb, con
                      hipStream_t st) {
   const FftDesc fd = fft_plan(g.Pb);
   const int ld = fft_ld(fd);
-  const int K = choose_k(ld, 8);
-  const size_t sh = 2 * (size_t)K * ld * sizeof(double2);
+  const int tb = fd.len * (int)sizeof(double2);
+  const int K = choose_k(ld, fd.len, tb, 8);
+  const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
   set_lds(k_inv_beta, sh);
   const int chunks = (g.nA + K - 1) / K;
   k_inv_beta<<<g.nG * chunks, kThreads, sh, st>>>(g, fd, K, ld, X1, X2, nb.base, phase);
   SO3_LAUNCH_CHECK();
 }
 
-void launch_alpha(const Geo &g, double2 *X2, int dir, hipStream_t st) {
+void launch_alpha(const Geo &g, double2 *X2, int dir, double scale, hipStream_t st) {
   const FftDesc fd = fft_plan(g.nA);
   const int ld = fft_ld(fd);
-  const int K = choose_k(ld, 8);
-  const size_t sh = 2 * (size_t)K * ld * sizeof(double2);
+  const int tb = fd.len * (int)sizeof(double2);
+  const int K = choose_k(ld, fd.len, tb, 8);
+  const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
   const int64_t nrows = (int64_t)g.nG * g.nB;
   const unsigned grid = (unsigned)((nrows + K - 1) / K);
   if (dir > 0) {
     set_lds(k_rows<+1>, sh);
-    k_rows<+1><<<grid, kThreads, sh, st>>>(fd, K, ld, X2, nrows, g.nA, 1.0);
+    k_rows<+1><<<grid, kThreads, sh, st>>>(fd, K, ld, X2, nrows, g.nA, scale);
   } else {
     set_lds(k_rows<-1>, sh);
-    k_rows<-1><<<grid, kThreads, sh, st>>>(fd, K, ld, X2, nrows, g.nA, 1.0 / g.nA);
+    k_rows<-1><<<grid, kThreads, sh, st>>>(fd, K, ld, X2, nrows, g.nA, scale);
   }
   SO3_LAUNCH_CHECK();
 }
 
-void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, hipStream_t st) {
+void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, double scale, hipStream_t st) {
   const FftDesc fd = fft_plan(2 * g.N - 1);
   const int ld = fft_ld(fd);
-  const int K = choose_k(ld, 16);
-  const size_t sh = 2 * (size_t)K * ld * sizeof(double2);
+  const int tb = fd.len * (int)sizeof(double2);
+  const int K = choose_k(ld, fd.len, tb, 16);
+  const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
   set_lds(k_inv_gamma, sh);
   const int64_t S = (int64_t)g.nB * g.nA;
   k_inv_gamma<<<(unsigned)((S + K - 1) / K), kThreads, sh, st>>>(
-      g, fd, K, ld, X2, g.real ? nullptr : (double2 *)f, g.real ? (double *)f : nullptr);
+      g, fd, K, ld, X2, g.real ? nullptr : (double2 *)f, g.real ? (double *)f : nullptr, scale);
   SO3_LAUNCH_CHECK();
 }
 
-void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, hipStream_t st) {
+void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, double scale, hipStream_t st) {
   const FftDesc fd = fft_plan(2 * g.N - 1);
   const int ld = fft_ld(fd);
-  const int K = choose_k(ld, 16);
-  const size_t sh = 2 * (size_t)K * ld * sizeof(double2);
+  const int tb = fd.len * (int)sizeof(double2);
+  const int K = choose_k(ld, fd.len, tb, 16);
+  const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
   set_lds(k_fwd_gamma, sh);
   const int64_t S = (int64_t)g.nB * g.nA;
   k_fwd_gamma<<<(unsigned)((S + K - 1) / K), kThreads, sh, st>>>(
-      g, fd, K, ld, g.real ? nullptr : (const double2 *)f, g.real ? (const double *)f : nullptr, X2);
+      g, fd, K, ld, g.real ? nullptr : (const double2 *)f, g.real ? (const double *)f : nullptr, X2, scale);
   SO3_LAUNCH_CHECK();
 }
 
 void launch_fwd_beta(const Geo &g, const double2 *X2, double2 *Gp, NBase nb, const double2 *phase,
-                     const double2 *wr, int Pw, hipStream_t st) {
+                     const double2 *wr, int Pw, int adjoint, hipStream_t st) {
   const FftDesc fb = fft_plan(g.Pb);
   const FftDesc fw = fft_plan(Pw);
   const int ld = std::max(fft_ld(fb), fft_ld(fw));
-  const int K = choose_k(ld, 8);
-  const size_t sh = 2 * (size_t)K * ld * sizeof(double2);
+  const int tb = (fb.len + fw.len) * (int)sizeof(double2);
+  const int K = choose_k(ld, std::max(fb.len, fw.len), tb, 8);
+  const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
   set_lds(k_fwd_beta, sh);
   const int chunks = (g.nA + K - 1) / K;
-  k_fwd_beta<<<g.nG * chunks, kThreads, sh, st>>>(g, fb, fw, K, ld, X2, Gp, nb.base, phase, wr, Pw);
+  k_fwd_beta<<<g.nG * chunks, kThreads, sh, st>>>(g, fb, fw, K, ld, X2, Gp, nb.base, phase, wr, Pw, adjoint);
+  SO3_LAUNCH_CHECK();
+}
+
+void launch_adjf_beta(const Geo &g, const double2 *X1, double2 *X2, NBase nb, const double2 *phase,
+                      const double2 *wr, int Pw, hipStream_t st) {
+  const FftDesc fb = fft_plan(g.Pb);
+  const FftDesc fw = fft_plan(Pw);
+  const int ld = std::max(fft_ld(fb), fft_ld(fw));
+  const int tb = (fb.len + fw.len) * (int)sizeof(double2);
+  const int K = choose_k(ld, std::max(fb.len, fw.len), tb, 8);
+  const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
+  set_lds(k_adjf_beta, sh);
+  const int chunks = (g.nA + K - 1) / K;
+  k_adjf_beta<<<g.nG * chunks, kThreads, sh, st>>>(g, fb, fw, K, ld, X1, X2, nb.base, phase, wr, Pw);
   SO3_LAUNCH_CHECK();
 }
 

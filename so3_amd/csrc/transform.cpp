@@ -33,9 +33,12 @@ struct Plan {
   int Pw = 0;
   double2 *phase_inv = nullptr;  // Pb entries
   double2 *phase_fwd = nullptr;  // 2L-1 entries, includes 1/Pb
+  double2 *phase_ai = nullptr;   // 2L-1: exp(-i m'' pi/(2L-1)), adjoint of the inverse
+  double2 *phase_af = nullptr;   // 2L-1: exp(+i m' pi/(2L-1)) / Pb, adjoint of the forward
   double2 *wr = nullptr;         // Pw entries
   int64_t *nbase = nullptr;      // 2*(2N-1): masked, then unmasked
-  size_t x1 = 0, x2 = 0;         // workspace sizes (double2)
+  int64_t *poff = nullptr;       // packed inverse stream offsets, N*L
+  size_t x1 = 0, x2 = 0, x3 = 0; // workspace sizes (double2)
 };
 
 int smooth_at_least(int n) {
@@ -73,6 +76,7 @@ Geo make_geo(const so3_parameters_t *p, Method method) {
   g.Pb = g.ss ? 2 * g.L : 2 * g.L - 1;
   g.NS = g.real ? 1 : 2;
   g.nmodeL = (method == kDirect && p->n_mode == SO3_N_MODE_L) ? 1 : 0;
+  g.clmode = 1;
   return g;
 }
 
@@ -112,6 +116,17 @@ Plan build_plan(const so3_parameters_t *p, Method method) {
     }
   }
   P.phase_fwd = upload(pf);
+  {
+    std::vector<double2> ai(2 * L - 1), af(2 * L - 1);
+    for (int mp = -(L - 1); mp <= L - 1; ++mp) {
+      const long double a = kPiL * mp / (2.0L * L - 1.0L);
+      ai[mp + L - 1] = g.ss ? make_double2(1.0, 0.0) : make_double2((double)cosl(a), (double)-sinl(a));
+      af[mp + L - 1] = g.ss ? make_double2(1.0 / g.Pb, 0.0)
+                            : make_double2((double)(cosl(a) / g.Pb), (double)(sinl(a) / g.Pb));
+    }
+    P.phase_ai = upload(ai);
+    P.phase_af = upload(af);
+  }
   // weights w(p) = int_0^pi exp(-i p b) sin b db (so3_sampling_weight) at p mod Pw,
   // backward DFT of length Pw >= 4L-3 (any such length is exact)
   P.Pw = smooth_at_least(4 * L - 3);
@@ -170,6 +185,14 @@ Plan build_plan(const so3_parameters_t *p, Method method) {
   }
   P.x1 = std::max<size_t>((size_t)g.nG * g.nA * L, (size_t)N * L * L * 2 * g.NS);
   P.x2 = (size_t)g.nG * g.nB * g.nA;
+  {
+    std::vector<int64_t> po((size_t)N * L);
+    P.x3 = (size_t)pack_inv_size(g, po.data());
+    int64_t *d = nullptr;
+    SO3_HIP(hipMalloc(&d, sizeof(int64_t) * po.size()));
+    SO3_HIP(hipMemcpy(d, po.data(), sizeof(int64_t) * po.size(), hipMemcpyHostToDevice));
+    P.poff = d;
+  }
   // DFT plans are built now so that a timed call does not build them
   fft_plan(g.Pb);
   fft_plan(g.nA);
@@ -287,7 +310,7 @@ size_t workspace_bytes(const so3_parameters_t *p, Method method) {
   validate(p, method);
   std::lock_guard<std::mutex> lk(g_mu);
   const Plan &P = get_plan(p, method);
-  return sizeof(double2) * (P.x1 + P.x2);
+  return sizeof(double2) * (P.x1 + P.x2 + P.x3);
 }
 
 void prepare(const so3_parameters_t *p, Method method) {
@@ -302,14 +325,14 @@ void run_inverse(const so3_parameters_t *p, Method method, const void *flmn, voi
   std::lock_guard<std::mutex> lk(g_mu);
   const Plan &P = get_plan(p, method);
   const DlTables &dl = dl_tables(p->L, (int)p->dl_method);
-  double2 *ws = arena(P.x1 + P.x2);
-  double2 *X1 = ws, *X2 = ws + P.x1;
+  double2 *ws = arena(P.x1 + P.x2 + P.x3);
+  double2 *X1 = ws, *X2 = ws + P.x1, *Gi = ws + P.x1 + P.x2;
   const Geo &g = P.g;
   NBase nb{P.nbase};
-  timed(0, st, [&] { launch_contract_inv(g, (const double2 *)flmn, nb, dl, X1, st); });
+  timed(0, st, [&] { launch_contract_inv(g, (const double2 *)flmn, nb, dl, P.poff, Gi, X1, st); });
   timed(1, st, [&] { launch_inv_beta(g, X1, X2, nb, P.phase_inv, st); });
-  timed(2, st, [&] { launch_alpha(g, X2, +1, st); });
-  timed(3, st, [&] { launch_inv_gamma(g, X2, f, st); });
+  timed(2, st, [&] { launch_alpha(g, X2, +1, 1.0, st); });
+  timed(3, st, [&] { launch_inv_gamma(g, X2, f, 1.0, st); });
 }
 
 void run_forward(const so3_parameters_t *p, Method method, const void *f, void *flmn, hipStream_t st) {
@@ -321,9 +344,9 @@ void run_forward(const so3_parameters_t *p, Method method, const void *f, void *
   double2 *Gp = ws, *X2 = ws + P.x1;
   const Geo &g = P.g;
   NBase nb{P.nbase};
-  timed(4, st, [&] { launch_fwd_gamma(g, f, X2, st); });
-  timed(5, st, [&] { launch_alpha(g, X2, -1, st); });
-  timed(6, st, [&] { launch_fwd_beta(g, X2, Gp, nb, P.phase_fwd, P.wr, P.Pw, st); });
+  timed(4, st, [&] { launch_fwd_gamma(g, f, X2, 2 * M_PI / (2.0 * g.N - 1.0), st); });
+  timed(5, st, [&] { launch_alpha(g, X2, -1, 1.0 / g.nA, st); });
+  timed(6, st, [&] { launch_fwd_beta(g, X2, Gp, nb, P.phase_fwd, P.wr, P.Pw, 0, st); });
   timed(7, st, [&] {
     launch_fwd_zero(g, (double2 *)flmn, nb, method == kDirect, 0, (int)p->storage, (int)p->n_order, st);
   });
@@ -353,14 +376,52 @@ int stage_times(const char **names, double *ms, int *count, int cap) {
   return kNStages;
 }
 
+// adjoint of so3_core_inverse_direct: f -> flmn (so3_adjoint.c:34-400)
+void run_adjoint_inverse(const so3_parameters_t *p, const void *f, void *flmn, hipStream_t st) {
+  validate(p, kDirect);
+  std::lock_guard<std::mutex> lk(g_mu);
+  const Plan &P = get_plan(p, kDirect);
+  const DlTables &dl = dl_tables(p->L, (int)p->dl_method);
+  double2 *ws = arena(P.x1 + P.x2 + P.x3);
+  double2 *Gp = ws, *X2 = ws + P.x1;
+  Geo g = P.g;
+  NBase nb{P.nbase};
+  launch_fwd_gamma(g, f, X2, 1.0, st);
+  launch_alpha(g, X2, -1, 1.0, st);
+  launch_fwd_beta(g, X2, Gp, nb, P.phase_ai, P.wr, P.Pw, 1, st);
+  launch_fwd_zero(g, (double2 *)flmn, nb, 1, 0, (int)p->storage, (int)p->n_order, st);
+  g.clmode = 2;
+  launch_contract_fwd(g, Gp, nb, dl, (double2 *)flmn, st);
+}
+
+// adjoint of so3_core_forward_direct: flmn -> f (so3_adjoint.c:402-937)
+void run_adjoint_forward(const so3_parameters_t *p, const void *flmn, void *f, hipStream_t st) {
+  validate(p, kDirect);
+  std::lock_guard<std::mutex> lk(g_mu);
+  const Plan &P = get_plan(p, kDirect);
+  const DlTables &dl = dl_tables(p->L, (int)p->dl_method);
+  double2 *ws = arena(P.x1 + P.x2 + P.x3);
+  double2 *X1 = ws, *X2 = ws + P.x1, *Gi = ws + P.x1 + P.x2;
+  Geo g = P.g;
+  g.clmode = 0;
+  NBase nb{P.nbase};
+  launch_contract_inv(g, (const double2 *)flmn, nb, dl, P.poff, Gi, X1, st);
+  launch_adjf_beta(g, X1, X2, nb, P.phase_af, P.wr, P.Pw, st);
+  launch_alpha(g, X2, +1, 1.0 / g.nA, st);
+  launch_inv_gamma(g, X2, f, 1.0 / (2.0 * g.N - 1.0), st);
+}
+
 void release_all() {
   std::lock_guard<std::mutex> lk(g_mu);
   (void)hipDeviceSynchronize();
   for (auto &kv : g_plans) {
     (void)hipFree(kv.second.phase_inv);
     (void)hipFree(kv.second.phase_fwd);
+    (void)hipFree(kv.second.phase_ai);
+    (void)hipFree(kv.second.phase_af);
     (void)hipFree(kv.second.wr);
     (void)hipFree(kv.second.nbase);
+    (void)hipFree(kv.second.poff);
   }
   g_plans.clear();
   for (auto &kv : g_arena) (void)hipFree(kv.second.buf);

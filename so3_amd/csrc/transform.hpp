@@ -15,6 +15,8 @@ void run_inverse(const so3_parameters_t *p, Method method, const void *flmn_dev,
                  hipStream_t st);
 void run_forward(const so3_parameters_t *p, Method method, const void *f_dev, void *flmn_dev,
                  hipStream_t st);
+void run_adjoint_inverse(const so3_parameters_t *p, const void *f_dev, void *flmn_dev, hipStream_t st);
+void run_adjoint_forward(const so3_parameters_t *p, const void *flmn_dev, void *f_dev, hipStream_t st);
 void prepare(const so3_parameters_t *p, Method method);
 size_t workspace_bytes(const so3_parameters_t *p, Method method);
 void release_all();
