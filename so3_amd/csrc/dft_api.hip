@@ -40,9 +40,10 @@ extern "C" int so3x_dft(void *x, int n, int howmany, int dir, void *stream) {
     if (n < 1 || howmany < 0 || (dir != 1 && dir != -1)) throw HipError("bad so3x_dft arguments");
     const FftDesc fd = fft_plan(n);
     const int ld = fft_ld(fd);
-    if (fd.len > kFftThreads * kFftElems) throw HipError("DFT length exceeds the register budget");
+    const int cap = fft_capacity(fd);
+    if (fd.len > cap) throw HipError("DFT length exceeds the register budget");
     const int tb = fd.len * (int)sizeof(double2);
-    int K = std::max(1, std::min(8, kFftThreads * kFftElems / fd.len));
+    int K = std::max(1, std::min(8, cap / fd.len));
     while (K > 1 && (size_t)K * ld * sizeof(double2) + tb > (size_t)lds_limit()) --K;
     const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
     if (sh > (size_t)lds_limit()) throw HipError("DFT length exceeds the LDS budget");

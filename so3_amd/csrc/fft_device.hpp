@@ -4,9 +4,9 @@
 // LDS buffer at X + s*ld.  Each mixed-radix Stockham pass reads every
 // butterfly's inputs into registers, synchronises, and writes the outputs
 // back in place, so one buffer suffices.  Twiddles come from an LDS copy of
-// exp(-2 pi i k/len).  Radices 2, 3, 4, 5, 7, 8, 11, 13, 16 have unrolled
-// kernels; lengths with a larger prime factor go through Bluestein's
-// chirp-z algorithm on a power-of-two inner length M >= 2n-1.
+// exp(-2 pi i k/len).  Radices 2, 3, 4, 5, 7, 8, 16 have unrolled kernels;
+// lengths with a larger prime factor go through Bluestein's chirp-z
+// algorithm on a power-of-two inner length M >= 2n-1.
 // Convention (FFTW's): DIR = -1 forward  X_k = sum_j x_j exp(-2 pi i jk/n),
 //                      DIR = +1 backward X_k = sum_j x_j exp(+2 pi i jk/n),
 // both unnormalised.
@@ -15,16 +15,22 @@
 
 namespace so3g {
 
-constexpr int kMaxPass = 24;
+constexpr int kMaxPass = 16;
 constexpr int kFftThreads = 256;
-constexpr int kFftElems = 16;  // complex values a thread holds per pass
+constexpr int kFftElems = 8;  // complex values a thread holds per pass
+
+// radix codes (4 bits per pass in FftDesc::radcode)
+constexpr int kNumRadix = 7;
+constexpr int kRadixOf[kNumRadix] = {2, 3, 4, 5, 7, 8, 16};
+// butterflies a thread keeps in registers per pass of radix R
+__host__ __device__ constexpr int fft_nb(int R) { return kFftElems / R > 1 ? kFftElems / R : 1; }
 
 struct FftDesc {
   int n;                 // logical length
   int blue;              // 1: Bluestein
   int len;               // inner transform length (n, or M for Bluestein)
   int npass;             // radix passes of the inner transform
-  int rad[kMaxPass];
+  unsigned long long radcode;  // pass p uses kRadixOf[(radcode >> 4p) & 15]
   const double2 *tw;     // tw[k] = exp(-2 pi i k / len), k < len (global)
   const double2 *chirp;  // exp(-pi i (j^2 mod 2n) / n), j < n
   const double2 *bhat_f; // FFT_len of the forward chirp kernel
@@ -67,35 +73,6 @@ struct OddConst<7> {
                                   0.974927912181823607018131682993931217,
                                   0.433883739117558120475768332848358755};
 };
-template <>
-struct OddConst<11> {
-  static constexpr double c[5] = {0.841253532831181168861811648919367718,
-                                  0.415415013001886425529274149229623204,
-                                  -0.142314838273285140443792668616369669,
-                                  -0.654860733945285064056925072466293553,
-                                  -0.959492973614497389890368057066327699};
-  static constexpr double s[5] = {0.540640817455597582107635954318691695,
-                                  0.90963199535451837141171538307902846,
-                                  0.989821441880932732376092037776718787,
-                                  0.75574957435425828377403584397234442,
-                                  0.281732556841429697711417915346616899};
-};
-template <>
-struct OddConst<13> {
-  static constexpr double c[6] = {0.885456025653209895900375522015098879,
-                                  0.568064746731155802511807559127516625,
-                                  0.120536680255323053349067687452543582,
-                                  -0.354604887042535625969637892600018474,
-                                  -0.748510748171101098634630599701351384,
-                                  -0.970941817426052027156982276293789227};
-  static constexpr double s[6] = {0.464723172043768545656015335133104778,
-                                  0.822983865893656394579617423439381991,
-                                  0.992708874098053992800751649492520179,
-                                  0.935016242685414823439784599837830729,
-                                  0.66312265824079520237678549266676628,
-                                  0.239315664287557767148753726260211895};
-};
-
 // odd prime radix by symmetric pairs:
 //   X_k = a_0 + sum_{q=1}^{(R-1)/2} [ (a_q + a_{R-q}) cos(2 pi qk/R)
 //                                    + DIR i (a_q - a_{R-q}) sin(2 pi qk/R) ]
@@ -191,7 +168,8 @@ __device__ __forceinline__ void dft_small(double2 *a) {
     a[3] = cadd(e3, o3);
     a[7] = csub(e3, o3);
   } else if constexpr (R == 16) {
-    // 4 x 4 Cooley-Tukey in registers: X_{k1+4k2} = sum_q w16^{q k1} w4^{q k2} Y_q[k1]
+    // 4 x 4 Cooley-Tukey in registers: X_{k1+4k2} = sum_q w16^{q k1} w4^{q k2} Y_q[k1];
+    // the result stays in slot 4*k1 + k2 (see fft_out_slot)
 #pragma unroll
     for (int q = 0; q < 4; ++q) dft4<DIR>(a[q], a[q + 4], a[q + 8], a[q + 12]);
     a[5] = w16<DIR, 1>(a[5]);
@@ -203,28 +181,21 @@ __device__ __forceinline__ void dft_small(double2 *a) {
     a[7] = w16<DIR, 3>(a[7]);
     a[11] = w16<DIR, 6>(a[11]);
     a[15] = w16<DIR, 9>(a[15]);
-    double2 t[16];
 #pragma unroll
-    for (int k1 = 0; k1 < 4; ++k1) {
-      double2 y0 = a[4 * k1 + 0], y1 = a[4 * k1 + 1], y2 = a[4 * k1 + 2], y3 = a[4 * k1 + 3];
-      dft4<DIR>(y0, y1, y2, y3);
-      t[k1] = y0;
-      t[k1 + 4] = y1;
-      t[k1 + 8] = y2;
-      t[k1 + 12] = y3;
-    }
-#pragma unroll
-    for (int k = 0; k < 16; ++k) a[k] = t[k];
+    for (int k1 = 0; k1 < 4; ++k1) dft4<DIR>(a[4 * k1 + 0], a[4 * k1 + 1], a[4 * k1 + 2], a[4 * k1 + 3]);
   } else {
     dft_odd<R, DIR>(a);
   }
 }
 
-// Note on dft16 layout: after the first loop a[q + 4*k1] holds Y_q[k1]; the
-// twiddle w16^{q k1} is applied to a[q + 4 k1] for q, k1 >= 1 -- the second
-// loop reads a[4 k1 + q] as the k1-th group.  For that the first loop must
-// have stored Y_q[k1] at index 4*k1 + q, which dft4 on (q, q+4, q+8, q+12)
-// does (output k1 lands at q + 4*k1).
+// Register slot that holds natural output index q after dft_small<R>:
+// radix 16 leaves X_{k1 + 4 k2} in slot 4 k1 + k2, the others are in order.
+// (The first dft4 loop puts Y_q[k1] in slot q + 4 k1; the twiddled groups
+// a[4 k1 .. 4 k1 + 3] then go through the second dft4 in place.)
+template <int R>
+__host__ __device__ constexpr int fft_out_slot(int q) {
+  return R == 16 ? 4 * (q & 3) + (q >> 2) : q;
+}
 
 template <int DIR>
 __device__ __forceinline__ double2 twl(const double2 *tw, int k) {
@@ -234,9 +205,11 @@ __device__ __forceinline__ double2 twl(const double2 *tw, int k) {
 }
 
 // One in-place Stockham pass of radix R over S sequences of length len.
+// Kept out of line: a pass is long, and separate functions keep the
+// register allocation of one radix from inflating the others.
 template <int R, int DIR>
-__device__ void fft_pass(double2 *X, int S, int ld, int len, int Ns, const double2 *tw) {
-  constexpr int NB = (kFftElems + R - 1) / R;
+__device__ __attribute__((noinline)) void fft_pass(double2 *X, int S, int ld, int len, int Ns, const double2 *tw) {
+  constexpr int NB = fft_nb(R);
   const int nb = len / R;
   const int tot = S * nb;
   const int twstep = len / (Ns * R);
@@ -266,7 +239,7 @@ __device__ void fft_pass(double2 *X, int S, int ld, int len, int Ns, const doubl
     const int idx = threadIdx.x + b * kFftThreads;
     if (idx < tot) {
 #pragma unroll
-      for (int q = 0; q < R; ++q) X[ob[b] + q * Ns] = v[b][q];
+      for (int q = 0; q < R; ++q) X[ob[b] + q * Ns] = v[b][fft_out_slot<R>(q)];
     }
   }
   __syncthreads();
@@ -276,20 +249,17 @@ __device__ void fft_pass(double2 *X, int S, int ld, int len, int Ns, const doubl
 template <int DIR>
 __device__ void fft_inner(double2 *X, int S, int ld, const FftDesc &d, const double2 *tw) {
   int Ns = 1;
-  for (int p = 0; p < d.npass; ++p) {
-    const int R = d.rad[p];
-    switch (R) {
-      case 2: fft_pass<2, DIR>(X, S, ld, d.len, Ns, tw); break;
-      case 3: fft_pass<3, DIR>(X, S, ld, d.len, Ns, tw); break;
-      case 4: fft_pass<4, DIR>(X, S, ld, d.len, Ns, tw); break;
-      case 5: fft_pass<5, DIR>(X, S, ld, d.len, Ns, tw); break;
-      case 7: fft_pass<7, DIR>(X, S, ld, d.len, Ns, tw); break;
-      case 8: fft_pass<8, DIR>(X, S, ld, d.len, Ns, tw); break;
-      case 11: fft_pass<11, DIR>(X, S, ld, d.len, Ns, tw); break;
-      case 13: fft_pass<13, DIR>(X, S, ld, d.len, Ns, tw); break;
-      default: fft_pass<16, DIR>(X, S, ld, d.len, Ns, tw); break;
+  unsigned long long code = d.radcode;
+  for (int p = 0; p < d.npass; ++p, code >>= 4) {
+    switch ((int)(code & 15)) {
+      case 0: fft_pass<2, DIR>(X, S, ld, d.len, Ns, tw); Ns *= 2; break;
+      case 1: fft_pass<3, DIR>(X, S, ld, d.len, Ns, tw); Ns *= 3; break;
+      case 2: fft_pass<4, DIR>(X, S, ld, d.len, Ns, tw); Ns *= 4; break;
+      case 3: fft_pass<5, DIR>(X, S, ld, d.len, Ns, tw); Ns *= 5; break;
+      case 4: fft_pass<7, DIR>(X, S, ld, d.len, Ns, tw); Ns *= 7; break;
+      case 5: fft_pass<8, DIR>(X, S, ld, d.len, Ns, tw); Ns *= 8; break;
+      default: fft_pass<16, DIR>(X, S, ld, d.len, Ns, tw); Ns *= 16; break;
     }
-    Ns *= R;
   }
 }
 
@@ -309,33 +279,130 @@ __device__ void fft_run(double2 *X, int S, int ld, const FftDesc &d, const doubl
     return;
   }
   const int n = d.n, M = d.len;
-  for (int idx = threadIdx.x; idx < S * M; idx += kFftThreads) {
-    const int s = idx / M, j = idx - s * M;
+  for (int j = threadIdx.x; j < M; j += kFftThreads) {
+    double2 c = make_double2(0.0, 0.0);
     if (j < n) {
-      double2 c = d.chirp[j];
+      c = d.chirp[j];
       if (DIR > 0) c.y = -c.y;
-      X[s * ld + j] = cmul(X[s * ld + j], c);
-    } else {
-      X[s * ld + j] = make_double2(0.0, 0.0);
     }
+    for (int s = 0; s < S; ++s) X[s * ld + j] = j < n ? cmul(X[s * ld + j], c) : c;
   }
   __syncthreads();
   fft_inner<-1>(X, S, ld, d, tw);
   const double2 *bh = DIR < 0 ? d.bhat_f : d.bhat_b;
-  for (int idx = threadIdx.x; idx < S * M; idx += kFftThreads) {
-    const int s = idx / M, j = idx - s * M;
-    X[s * ld + j] = cmul(X[s * ld + j], bh[j]);
+  for (int j = threadIdx.x; j < M; j += kFftThreads) {
+    const double2 b = bh[j];
+    for (int s = 0; s < S; ++s) X[s * ld + j] = cmul(X[s * ld + j], b);
   }
   __syncthreads();
   fft_inner<+1>(X, S, ld, d, tw);
   const double inv = 1.0 / (double)M;
-  for (int idx = threadIdx.x; idx < S * n; idx += kFftThreads) {
-    const int s = idx / n, j = idx - s * n;
+  for (int j = threadIdx.x; j < n; j += kFftThreads) {
     double2 c = d.chirp[j];
     if (DIR > 0) c.y = -c.y;
-    X[s * ld + j] = cscale(cmul(X[s * ld + j], c), inv);
+    c = cscale(c, inv);
+    for (int s = 0; s < S; ++s) X[s * ld + j] = cmul(X[s * ld + j], c);
   }
   __syncthreads();
+}
+
+}  // namespace so3g
+
+namespace so3g {
+
+// ---------------------------------------------------------------------------
+// Wave-level engine: one wavefront transforms one sequence of length
+// len <= kWaveMax held contiguously in LDS.  Within a pass a wavefront
+// reads all its inputs before writing, and LDS operations of a wavefront
+// complete in order, so no workgroup barrier is needed anywhere; only a
+// compiler scheduling fence separates the read and write phases.
+constexpr int kWaveElems = 16;  // complex values a lane holds per pass
+constexpr int kWaveMax = 64 * kWaveElems;
+__host__ __device__ constexpr int wave_nb(int R) { return kWaveElems / R > 1 ? kWaveElems / R : 1; }
+
+template <int R, int DIR>
+__device__ __attribute__((noinline)) void wave_pass(double2 *X, int len, int Ns, const double2 *tw, int lane) {
+  constexpr int NB = wave_nb(R);
+  const int nb = len / R;
+  const int twstep = len / (Ns * R);
+  const unsigned mns = fdiv_magic(Ns);
+  double2 v[NB][R];
+  int ob[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int t = lane + 64 * b;
+    if (t < nb) {
+      const int tq = fdiv(t, mns), j = t - tq * Ns;
+#pragma unroll
+      for (int q = 0; q < R; ++q) v[b][q] = X[t + q * nb];
+      if (Ns > 1) {
+#pragma unroll
+        for (int q = 1; q < R; ++q) v[b][q] = cmul(v[b][q], twl<DIR>(tw, q * j * twstep));
+      }
+      dft_small<R, DIR>(v[b]);
+      ob[b] = tq * Ns * R + j;
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const int t = lane + 64 * b;
+    if (t < nb) {
+#pragma unroll
+      for (int q = 0; q < R; ++q) X[ob[b] + q * Ns] = v[b][fft_out_slot<R>(q)];
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+}
+
+template <int DIR>
+__device__ void wave_inner(double2 *X, const FftDesc &d, const double2 *tw, int lane) {
+  int Ns = 1;
+  unsigned long long code = d.radcode;
+  for (int p = 0; p < d.npass; ++p, code >>= 4) {
+    switch ((int)(code & 15)) {
+      case 0: wave_pass<2, DIR>(X, d.len, Ns, tw, lane); Ns *= 2; break;
+      case 1: wave_pass<3, DIR>(X, d.len, Ns, tw, lane); Ns *= 3; break;
+      case 2: wave_pass<4, DIR>(X, d.len, Ns, tw, lane); Ns *= 4; break;
+      case 3: wave_pass<5, DIR>(X, d.len, Ns, tw, lane); Ns *= 5; break;
+      case 4: wave_pass<7, DIR>(X, d.len, Ns, tw, lane); Ns *= 7; break;
+      case 5: wave_pass<8, DIR>(X, d.len, Ns, tw, lane); Ns *= 8; break;
+      default: wave_pass<16, DIR>(X, d.len, Ns, tw, lane); Ns *= 16; break;
+    }
+  }
+}
+
+// Full DFT of length d.n in place on one wavefront's sequence X (n values,
+// room for d.len).  Bluestein's chirp and kernel multiplies are per lane.
+template <int DIR>
+__device__ void wave_fft(double2 *X, const FftDesc &d, const double2 *tw, int lane) {
+  if (!d.blue) {
+    wave_inner<DIR>(X, d, tw, lane);
+    return;
+  }
+  const int n = d.n, M = d.len;
+  for (int j = lane; j < M; j += 64) {
+    if (j < n) {
+      double2 c = d.chirp[j];
+      if (DIR > 0) c.y = -c.y;
+      X[j] = cmul(X[j], c);
+    } else {
+      X[j] = make_double2(0.0, 0.0);
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  wave_inner<-1>(X, d, tw, lane);
+  const double2 *bh = DIR < 0 ? d.bhat_f : d.bhat_b;
+  for (int j = lane; j < M; j += 64) X[j] = cmul(X[j], bh[j]);
+  __builtin_amdgcn_wave_barrier();
+  wave_inner<+1>(X, d, tw, lane);
+  const double inv = 1.0 / (double)M;
+  for (int j = lane; j < n; j += 64) {
+    double2 c = d.chirp[j];
+    if (DIR > 0) c.y = -c.y;
+    X[j] = cmul(X[j], cscale(c, inv));
+  }
+  __builtin_amdgcn_wave_barrier();
 }
 
 }  // namespace so3g

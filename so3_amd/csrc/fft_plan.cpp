@@ -3,6 +3,7 @@
 // host and rounded once to FP64.
 #include "fft_plan.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <map>
@@ -48,10 +49,11 @@ void host_fft_pow2(std::vector<cld> &x, int sign) {
 bool factor(int n, std::vector<int> &rad) {
   rad.clear();
   int m = n;
+  while (m % 16 == 0) { rad.push_back(16); m /= 16; }
   while (m % 8 == 0) { rad.push_back(8); m /= 8; }
   while (m % 4 == 0) { rad.push_back(4); m /= 4; }
   while (m % 2 == 0) { rad.push_back(2); m /= 2; }
-  for (int p : {3, 5, 7, 11, 13})
+  for (int p : {3, 5, 7})
     while (m % p == 0) { rad.push_back(p); m /= p; }
   return m == 1 && (int)rad.size() <= kMaxPass;
 }
@@ -104,7 +106,14 @@ FftDesc build(int n) {
     d.bhat_b = upload(bb);
   }
   d.npass = (int)rad.size();
-  for (int i = 0; i < d.npass; ++i) d.rad[i] = rad[i];
+  d.radcode = 0;
+  for (int i = 0; i < d.npass; ++i) {
+    int code = -1;
+    for (int c = 0; c < kNumRadix; ++c)
+      if (kRadixOf[c] == rad[i]) code = c;
+    if (code < 0) throw HipError("unsupported radix");
+    d.radcode |= (unsigned long long)code << (4 * i);
+  }
   std::vector<cld> tw(d.len);
   for (int k = 0; k < d.len; ++k) tw[k] = root(k, d.len);
   d.tw = upload(tw);
@@ -112,6 +121,16 @@ FftDesc build(int n) {
 }
 
 }  // namespace
+
+int fft_capacity(const FftDesc &d) {
+  int cap = kFftThreads * kFftElems * 4;
+  unsigned long long code = d.radcode;
+  for (int p = 0; p < d.npass; ++p, code >>= 4) {
+    const int R = kRadixOf[code & 15];
+    cap = std::min(cap, kFftThreads * fft_nb(R) * R);
+  }
+  return cap;
+}
 
 FftDesc fft_plan(int n) {
   static std::mutex mu;
@@ -127,4 +146,16 @@ FftDesc fft_plan(int n) {
   return d;
 }
 
+}  // namespace so3g
+
+namespace so3g {
+bool wave_fits(const FftDesc &d) {
+  if (d.len > kWaveMax) return false;
+  unsigned long long code = d.radcode;
+  for (int p = 0; p < d.npass; ++p, code >>= 4) {
+    const int R = kRadixOf[code & 15];
+    if (d.len / R > 64 * wave_nb(R)) return false;
+  }
+  return true;
+}
 }  // namespace so3g

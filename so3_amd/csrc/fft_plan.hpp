@@ -9,7 +9,17 @@ namespace so3g {
 // device for the lifetime of the process.  Thread-safe.
 FftDesc fft_plan(int n);
 
+// Largest S * len one workgroup can run through every pass of d (each
+// thread holds fft_nb(R) butterflies of radix R).
+int fft_capacity(const FftDesc &d);
+
 // LDS stride (in double2) needed per sequence to run FftDesc d.
 inline int fft_ld(const FftDesc &d) { return d.len + (d.len % 32 == 0 ? 1 : 0); }
 
+}  // namespace so3g
+
+namespace so3g {
+// true when one wavefront can run d (every pass fits wave_nb(R) butterflies
+// per lane)
+bool wave_fits(const FftDesc &d);
 }  // namespace so3g

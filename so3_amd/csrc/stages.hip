@@ -199,7 +199,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_beta(Geo g, FftDesc fb, FftDes
   // F~(m'') = DFT/Pb * phase, placed at m'' mod Pw (zero elsewhere); the
   // remap runs through registers because it widens the sequence in place
   {
-    constexpr int kMaxPer = 16;
+    constexpr int kMaxPer = kFftElems * 2;
     double2 v[kMaxPer];
 #pragma unroll
     for (int b = 0; b < kMaxPer; ++b) {
@@ -305,7 +305,7 @@ __global__ __launch_bounds__(kThreads) void k_adjf_beta(Geo g, FftDesc fb, FftDe
   // x 4 pi^2 / Pw, phase exp(+i m' pi/(2L-1)) / Pb, to m' mod Pb (in place
   // through registers: the sequence shrinks from Pw to Pb)
   {
-    constexpr int kMaxPer = 16;
+    constexpr int kMaxPer = kFftElems * 2;
     const double sc = 2.0 * kTwoPi * 3.14159265358979323846264338327950288 / (double)Pw;
     double2 v[kMaxPer];
 #pragma unroll
@@ -378,11 +378,11 @@ __global__ void k_fwd_zero(Geo g, double2 *__restrict__ flmn, const int64_t *__r
 // sequences per workgroup: every pass must fit kFftElems values per thread,
 // and the LDS (data + twiddle tables) should leave room for two workgroups
 // per CU when it can
-int choose_k(int ld, int len, int tables, int want) {
-  int k = std::min(want, kThreads * kFftElems / len);
+int choose_k(int ld, int cap, int len, int tables, int want) {
+  int k = std::min(want, cap / len);
   const int budget = std::max(lds_limit() / 2, (int)(ld * sizeof(double2)) + tables);
   while (k > 1 && (size_t)k * ld * sizeof(double2) + tables > (size_t)budget) --k;
-  if (k < 1 || (size_t)k * ld * sizeof(double2) + tables > (size_t)lds_limit() || len > kThreads * kFftElems)
+  if (k < 1 || (size_t)k * ld * sizeof(double2) + tables > (size_t)lds_limit() || len > cap)
     throw HipError("DFT length " + std::to_string(len) + " exceeds the LDS / register budget");
   return k;
 }
@@ -410,7 +410,7 @@ void launch_inv_beta(const Geo &g, const double2 *X1, double2 *X2, NBase nb, con
   const FftDesc fd = fft_plan(g.Pb);
   const int ld = fft_ld(fd);
   const int tb = fd.len * (int)sizeof(double2);
-  const int K = choose_k(ld, fd.len, tb, 8);
+  const int K = choose_k(ld, fft_capacity(fd), fd.len, tb, 8);
   const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
   set_lds(k_inv_beta, sh);
   const int chunks = (g.nA + K - 1) / K;
@@ -422,7 +422,7 @@ void launch_alpha(const Geo &g, double2 *X2, int dir, double scale, hipStream_t 
   const FftDesc fd = fft_plan(g.nA);
   const int ld = fft_ld(fd);
   const int tb = fd.len * (int)sizeof(double2);
-  const int K = choose_k(ld, fd.len, tb, 8);
+  const int K = choose_k(ld, fft_capacity(fd), fd.len, tb, 8);
   const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
   const int64_t nrows = (int64_t)g.nG * g.nB;
   const unsigned grid = (unsigned)((nrows + K - 1) / K);
@@ -440,7 +440,7 @@ void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, double scale, hi
   const FftDesc fd = fft_plan(2 * g.N - 1);
   const int ld = fft_ld(fd);
   const int tb = fd.len * (int)sizeof(double2);
-  const int K = choose_k(ld, fd.len, tb, 16);
+  const int K = choose_k(ld, fft_capacity(fd), fd.len, tb, 16);
   const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
   set_lds(k_inv_gamma, sh);
   const int64_t S = (int64_t)g.nB * g.nA;
@@ -453,7 +453,7 @@ void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, double scale, hi
   const FftDesc fd = fft_plan(2 * g.N - 1);
   const int ld = fft_ld(fd);
   const int tb = fd.len * (int)sizeof(double2);
-  const int K = choose_k(ld, fd.len, tb, 16);
+  const int K = choose_k(ld, fft_capacity(fd), fd.len, tb, 16);
   const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
   set_lds(k_fwd_gamma, sh);
   const int64_t S = (int64_t)g.nB * g.nA;
@@ -468,7 +468,7 @@ void launch_fwd_beta(const Geo &g, const double2 *X2, double2 *Gp, NBase nb, con
   const FftDesc fw = fft_plan(Pw);
   const int ld = std::max(fft_ld(fb), fft_ld(fw));
   const int tb = (fb.len + fw.len) * (int)sizeof(double2);
-  const int K = choose_k(ld, std::max(fb.len, fw.len), tb, 8);
+  const int K = choose_k(ld, std::min(fft_capacity(fb), fft_capacity(fw)), std::max(fb.len, fw.len), tb, 8);
   const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
   set_lds(k_fwd_beta, sh);
   const int chunks = (g.nA + K - 1) / K;
@@ -482,7 +482,7 @@ void launch_adjf_beta(const Geo &g, const double2 *X1, double2 *X2, NBase nb, co
   const FftDesc fw = fft_plan(Pw);
   const int ld = std::max(fft_ld(fb), fft_ld(fw));
   const int tb = (fb.len + fw.len) * (int)sizeof(double2);
-  const int K = choose_k(ld, std::max(fb.len, fw.len), tb, 8);
+  const int K = choose_k(ld, std::min(fft_capacity(fb), fft_capacity(fw)), std::max(fb.len, fw.len), tb, 8);
   const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
   set_lds(k_adjf_beta, sh);
   const int chunks = (g.nA + K - 1) / K;
