@@ -7,40 +7,66 @@
 // Forward (so3_core.c:1589-1617, ssht_core_mw_lb_forward_sov_conv_sym at :410):
 //   f_lmn = i^{m-n} sum_{0 <= m' <= l} D^l_{m'm} D^l_{m'n} G'(m,n,m').
 //
-// The four sign combinations (+-m, +-n) of a pair (|m|, |n|) are treated
-// together: with D^l_{m',-m} = (-1)^{l+m'} D^l_{m',m}, one product
-// D^l_{m'|m|} D^l_{m'|n|} feeds four complex FMAs.  The factor (-1)^l of the
+// The four sign combinations (+-m, +-n) of a pair (x, y) = (|m|, |n|) are
+// treated together: with D^l_{m',-m} = (-1)^{l+m'} D^l_{m',m}, one product
+// Q(l, m') = D^l_{m'x} D^l_{m'y} feeds all four.  The factor (-1)^l of the
 // mixed-sign combinations sits in the packed coefficients (inverse) or is
 // applied at the store (forward); (-1)^{m'} is applied at the store
 // (inverse) or folded into G' by the forward beta stage.
 //
-// Work decomposition (both directions): a workgroup of four wavefronts owns
-// a tile of 4 |m| x 4 |n| pairs and one 64-wide block of the lane index
-// (m' for the inverse, l for the forward); each wavefront owns 2 x 2 pairs.
-// Per step of the reduction loop (l for the inverse, m' for the forward)
-// the workgroup stages the 8 Delta columns of the tile (4 |m| + 4 |n|, one
-// 512-byte row each) in LDS, prefetched one step ahead, and every wavefront
-// reads the coefficients of its pairs as one 64-byte scalar load per pair
-// from a pair-contiguous stream, so f_lmn / G' never occupy vector registers.
+// For one pair the contraction is a small matrix product over the reduction
+// index r (m' forward, l inverse) with the lane index s (l forward, m'
+// inverse):  out(s, col) = sum_r Q(s, r) coef(r, col), col = (combination,
+// re/im), 8 columns (complex) or 4 (real).  The pairs (x, y) and (y, x) share
+// Q, so the two together fill the 16 columns of one 16x16x4 FP64 matrix
+// instruction; it runs as D'(col, s) = coef^T Q^T so that each lane holds the
+// re/im parts of its combinations in adjacent accumulator registers.  Only
+// the product Q = Dx * Dy is a vector instruction.
+//
+// Work decomposition: a wavefront owns one y, four consecutive x >= y (the
+// pairs (x, y) and, when x < N, (y, x)) and 64 values of the lane index
+// (four 16-wide tiles); the four wavefronts of a workgroup take four
+// consecutive y with the same x group and lane block, so the Dx tiles they
+// read meet in the cache.
 #include "kernels.hpp"
 
 namespace so3g {
 
 namespace {
 
-typedef const __attribute__((address_space(4))) double cdouble;
-
 constexpr double kInv8Pi2 = 0.012665147955292221430953884060716700591;  // 1/(8 pi^2)
-constexpr int kTile = 4;     // |m| and |n| per workgroup tile
-constexpr int kWaves = 4;    // wavefronts per workgroup (2 x 2 pairs each)
-constexpr int kCols = 2 * kTile;
+constexpr int kXG = 4;   // x values per wavefront
+constexpr int kNT = 4;   // 16-wide lane-index tiles per wavefront
 
-__device__ __forceinline__ double2 ld2(cdouble *p, int64_t i) { return make_double2(p[2 * i], p[2 * i + 1]); }
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ d4 mma(double a, double b, d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__host__ __device__ inline int64_t tri(int64_t k) { return k * (k + 1) / 2; }
+
+// Operand-row map of the coefficient matrix: lane i supplies row rho = i & 15
+// of coef^T at reduction offset i >> 4.  Row rho = 4 j + q of the product
+// lands in accumulator register j of the lanes with i >> 4 == q, so
+//   q >> 1      : pair half h (0: (x, y), 1: (y, x)),
+//   complex (C = 4): combination 2 (q & 1) + (j >> 1), part j & 1,
+//   real    (C = 2): combination q & 1, part j & 1 (registers 2, 3 unused).
+template <int C>
+__device__ __forceinline__ int coef_col(int rho, bool &used) {
+  const int j = rho >> 2, q = rho & 3;
+  if (C == 4) {
+    used = true;
+    return (2 * (q & 1) + (j >> 1)) * 2 + (j & 1);
+  }
+  used = j < 2;
+  return (q & 1) * 2 + (j & 1);
+}
 
 // ------------------------------------------------------------------ packing
-// Gi[(off(an,am) + (l - l0(an,am))) * C + c], l0 = max(an & ~3, am & ~3):
-// c_l f_{l, +-am, +-an} with the (-1)^l of mixed signs folded in, zero where
-// the coefficient is not part of the transform.
+// Gi[(off(an,am) + (l - l0(an,am))) * C + c], l0 = max(an & ~3, am & ~3),
+// l < P4(L): c_l f_{l, +-am, +-an} with the (-1)^l of mixed signs folded in,
+// zero where the coefficient is not part of the transform.
 __global__ void k_pack_inv(Geo g, const double2 *__restrict__ flmn, const int64_t *__restrict__ nbase,
                            const int64_t *__restrict__ poff, double2 *__restrict__ Gi) {
   const int an = blockIdx.y;
@@ -48,12 +74,13 @@ __global__ void k_pack_inv(Geo g, const double2 *__restrict__ flmn, const int64_
   const int C = 2 * g.NS;
   const int l0 = max(an & ~3, am & ~3);
   const int64_t off = poff[(int64_t)an * g.L + am];
-  for (int t = threadIdx.x; t < (g.L - l0) * C; t += blockDim.x) {
+  const int lend = (g.L + 3) & ~3;
+  for (int t = threadIdx.x; t < (lend - l0) * C; t += blockDim.x) {
     const int l = l0 + t / C, c = t % C;
     const int sm = c & 1, sn = c >> 1;
     const int m = sm ? -am : am, n = sn ? -an : an;
     double2 v = make_double2(0.0, 0.0);
-    const bool ok = an < g.N && l >= g.L0 && l >= am && l >= an && (!g.nmodeL || l == an);
+    const bool ok = an < g.N && l < g.L && l >= g.L0 && l >= am && l >= an && (!g.nmodeL || l == an);
     if (ok) {
       const int64_t b = nbase[n + g.N - 1];
       if (b != kNoBlock) {
@@ -67,101 +94,193 @@ __global__ void k_pack_inv(Geo g, const double2 *__restrict__ flmn, const int64_
   }
 }
 
+// ------------------------------------------------------------------ staging
+// Operand staging shared by both kernels.  A workgroup of four wavefronts
+// owns the pairs (x0 + a, y0 + w), a = 0..3, for wavefront w.  Per
+// reduction chunk (4 values of m' forward, of l inverse) it needs
+//   DX[a][t]  the Delta tile of column x0 + a, lane tile t   (16 x 64 doubles)
+//   DY[w][t]  the Delta tile of column y0 + w, lane tile t   (16 x 64 doubles)
+//   CV[w][a][h] four coefficient rows of pair half h           (8 x 4*2C doubles)
+// Every one of them is a contiguous run in HBM, so staging is a plain copy:
+// each wavefront copies its quarter (DX of a = w, its DY and its CV) with
+// 16-byte direct-to-LDS loads into a ring of kRing buffers, kRing - 1 chunks
+// ahead of the matrix instructions.  Regions outside the transform are
+// copied from a page of zeros, so the inner loop has no per-pair branches.
+constexpr int kRing = 3;
+constexpr int kBufD = 3 * 1024;  // doubles per ring buffer: DX, DY, CV
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+__device__ __forceinline__ void dma16(const double *src, double *lds_base) {
+  __builtin_amdgcn_global_load_lds((const void *)src, (lds_void *)lds_base, 16, 0, 0);
+}
+
+// wait until at most N vector-memory operations of this wavefront are
+// outstanding (expcnt and lgkmcnt left unconstrained)
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+  __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+}
+
+template <int C>
+struct Stage {
+  static constexpr int kRegD = 4 * 2 * C;      // doubles per coefficient region
+  static constexpr int kCvI = kRegD / 16;       // copy instructions for CV per wave
+  static constexpr int kInstr = 4 + kCvI;       // copy instructions per chunk per wave
+};
+
+__device__ __forceinline__ void wait_ring(int ahead, int n) {
+  // outstanding copies allowed = chunks issued after the one needed now
+  if (n == 6) {
+    if (ahead >= 2) wait_vm<12>(); else if (ahead == 1) wait_vm<6>(); else wait_vm<0>();
+  } else {
+    if (ahead >= 2) wait_vm<10>(); else if (ahead == 1) wait_vm<5>(); else wait_vm<0>();
+  }
+}
+
 // ------------------------------------------------------------------ inverse
+// Workgroup: x group x0 .. x0+3, y0 + w for wavefront w, m' block mpb ..
+// mpb+63 (four 16-wide tiles); reduction over l in chunks of 4.
 template <int NS>
-__global__ __launch_bounds__(256) void k_contract_inv(Geo g, const double2 *__restrict__ Gi,
+__global__ __launch_bounds__(256) void k_contract_inv(Geo g, const double *__restrict__ Gi,
                                                       const int64_t *__restrict__ poff,
-                                                      const double *__restrict__ Dinv,
-                                                      const int64_t *__restrict__ inv_off,
+                                                      const double *__restrict__ ti,
+                                                      const int64_t *__restrict__ offi, int nmt,
+                                                      const double *__restrict__ zero,
                                                       double2 *__restrict__ X1) {
   constexpr int C = 2 * NS;
-  __shared__ double2 sD[2][kCols][32];  // [buffer][column][m' pairs] = 64 doubles per column
-  const int tid = threadIdx.x;
-  // the wavefront index is uniform: tell the compiler, so that everything
-  // derived from it lives in scalar registers and feeds scalar loads
-  const int lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int mpb = (g.L + 63) >> 6;
-  const int mb = blockIdx.x % mpb;
-  const int mt = blockIdx.x / mpb;
-  const int m0 = mt * kTile, n0 = blockIdx.y * kTile;
-  const int mp = mb * 64 + lane;
-  // this wavefront's 2 x 2 pairs
-  const int am0 = m0 + 2 * (w & 1), an0 = n0 + 2 * (w >> 1);
-  cdouble *G = (cdouble *)Gi;
-  const int lt = max(m0, n0);  // first l of the tile's coefficient streams
-  int64_t gofs[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      const int am = am0 + a, an = an0 + b;
-      gofs[a][b] = (am < g.L && an < g.N) ? poff[(int64_t)an * g.L + am] : -1;
-    }
-  double2 acc[2][2][C];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int c = 0; c < C; ++c) acc[a][b][c] = make_double2(0.0, 0.0);
+  using St = Stage<C>;
+  __shared__ __attribute__((aligned(16))) double sm[kRing * kBufD];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nmb = (g.L + 63) >> 6;
+  const int X = blockIdx.x / nmb, mb = blockIdx.x % nmb;
+  const int Np = min(g.N, g.L);
+  const int x0 = X * kXG, y0 = blockIdx.y * 4;
+  if (y0 >= Np || x0 + kXG - 1 < y0 || x0 >= g.L) return;  // whole workgroup
+  const int y = y0 + w;
+  const bool yv = y < Np;
+  const int mpb = mb * 64;
 
-  // staging map: thread -> (column, two consecutive m')
-  const int col = tid >> 5, pr = tid & 31;
-  const int colv = col < kTile ? m0 + col : n0 + (col - kTile);
-  auto fetch = [&](int l) -> double2 {
-    const int rowlen = (l + 64) & ~63;
-    const int cc = min(colv, l);
-    const double *src = Dinv + inv_off[l] + (int64_t)cc * rowlen + mb * 64 + 2 * pr;
-    return *(const double2 *)src;
-  };
-  const int lbeg = max(max(g.L0, mb * 64), lt);
-  double2 nxt = make_double2(0.0, 0.0);
-  if (lbeg < g.L) nxt = fetch(lbeg);
-  int buf = 0;
-  for (int l = lbeg; l < g.L; ++l) {
-    sD[buf][col][pr] = nxt;
-    __syncthreads();
-    if (l + 1 < g.L) nxt = fetch(l + 1);
-    const double *col_d = (const double *)&sD[buf][0][0];
-    double dm[2], dn[2];
+  bool pv[kXG], pp[kXG];
+  int pmax[kXG];
 #pragma unroll
-    for (int a = 0; a < 2; ++a) dm[a] = col_d[(2 * (w & 1) + a) * 64 + lane];
-#pragma unroll
-    for (int b = 0; b < 2; ++b) dn[b] = col_d[(kTile + 2 * (w >> 1) + b) * 64 + lane];
-    const int64_t li = l - lt;
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        if (gofs[a][b] < 0) continue;
-        const double wgt = dm[a] * dn[b];
-        const int64_t q = (gofs[a][b] + li) * C;
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-          const double2 v = ld2(G, q + c);
-          acc[a][b][c].x += wgt * v.x;
-          acc[a][b][c].y += wgt * v.y;
-        }
-      }
-    buf ^= 1;
+  for (int a = 0; a < kXG; ++a) {
+    const int x = x0 + a;
+    pv[a] = yv && x < g.L && x >= y;
+    pp[a] = pv[a] && x < Np && x != y;
+    pmax[a] = max(x, y);
   }
 
-  if (mp >= g.L) return;
+  // ---- copy streams of this wavefront (per lane: source, lcmin, validity)
+  const int e2 = 2 * (lane & 31);
+  const int th = lane >> 5;  // tile within a DX / DY copy instruction
+  const double *dsrc[4];     // DX (j = 0, 1), DY (j = 2, 3): base so that chunk lc is at + 64 lc
+  int dlc[4];                // first valid chunk, INT_MAX if never valid
 #pragma unroll
-  for (int a = 0; a < 2; ++a) {
-    const int mi = am0 + a;
-    if (mi >= g.L) continue;
+  for (int j = 0; j < 4; ++j) {
+    const int t = 2 * (j & 1) + th;
+    const int mp0 = mpb + 16 * t;
+    const int c = j < 2 ? x0 + w : y;
+    const bool ok = (j < 2 ? c < g.L : yv) && mp0 < g.L;
+    const int cc = min(c, g.L - 1), mpc = min(mp0, g.L - 1);
+    const int lcmin = max(cc, mpc) >> 2;
+    dsrc[j] = ti + offi[(int64_t)cc * nmt + (mpc >> 4)] - 64 * lcmin + e2;
+    dlc[j] = ok ? lcmin : 0x7fffffff;
+  }
+  const double *csrc[St::kCvI];
+  bool cok[St::kCvI];
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      const int nj = an0 + b;
-      if (nj >= g.N) continue;
+  for (int j = 0; j < St::kCvI; ++j) {
+    const int lpr = St::kRegD / 2;                // lanes per region
+    const int rho = j * (64 / lpr) + lane / lpr;  // region = 2 a + h
+    const int a = rho >> 1, h = rho & 1;
+    const int x = x0 + a;
+    const bool ok = h == 0 ? pv[a] : pp[a];
+    const int l0p = max(x & ~3, y & ~3);
+    const int64_t o = ok ? (h == 0 ? poff[(int64_t)y * g.L + x] : poff[(int64_t)x * g.L + y]) : 0;
+    // rows l of the stream start at (o + l - l0p) * 2C
+    csrc[j] = Gi + (o - l0p) * (2 * C) + 2 * (lane % lpr);
+    cok[j] = ok;
+  }
+  auto issue = [&](int l0, double *buf) {
+    const int lc = l0 >> 2;
 #pragma unroll
-      for (int c = 0; c < C; ++c) {
+    for (int j = 0; j < 4; ++j) {
+      const double *src = lc >= dlc[j] ? dsrc[j] + 64 * lc : zero + e2;
+      const int row = j < 2 ? 4 * w + 2 * j : 1024 / 64 + 4 * w + 2 * (j - 2);
+      dma16(src, buf + 64 * row);
+    }
+#pragma unroll
+    for (int j = 0; j < St::kCvI; ++j) {
+      const double *src = cok[j] ? csrc[j] + (int64_t)l0 * (2 * C) : zero + 2 * (lane % (St::kRegD / 2));
+      dma16(src, buf + 2048 + w * 8 * St::kRegD + j * 128);
+    }
+  };
+
+  // ---- operand reads of this lane
+  bool used;
+  const int col = coef_col<C>(lane & 15, used);
+  const int h = ((lane & 15) & 3) >> 1;
+  const int k = lane >> 4;
+  const int cvo = 2048 + (w * 8 + h) * St::kRegD + k * (2 * C) + col;  // + 2 a kRegD
+
+  d4 acc[kXG][kNT];
+#pragma unroll
+  for (int a = 0; a < kXG; ++a)
+#pragma unroll
+    for (int t = 0; t < kNT; ++t) acc[a][t] = d4{0.0, 0.0, 0.0, 0.0};
+
+  const int lend = (g.L + 3) & ~3;
+  const int lbeg = max(max(y0, x0), mpb) & ~3;
+  const int nch = (lend - lbeg) >> 2;
+  for (int r = 0; r < kRing - 1 && r < nch; ++r) issue(lbeg + 4 * r, sm + r * kBufD);
+  for (int q = 0; q < nch; ++q) {
+    if (q + kRing - 1 < nch) issue(lbeg + 4 * (q + kRing - 1), sm + ((q + kRing - 1) % kRing) * kBufD);
+    wait_ring(min(nch - 1 - q, kRing - 1), St::kInstr);
+    // this wavefront's copies of chunk q have landed; the barrier makes
+    // everyone's visible (no fence: it would wait for the chunks in flight)
+    __builtin_amdgcn_s_barrier();
+    const double *buf = sm + (q % kRing) * kBufD;
+    const int l0 = lbeg + 4 * q;
+    double cv[kXG], dy[kNT];
+#pragma unroll
+    for (int a = 0; a < kXG; ++a) cv[a] = used ? buf[cvo + 2 * a * St::kRegD] : 0.0;
+#pragma unroll
+    for (int t = 0; t < kNT; ++t) dy[t] = buf[1024 + (4 * w + t) * 64 + lane];
+#pragma unroll
+    for (int t = 0; t < kNT; ++t) {
+      const int mp0 = mpb + 16 * t;
+      if (mp0 >= g.L || mp0 > l0 + 3) continue;
+#pragma unroll
+      for (int a = 0; a < kXG; ++a) acc[a][t] = mma(cv[a], buf[(4 * a + t) * 64 + lane] * dy[t], acc[a][t]);
+    }
+    // every read of this buffer has returned (its values fed the matrix
+    // instructions) before any wavefront refills it
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+    __builtin_amdgcn_s_barrier();
+  }
+  if (!yv) return;
+
+  // store: lane holds m' = mp0 + (lane & 15), half q >> 1, registers j
+  const int q = lane >> 4, hh = q >> 1;
+#pragma unroll
+  for (int t = 0; t < kNT; ++t) {
+    const int mp = mpb + 16 * t + (lane & 15);
+    if (mp >= g.L) continue;
+#pragma unroll
+    for (int a = 0; a < kXG; ++a) {
+      if (!(hh == 0 ? pv[a] : pp[a])) continue;
+      const int x = x0 + a;
+      const int mi = hh == 0 ? x : y, nj = hh == 0 ? y : x;
+#pragma unroll
+      for (int e = 0; e < (C == 4 ? 2 : 1); ++e) {
+        const int c = C == 4 ? 2 * (q & 1) + e : (q & 1);
         const int sm = c & 1, sn = c >> 1;
         if (sm && mi == 0) continue;
         if (sn && nj == 0) continue;
         const int m = sm ? -mi : mi, n = sn ? -nj : nj;
-        double2 v = acc[a][b][c];
+        double2 v = make_double2(acc[a][t][2 * e], acc[a][t][2 * e + 1]);
         if ((sm != sn) && (mp & 1)) v = make_double2(-v.x, -v.y);
         v = cmul_ipow(v, n - m);
         const int nslot = g.real ? n : (n + g.nG) % g.nG;
@@ -172,103 +291,156 @@ __global__ __launch_bounds__(256) void k_contract_inv(Geo g, const double2 *__re
   }
 }
 
+// X1 rows of |n| in [L, N): no l reaches them
+__global__ void k_zero_rows(Geo g, double2 *__restrict__ X1) {
+  const int an = g.L + blockIdx.y;
+  for (int s = 0; s < (g.real ? 1 : 2); ++s) {
+    if (s == 1 && an == 0) continue;
+    const int n = s ? -an : an;
+    const int nslot = g.real ? n : (n + g.nG) % g.nG;
+    double2 *row = X1 + (int64_t)nslot * g.nA * g.L;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (int64_t)g.nA * g.L;
+         i += (int64_t)gridDim.x * blockDim.x)
+      row[i] = make_double2(0.0, 0.0);
+  }
+}
+
 // ------------------------------------------------------------------ forward
+// Workgroup: x group x0 .. x0+3, y0 + w for wavefront w, l block lo .. lo+63
+// (four 16-wide tiles); reduction over m' in chunks of 4.
 template <int NS>
-__global__ __launch_bounds__(256) void k_contract_fwd(Geo g, const double2 *__restrict__ Gp,
+__global__ __launch_bounds__(256) void k_contract_fwd(Geo g, const double *__restrict__ Gp,
                                                       const int64_t *__restrict__ nbase,
-                                                      const double *__restrict__ Dfwd,
-                                                      const int64_t *__restrict__ fwd_off,
+                                                      const double *__restrict__ tf,
+                                                      const int64_t *__restrict__ offf,
+                                                      const double *__restrict__ zero,
                                                       double2 *__restrict__ flmn) {
   constexpr int C = 2 * NS;
-  __shared__ double2 sD[2][kCols][32];
-  const int tid = threadIdx.x;
-  // the wavefront index is uniform: tell the compiler, so that everything
-  // derived from it lives in scalar registers and feeds scalar loads
-  const int lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int lbn = (g.L + 63) >> 6;
-  const int lb = blockIdx.x % lbn;
-  const int mt = blockIdx.x / lbn;
-  const int m0 = mt * kTile, n0 = blockIdx.y * kTile;
+  using St = Stage<C>;
+  __shared__ __attribute__((aligned(16))) double sm[kRing * kBufD];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nlb = (g.L + 63) >> 6;
+  const int X = blockIdx.x / nlb, lb = blockIdx.x % nlb;
+  const int Np = min(g.N, g.L);
+  const int x0 = X * kXG, y0 = blockIdx.y * 4;
   const int lo = lb * 64;
-  if (max(m0, n0) > lo + 63) return;  // no l of this block reaches the tile
-  const int l = lo + lane;
-  const int am0 = m0 + 2 * (w & 1), an0 = n0 + 2 * (w >> 1);
-  cdouble *G = (cdouble *)Gp;
-  double2 acc[2][2][C];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int c = 0; c < C; ++c) acc[a][b][c] = make_double2(0.0, 0.0);
-  bool pv[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) pv[a][b] = (am0 + a < g.L) && (an0 + b < g.N) && (an0 + b < g.L);
+  if (y0 >= Np || x0 + kXG - 1 < y0 || x0 >= g.L || max(x0, y0) > lo + 63) return;
+  const int y = y0 + w;
+  const bool yv = y < Np;
 
-  const int col = tid >> 5, pr = tid & 31;
-  const int colv = col < kTile ? m0 + col : n0 + (col - kTile);
-  const bool colok = colv < g.L && (col < kTile || colv < g.N);
-  // Delta^l_{m', colv} for l = lo + 2 pr + {0, 1}
-  auto fetch = [&](int mp) -> double2 {
-    const int mx = max(mp, colv);
-    if (!colok || mx > lo + 63) return make_double2(0.0, 0.0);
-    const int64_t base = fwd_off[(int64_t)mp * g.L + colv] - (mx & ~63);
-    const int l0 = lo + 2 * pr;
-    const int la = min(l0, g.L - 1), lbb = min(l0 + 1, g.L - 1);
-    return make_double2(Dfwd[base + la], Dfwd[base + lbb]);
-  };
-  const int mpmax = min(g.L - 1, lo + 63);
-  double2 nxt = fetch(0);
-  int buf = 0;
-  for (int mp = 0; mp <= mpmax; ++mp) {
-    sD[buf][col][pr] = nxt;
-    __syncthreads();
-    if (mp + 1 <= mpmax) nxt = fetch(mp + 1);
-    const double *col_d = (const double *)&sD[buf][0][0];
-    double dm[2], dn[2];
+  bool pv[kXG], pp[kXG];
+  int pmax[kXG];
 #pragma unroll
-    for (int a = 0; a < 2; ++a) dm[a] = col_d[(2 * (w & 1) + a) * 64 + lane];
-#pragma unroll
-    for (int b = 0; b < 2; ++b) dn[b] = col_d[(kTile + 2 * (w >> 1) + b) * 64 + lane];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        if (!pv[a][b]) continue;
-        const double wgt = dm[a] * dn[b];
-        const int64_t q = ((((int64_t)(an0 + b) * g.L + (am0 + a)) * g.L) + mp) * C;
-#pragma unroll
-        for (int c = 0; c < C; ++c) {
-          const double2 v = ld2(G, q + c);
-          acc[a][b][c].x += wgt * v.x;
-          acc[a][b][c].y += wgt * v.y;
-        }
-      }
-    buf ^= 1;
+  for (int a = 0; a < kXG; ++a) {
+    const int x = x0 + a;
+    pv[a] = yv && x < g.L && x >= y;
+    pp[a] = pv[a] && x < Np && x != y;
+    pmax[a] = max(x, y);
   }
 
-  if (l >= g.L || l < g.L0) return;
-  const int64_t ll = (int64_t)l * l + l;
+  // ---- copy streams of this wavefront
+  const int e2 = 2 * (lane & 31);
+  const int th = lane >> 5;
+  const double *dsrc[4];  // DX (j = 0, 1), DY (j = 2, 3): chunk m0 / 4 at + 16 m0
 #pragma unroll
-  for (int a = 0; a < 2; ++a) {
-    const int mi = am0 + a;
-    if (!(mi < g.L) || mi > l) continue;
+  for (int j = 0; j < 4; ++j) {
+    const int t = 2 * (j & 1) + th;
+    const int l0 = lo + 16 * t;
+    const int c = j < 2 ? x0 + w : y;
+    const bool ok = (j < 2 ? c < g.L : yv) && l0 < g.L && c <= l0 + 15;
+    dsrc[j] = ok ? tf + offf[c] - 256 * tri(c >> 4) + 256 * tri(l0 >> 4) + e2 : nullptr;
+  }
+  const double *csrc[St::kCvI];
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      const int nj = an0 + b;
-      if (!pv[a][b] || nj > l) continue;
+  for (int j = 0; j < St::kCvI; ++j) {
+    const int lpr = St::kRegD / 2;
+    const int rho = j * (64 / lpr) + lane / lpr;
+    const int a = rho >> 1, h = rho & 1;
+    const int x = x0 + a;
+    const bool ok = h == 0 ? pv[a] : pp[a];
+    const int mi = h == 0 ? x : y, nj = h == 0 ? y : x;
+    csrc[j] = ok ? Gp + (((int64_t)nj * g.L + mi) * g.L) * (2 * C) + 2 * (lane % lpr) : nullptr;
+  }
+  auto issue = [&](int m0, double *buf) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const double *src = dsrc[j] ? dsrc[j] + 16 * m0 : zero + e2;
+      const int row = j < 2 ? 4 * w + 2 * j : 1024 / 64 + 4 * w + 2 * (j - 2);
+      dma16(src, buf + 64 * row);
+    }
+#pragma unroll
+    for (int j = 0; j < St::kCvI; ++j) {
+      const double *src = csrc[j] ? csrc[j] + (int64_t)m0 * (2 * C) : zero + 2 * (lane % (St::kRegD / 2));
+      dma16(src, buf + 2048 + w * 8 * St::kRegD + j * 128);
+    }
+  };
+
+  bool used;
+  const int col = coef_col<C>(lane & 15, used);
+  const int h = ((lane & 15) & 3) >> 1;
+  const int k = lane >> 4;
+  const int cvo = 2048 + (w * 8 + h) * St::kRegD + k * (2 * C) + col;
+
+  d4 acc[kXG][kNT];
+#pragma unroll
+  for (int a = 0; a < kXG; ++a)
+#pragma unroll
+    for (int t = 0; t < kNT; ++t) acc[a][t] = d4{0.0, 0.0, 0.0, 0.0};
+
+  const int nch = (min(lo + 63, g.L - 1) >> 2) + 1;
+  for (int r = 0; r < kRing - 1 && r < nch; ++r) issue(4 * r, sm + r * kBufD);
+  for (int q = 0; q < nch; ++q) {
+    if (q + kRing - 1 < nch) issue(4 * (q + kRing - 1), sm + ((q + kRing - 1) % kRing) * kBufD);
+    wait_ring(min(nch - 1 - q, kRing - 1), St::kInstr);
+    // this wavefront's copies of chunk q have landed; the barrier makes
+    // everyone's visible (no fence: it would wait for the chunks in flight)
+    __builtin_amdgcn_s_barrier();
+    const double *buf = sm + (q % kRing) * kBufD;
+    const int m0 = 4 * q;
+    double cv[kXG], dy[kNT];
+#pragma unroll
+    for (int a = 0; a < kXG; ++a) cv[a] = used ? buf[cvo + 2 * a * St::kRegD] : 0.0;
+#pragma unroll
+    for (int t = 0; t < kNT; ++t) dy[t] = buf[1024 + (4 * w + t) * 64 + lane];
+#pragma unroll
+    for (int t = 0; t < kNT; ++t) {
+      const int l0 = lo + 16 * t;
+      if (l0 >= g.L || m0 > l0 + 15) continue;
+#pragma unroll
+      for (int a = 0; a < kXG; ++a) acc[a][t] = mma(cv[a], buf[(4 * a + t) * 64 + lane] * dy[t], acc[a][t]);
+    }
+    // every read of this buffer has returned (its values fed the matrix
+    // instructions) before any wavefront refills it
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+    __builtin_amdgcn_s_barrier();
+  }
+  if (!yv) return;
+
+  // store: lane holds l = l0 + (lane & 15), half q >> 1, registers j
+  const int q = lane >> 4, hh = q >> 1;
+#pragma unroll
+  for (int t = 0; t < kNT; ++t) {
+    const int l = lo + 16 * t + (lane & 15);
+    if (l >= g.L || l < g.L0) continue;
+    const int64_t ll = (int64_t)l * l + l;
+#pragma unroll
+    for (int a = 0; a < kXG; ++a) {
+      if (!(hh == 0 ? pv[a] : pp[a])) continue;
+      const int x = x0 + a;
+      const int mi = hh == 0 ? x : y, nj = hh == 0 ? y : x;
+      if (mi > l || nj > l) continue;
       if (g.nmodeL && nj != l) continue;
 #pragma unroll
-      for (int c = 0; c < C; ++c) {
+      for (int e = 0; e < (C == 4 ? 2 : 1); ++e) {
+        const int c = C == 4 ? 2 * (q & 1) + e : (q & 1);
         const int sm = c & 1, sn = c >> 1;
         if (sm && mi == 0) continue;
         if (sn && nj == 0) continue;
         const int m = sm ? -mi : mi, n = sn ? -nj : nj;
         const int64_t bb = nbase[n + g.N - 1];
         if (bb == kNoBlock) continue;
-        double2 v = acc[a][b][c];
+        double2 v = make_double2(acc[a][t][2 * e], acc[a][t][2 * e + 1]);
         if ((sm != sn) && (l & 1)) v = make_double2(-v.x, -v.y);
         if (g.clmode == 2) v = cscale(v, (2.0 * l + 1.0) * kInv8Pi2);
         flmn[bb + ll + m] = cmul_ipow(v, m - n);
@@ -285,7 +457,7 @@ int64_t pack_inv_size(const Geo &g, int64_t *poff_host) {
     for (int am = 0; am < g.L; ++am) {
       if (poff_host) poff_host[(int64_t)an * g.L + am] = acc;
       const int l0 = std::max(an & ~3, am & ~3);
-      acc += std::max(0, g.L - l0);
+      acc += std::max(0, ((g.L + 3) & ~3) - l0);
     }
   return acc * 2 * g.NS;  // double2 elements
 }
@@ -294,23 +466,29 @@ void launch_contract_inv(const Geo &g, const double2 *flmn, NBase nb, const DlTa
                          const int64_t *poff, double2 *Gi, double2 *X1, hipStream_t st) {
   k_pack_inv<<<dim3(g.L, g.N), 256, 0, st>>>(g, flmn, nb.base, poff, Gi);
   SO3_LAUNCH_CHECK();
-  const int mpb = (g.L + 63) / 64;
-  dim3 grid(mpb * ((g.L + kTile - 1) / kTile), (g.N + kTile - 1) / kTile);
+  const int Np = std::min(g.N, g.L);
+  const int nmb = (g.L + 63) / 64;
+  dim3 grid(nmb * ((g.L + kXG - 1) / kXG), (Np + 3) / 4);
   if (g.NS == 2)
-    k_contract_inv<2><<<grid, 256, 0, st>>>(g, Gi, poff, dl.inv, dl.inv_off, X1);
+    k_contract_inv<2><<<grid, 256, 0, st>>>(g, (const double *)Gi, poff, dl.ti, dl.offi, dl.nmt, zero_page(), X1);
   else
-    k_contract_inv<1><<<grid, 256, 0, st>>>(g, Gi, poff, dl.inv, dl.inv_off, X1);
+    k_contract_inv<1><<<grid, 256, 0, st>>>(g, (const double *)Gi, poff, dl.ti, dl.offi, dl.nmt, zero_page(), X1);
   SO3_LAUNCH_CHECK();
+  if (g.N > g.L) {
+    k_zero_rows<<<dim3(64, g.N - g.L), 256, 0, st>>>(g, X1);
+    SO3_LAUNCH_CHECK();
+  }
 }
 
 void launch_contract_fwd(const Geo &g, const double2 *Gp, NBase nb, const DlTables &dl,
                          double2 *flmn, hipStream_t st) {
-  const int lbn = (g.L + 63) / 64;
-  dim3 grid(lbn * ((g.L + kTile - 1) / kTile), (g.N + kTile - 1) / kTile);
+  const int Np = std::min(g.N, g.L);
+  const int nlb = (g.L + 63) / 64;
+  dim3 grid(nlb * ((g.L + kXG - 1) / kXG), (Np + 3) / 4);
   if (g.NS == 2)
-    k_contract_fwd<2><<<grid, 256, 0, st>>>(g, Gp, nb.base, dl.fwd, dl.fwd_off, flmn);
+    k_contract_fwd<2><<<grid, 256, 0, st>>>(g, (const double *)Gp, nb.base, dl.tf, dl.offf, zero_page(), flmn);
   else
-    k_contract_fwd<1><<<grid, 256, 0, st>>>(g, Gp, nb.base, dl.fwd, dl.fwd_off, flmn);
+    k_contract_fwd<1><<<grid, 256, 0, st>>>(g, (const double *)Gp, nb.base, dl.tf, dl.offf, zero_page(), flmn);
   SO3_LAUNCH_CHECK();
 }
 

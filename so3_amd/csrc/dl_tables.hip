@@ -12,6 +12,7 @@
 // The reference obtains the same tables from SSHT
 // (so3_core.c:983-1035, ssht_dl_beta_risbo_eighth_table /
 // ssht_dl_halfpi_trapani_eighth_table).
+#include <algorithm>
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -24,18 +25,20 @@ namespace so3g {
 namespace {
 
 struct Sink {
-  double *inv;
-  const int64_t *inv_off;
-  double *fwd;
-  const int64_t *fwd_off;
-  int L;
+  double *tf;
+  const int64_t *offf;
+  double *ti;
+  const int64_t *offi;
+  int L, nmt;
 };
 
+__host__ __device__ inline int64_t tri(int64_t k) { return k * (k + 1) / 2; }
+
+// Delta^l_{mp, c} into both operand-tile layouts (mp, c <= l)
 __device__ inline void put(const Sink &s, int l, int mp, int c, double v) {
-  const int row = (l + 64) & ~63;  // P64(l+1)
-  s.inv[s.inv_off[l] + (int64_t)c * row + mp] = v;
-  const int mx = mp > c ? mp : c;
-  s.fwd[s.fwd_off[(int64_t)mp * s.L + c] + (l - (mx & ~63))] = v;
+  s.tf[s.offf[c] + 256 * (tri(l >> 4) - tri(c >> 4)) + 64 * (mp >> 2) + (mp & 3) * 16 + (l & 15)] = v;
+  const int lcmin = max(c, mp & ~15) >> 2;
+  s.ti[s.offi[(int64_t)c * s.nmt + (mp >> 4)] + 64 * ((l >> 2) - lcmin) + (l & 3) * 16 + (mp & 15)] = v;
 }
 
 __global__ void trapani_top(double *top, int L) {
@@ -101,32 +104,33 @@ DlTables build(int L, int method) {
   DlTables T;
   T.L = L;
   T.method = method;
-  std::vector<int64_t> inv_off(L + 1), fwd_off((size_t)L * L);
+  const int nlt = (L + 15) / 16, nlc = (L + 3) / 4;
+  T.nmt = nlt;
+  std::vector<int64_t> offf(L + 1), offi((size_t)L * nlt);
   int64_t acc = 0;
-  for (int l = 0; l < L; ++l) {
-    inv_off[l] = acc;
-    acc += (int64_t)(l + 1) * ((l + 64) & ~63);
+  for (int c = 0; c < L; ++c) {
+    offf[c] = acc;
+    acc += 256 * (tri(nlt) - tri(c >> 4));
   }
-  inv_off[L] = acc;
-  const int64_t ninv = acc;
+  offf[L] = acc;
+  const int64_t nf = acc;
   acc = 0;
-  for (int mp = 0; mp < L; ++mp)
-    for (int c = 0; c < L; ++c) {
-      fwd_off[(size_t)mp * L + c] = acc;
-      const int mx = mp > c ? mp : c;
-      acc += L - (mx & ~63);
+  for (int c = 0; c < L; ++c)
+    for (int mt = 0; mt < nlt; ++mt) {
+      offi[(size_t)c * nlt + mt] = acc;
+      acc += 64 * (nlc - (std::max(c, 16 * mt) >> 2));
     }
-  const int64_t nfwd = acc;
-  SO3_HIP(hipMalloc(&T.inv, sizeof(double) * ninv));
-  SO3_HIP(hipMalloc(&T.fwd, sizeof(double) * nfwd));
-  SO3_HIP(hipMalloc(&T.inv_off, sizeof(int64_t) * (L + 1)));
-  SO3_HIP(hipMalloc(&T.fwd_off, sizeof(int64_t) * L * L));
-  SO3_HIP(hipMemset(T.inv, 0, sizeof(double) * ninv));
-  SO3_HIP(hipMemset(T.fwd, 0, sizeof(double) * nfwd));
-  SO3_HIP(hipMemcpy(T.inv_off, inv_off.data(), sizeof(int64_t) * (L + 1), hipMemcpyHostToDevice));
-  SO3_HIP(hipMemcpy(T.fwd_off, fwd_off.data(), sizeof(int64_t) * L * L, hipMemcpyHostToDevice));
-  T.bytes = sizeof(double) * (ninv + nfwd) + sizeof(int64_t) * (L + 1 + (size_t)L * L);
-  Sink s{T.inv, T.inv_off, T.fwd, T.fwd_off, L};
+  const int64_t ni = acc;
+  SO3_HIP(hipMalloc(&T.tf, sizeof(double) * nf));
+  SO3_HIP(hipMalloc(&T.ti, sizeof(double) * ni));
+  SO3_HIP(hipMalloc(&T.offf, sizeof(int64_t) * offf.size()));
+  SO3_HIP(hipMalloc(&T.offi, sizeof(int64_t) * offi.size()));
+  SO3_HIP(hipMemset(T.tf, 0, sizeof(double) * nf));
+  SO3_HIP(hipMemset(T.ti, 0, sizeof(double) * ni));
+  SO3_HIP(hipMemcpy(T.offf, offf.data(), sizeof(int64_t) * offf.size(), hipMemcpyHostToDevice));
+  SO3_HIP(hipMemcpy(T.offi, offi.data(), sizeof(int64_t) * offi.size(), hipMemcpyHostToDevice));
+  T.bytes = sizeof(double) * (nf + ni) + sizeof(int64_t) * (offf.size() + offi.size());
+  Sink s{T.tf, T.offf, T.ti, T.offi, L, nlt};
   hipEvent_t e0, e1;
   SO3_HIP(hipEventCreate(&e0));
   SO3_HIP(hipEventCreate(&e1));
@@ -177,6 +181,7 @@ DlTables build(int L, int method) {
 
 std::mutex g_mu;
 std::map<std::tuple<int, int, int>, DlTables> g_cache;
+std::map<int, double *> g_zero;
 
 }  // namespace
 
@@ -190,13 +195,28 @@ const DlTables &dl_tables(int L, int method) {
   return g_cache.emplace(key, build(L, method)).first->second;
 }
 
+const double *zero_page() {
+  int dev = 0;
+  SO3_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> g(g_mu);
+  auto it = g_zero.find(dev);
+  if (it != g_zero.end()) return it->second;
+  double *z = nullptr;
+  SO3_HIP(hipMalloc(&z, sizeof(double) * kZeroPage));
+  SO3_HIP(hipMemset(z, 0, sizeof(double) * kZeroPage));
+  g_zero[dev] = z;
+  return z;
+}
+
 void dl_tables_clear() {
   std::lock_guard<std::mutex> g(g_mu);
+  for (auto &kv : g_zero) (void)hipFree(kv.second);
+  g_zero.clear();
   for (auto &kv : g_cache) {
-    (void)hipFree(kv.second.inv);
-    (void)hipFree(kv.second.fwd);
-    (void)hipFree(kv.second.inv_off);
-    (void)hipFree(kv.second.fwd_off);
+    (void)hipFree(kv.second.tf);
+    (void)hipFree(kv.second.ti);
+    (void)hipFree(kv.second.offf);
+    (void)hipFree(kv.second.offi);
   }
   g_cache.clear();
 }

@@ -1,13 +1,18 @@
 // dl_tables.hpp -- device-resident Delta^l_{m'm} = d^l_{m'm}(pi/2) tables.
 //
 // Generated on the GPU in FP64 (Risbo or Trapani recursion), once per
-// (device, L, method), and kept for the life of the process.  Two layouts of
-// the quarter m', m >= 0 serve the two contraction kernels:
-//   inv: for each l and column c, a row over m' of length P64(l+1)
-//        (zero beyond l):  inv[inv_off[l] + c*P64(l+1) + m'] = Delta^l_{m',c}
-//   fwd: for each (m', c), a segment over l from lb = 64*floor(max(m',c)/64)
-//        to L-1 (zero below max(m',c)):
-//        fwd[fwd_off[m'*L + c] + (l - lb)] = Delta^l_{m',c}
+// (device, L, method), and kept for the life of the process.  The quarter
+// m', c >= 0 is stored twice, each time as ready-made operand tiles of the
+// 16x16x4 FP64 matrix instruction (64 doubles = one 512-byte load per
+// wavefront, lane i taking element i), zero where l < max(m', c) or l >= L:
+//   tf (forward, lane index l, reduction over m'), for column c, l-tile
+//      lt >= c/16 (l = 16 lt + (i & 15)) and m'-chunk mc <= 4 lt + 3
+//      (m' = 4 mc + (i >> 4)):
+//        tf[offf[c] + 256 (T(lt) - T(c >> 4)) + 64 mc + i],  T(k) = k(k+1)/2
+//   ti (inverse, lane index m', reduction over l), for column c, m'-tile mt
+//      (m' = 16 mt + (i & 15)) and l-chunk lc >= lcmin = max(c, 16 mt) / 4
+//      (l = 4 lc + (i >> 4)):
+//        ti[offi[c * nmt + mt] + 64 (lc - lcmin) + i]
 #pragma once
 #include <cstdint>
 
@@ -17,10 +22,11 @@ namespace so3g {
 
 struct DlTables {
   int L = 0, method = 0;
-  double *inv = nullptr;
-  int64_t *inv_off = nullptr;  // device, L+1 entries
-  double *fwd = nullptr;
-  int64_t *fwd_off = nullptr;  // device, L*L entries
+  int nmt = 0;                 // ceil(L / 16)
+  double *tf = nullptr;
+  int64_t *offf = nullptr;     // device, L + 1 entries
+  double *ti = nullptr;
+  int64_t *offi = nullptr;     // device, L * nmt entries
   size_t bytes = 0;
   float build_ms = 0.f;
 };
@@ -28,7 +34,12 @@ struct DlTables {
 // Cached per (device, L, method); builds on first use (synchronously).
 const DlTables &dl_tables(int L, int method);
 
-// Drop every cached table (frees device memory).
+// Drop every cached table and zero page (frees device memory).
 void dl_tables_clear();
+
+// A device page of kZeroPage zero doubles on the current device: the copy
+// source of operand regions that lie outside a transform.
+constexpr int kZeroPage = 1024;
+const double *zero_page();
 
 }  // namespace so3g

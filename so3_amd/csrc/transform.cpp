@@ -183,7 +183,9 @@ Plan build_plan(const so3_parameters_t *p, Method method) {
     SO3_HIP(hipMemcpy(d, nb.data(), sizeof(int64_t) * nb.size(), hipMemcpyHostToDevice));
     P.nbase = d;
   }
-  P.x1 = std::max<size_t>((size_t)g.nG * g.nA * L, (size_t)N * L * L * 2 * g.NS);
+  // the forward contraction copies G' in 4-row chunks: up to 3 rows past the
+  // last pair's stream
+  P.x1 = std::max<size_t>((size_t)g.nG * g.nA * L, (size_t)N * L * L * 2 * g.NS + 8 * g.NS);
   P.x2 = (size_t)g.nG * g.nB * g.nA;
   {
     std::vector<int64_t> po((size_t)N * L);
@@ -193,7 +195,9 @@ Plan build_plan(const so3_parameters_t *p, Method method) {
     SO3_HIP(hipMemcpy(d, po.data(), sizeof(int64_t) * po.size(), hipMemcpyHostToDevice));
     P.poff = d;
   }
-  // DFT plans are built now so that a timed call does not build them
+  // DFT plans and the zero page are made now so that a timed (or captured)
+  // call does not allocate
+  zero_page();
   fft_plan(g.Pb);
   fft_plan(g.nA);
   fft_plan(2 * N - 1);
