@@ -360,7 +360,10 @@ __global__ __launch_bounds__(256) void k_contract_fwd(Geo g, const double *__res
     const int x = x0 + a;
     const bool ok = h == 0 ? pv[a] : pp[a];
     const int mi = h == 0 ? x : y, nj = h == 0 ? y : x;
-    csrc[j] = ok ? Gp + (((int64_t)nj * g.L + mi) * g.L) * (2 * C) + 2 * (lane % lpr) : nullptr;
+    // G' is stored [|n|][|m|][combination][m']: lane i of the region copies
+    // row (i / C) of combination (i % C), one complex value
+    const int li = lane % lpr;
+    csrc[j] = ok ? Gp + 2 * ((((int64_t)nj * g.L + mi) * C + li % C) * g.L + li / C) : nullptr;
   }
   auto issue = [&](int m0, double *buf) {
 #pragma unroll
@@ -371,7 +374,7 @@ __global__ __launch_bounds__(256) void k_contract_fwd(Geo g, const double *__res
     }
 #pragma unroll
     for (int j = 0; j < St::kCvI; ++j) {
-      const double *src = csrc[j] ? csrc[j] + (int64_t)m0 * (2 * C) : zero + 2 * (lane % (St::kRegD / 2));
+      const double *src = csrc[j] ? csrc[j] + 2 * m0 : zero + 2 * (lane % (St::kRegD / 2));
       dma16(src, buf + 2048 + w * 8 * St::kRegD + j * 128);
     }
   };

@@ -35,6 +35,8 @@ struct FftDesc {
   const double2 *chirp;  // exp(-pi i (j^2 mod 2n) / n), j < n
   const double2 *bhat_f; // FFT_len of the forward chirp kernel
   const double2 *bhat_b; // FFT_len of the backward chirp kernel
+  const double2 *tws;    // len == 1024: twiddles of the register FFT
+                         // (fft_reg.hpp), else null
 };
 
 // floor(x / d) for 0 <= x < 2^16, 1 <= d <= 2^16 with m = floor(2^32/d) + 1;

@@ -117,6 +117,16 @@ FftDesc build(int n) {
   std::vector<cld> tw(d.len);
   for (int k = 0; k < d.len; ++k) tw[k] = root(k, d.len);
   d.tw = upload(tw);
+  if (d.len == 1024) {
+    // tws[k1 * 64 + l] = w1024^{l k1} (k1 < 16, l < 64), then
+    // tws[1024 + c * 4 + a] = w64^{a c} (c < 16, a < 4)
+    std::vector<cld> tws(1024 + 64);
+    for (int k1 = 0; k1 < 16; ++k1)
+      for (int l = 0; l < 64; ++l) tws[k1 * 64 + l] = root((long long)l * k1, 1024);
+    for (int c = 0; c < 16; ++c)
+      for (int a = 0; a < 4; ++a) tws[1024 + c * 4 + a] = root((long long)a * c, 64);
+    d.tws = upload(tws);
+  }
   return d;
 }
 

@@ -34,7 +34,7 @@ struct NBase {
 
 // Workspace layouts (double2):
 //   X1 rows  : [n_slot][m_slot][m'], m' < L          (inverse contraction out)
-//   Gp       : [|n|][|m|][m'][2*NS] sign combos       (forward contraction in)
+//   Gp       : [|n|][|m|][2*NS sign combos][m']        (forward contraction in)
 //   X2       : [n_slot][b][a or m_slot]               (beta / alpha stages)
 // Packed inverse coefficient streams: pair (|n|, |m|) at poff[|n| L + |m|]
 // (in units of 2*NS double2), covering l from max(|n| & ~3, |m| & ~3).
@@ -48,6 +48,8 @@ void launch_contract_fwd(const Geo &g, const double2 *Gp, NBase nb, const DlTabl
 void launch_inv_beta(const Geo &g, const double2 *X1, double2 *X2, NBase nb, const double2 *phase,
                      hipStream_t st);
 void launch_alpha(const Geo &g, double2 *X2, int dir, double scale, hipStream_t st);
+// nrows contiguous DFTs of length n (sign dir), results times scale
+void launch_rows(double2 *X, int64_t nrows, int n, int dir, double scale, hipStream_t st);
 void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, double scale, hipStream_t st);
 void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, double scale, hipStream_t st);
 // adjoint != 0: zero-fill instead of the beta extension and no weights (the

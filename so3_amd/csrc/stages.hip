@@ -16,6 +16,7 @@
 #include <algorithm>
 
 #include "fft_plan.hpp"
+#include "fft_reg.hpp"
 #include "kernels.hpp"
 
 namespace so3g {
@@ -86,6 +87,205 @@ __global__ __launch_bounds__(kThreads) void k_rows(FftDesc fd, int K, int ld, do
   for (int idx = threadIdx.x; idx < K * n; idx += blockDim.x) {
     const int s = idx / n, k = idx - s * n;
     if (r0 + s < nrows) X[(r0 + s) * n + k] = cscale(A[s * ld + k], scale);
+  }
+}
+
+// ---------------------------------------------------------------- rows, register path
+// Transforms whose inner length is 1024 (n = 1024, or a Bluestein n in
+// [513, 1023]): one row per wavefront in the register layouts of
+// fft_reg.hpp.
+constexpr int kRegWaves = kThreads / 64;
+constexpr size_t kRegShared = sizeof(double2) * kRegWaves * kRegLds;
+
+template <int DIR, int BLUE>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_rows_reg(
+    FftDesc fd, double2 *__restrict__ X, int64_t nrows, int n, double scale) {
+  extern __shared__ double2 lds[];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t r = (int64_t)blockIdx.x * kRegWaves + wv;
+  if (r >= nrows) return;  // the whole wavefront
+  double2 *T = lds + wv * kRegLds;
+  double2 *row = X + r * n;
+  double2 v[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int k = lane + 64 * q;
+    v[q] = k < n ? row[k] : make_double2(0.0, 0.0);
+  }
+  if constexpr (BLUE) {
+    blue_reg<DIR>(v, fd, T, lane);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int k = lane + 64 * q;
+      if (k < n) row[k] = cscale(v[q], scale);
+    }
+  } else {
+    fft_F<DIR>(v, T, fd.tws, fd.tws + 1024, lane);
+#pragma unroll
+    for (int c = 0; c < 16; ++c) row[o_index(lane, c)] = cscale(v[c], scale);
+  }
+}
+
+// ---------------------------------------------------------------- columns, register path
+// Gamma stages and the inverse beta stage with inner length 1024: a
+// workgroup of 8 wavefronts owns 8 sequences whose elements are adjacent in
+// HBM across the sequences (8 consecutive columns p of the gamma stages, 8
+// consecutive m of the beta output), moved through an LDS tile with rows of
+// 9 entries so that HBM sees 128-byte runs and LDS no bank conflicts.  The
+// tile shares LDS with the transpose buffers (barriers separate the uses).
+constexpr int kColWaves = 8;
+constexpr int kColThreads = 64 * kColWaves;
+constexpr int kTileLd = kColWaves + 1;
+constexpr size_t kColShared =
+    sizeof(double2) * (1024 * kTileLd > kColWaves * kRegLds ? 1024 * kTileLd : kColWaves * kRegLds);
+
+template <int BLUE>
+__global__ __launch_bounds__(kColThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_inv_gamma_reg(
+    Geo g, FftDesc fd, const double2 *__restrict__ X2, double2 *__restrict__ fc, double *__restrict__ fr,
+    double scale) {
+  extern __shared__ double2 lds[];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t S = (int64_t)g.nB * g.nA;
+  const int64_t p0 = (int64_t)blockIdx.x * kColWaves;
+  const int nG = 2 * g.N - 1;
+  for (int idx = threadIdx.x; idx < nG * kColWaves; idx += kColThreads) {
+    const int k = idx / kColWaves, c = idx - k * kColWaves;
+    const int64_t p = p0 + c;
+    double2 v = make_double2(0.0, 0.0);
+    if (p < S) {
+      if (!g.real) v = X2[k * S + p];
+      else if (k == 0) v = make_double2(X2[p].x, 0.0);
+      else if (k < g.N) v = X2[k * S + p];
+      else v = cconj(X2[(nG - k) * S + p]);
+    }
+    lds[k * kTileLd + c] = v;
+  }
+  __syncthreads();
+  double2 v[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int k = lane + 64 * q;
+    v[q] = k < nG ? lds[k * kTileLd + wv] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  double2 *T = lds + wv * kRegLds;
+  if constexpr (BLUE) {
+    blue_reg<+1>(v, fd, T, lane);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int k = lane + 64 * q;
+      if (k < nG) lds[k * kTileLd + wv] = v[q];
+    }
+  } else {
+    fft_F<+1>(v, T, fd.tws, fd.tws + 1024, lane);
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 16; ++c) lds[o_index(lane, c) * kTileLd + wv] = v[c];
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < nG * kColWaves; idx += kColThreads) {
+    const int k = idx / kColWaves, c = idx - k * kColWaves;
+    const int64_t p = p0 + c;
+    if (p >= S) continue;
+    const double2 v2 = lds[k * kTileLd + c];
+    if (g.real) fr[k * S + p] = v2.x * scale;
+    else fc[k * S + p] = cscale(v2, scale);
+  }
+}
+
+template <int BLUE>
+__global__ __launch_bounds__(kColThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_fwd_gamma_reg(
+    Geo g, FftDesc fd, const double2 *__restrict__ fc, const double *__restrict__ fr,
+    double2 *__restrict__ X2, double sc) {
+  extern __shared__ double2 lds[];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t S = (int64_t)g.nB * g.nA;
+  const int64_t p0 = (int64_t)blockIdx.x * kColWaves;
+  const int nG = 2 * g.N - 1;
+  for (int idx = threadIdx.x; idx < nG * kColWaves; idx += kColThreads) {
+    const int k = idx / kColWaves, c = idx - k * kColWaves;
+    const int64_t p = p0 + c;
+    double2 v = make_double2(0.0, 0.0);
+    if (p < S) v = g.real ? make_double2(fr[k * S + p], 0.0) : fc[k * S + p];
+    lds[k * kTileLd + c] = v;
+  }
+  __syncthreads();
+  double2 v[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int k = lane + 64 * q;
+    v[q] = k < nG ? lds[k * kTileLd + wv] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  double2 *T = lds + wv * kRegLds;
+  if constexpr (BLUE) {
+    blue_reg<-1>(v, fd, T, lane);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int k = lane + 64 * q;
+      if (k < nG) lds[k * kTileLd + wv] = v[q];
+    }
+  } else {
+    fft_F<-1>(v, T, fd.tws, fd.tws + 1024, lane);
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 16; ++c) lds[o_index(lane, c) * kTileLd + wv] = v[c];
+  }
+  __syncthreads();
+  const int nout = g.real ? g.N : nG;
+  for (int idx = threadIdx.x; idx < nout * kColWaves; idx += kColThreads) {
+    const int k = idx / kColWaves, c = idx - k * kColWaves;
+    const int64_t p = p0 + c;
+    if (p < S) X2[k * S + p] = cscale(lds[k * kTileLd + c], sc);
+  }
+}
+
+// inverse beta: sequence (n slot, m slot) per wavefront, 8 consecutive m
+// slots per workgroup; Pb is a Bluestein length with inner length 1024
+__global__ __launch_bounds__(kColThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_inv_beta_reg(
+    Geo g, FftDesc fd, const double2 *__restrict__ X1, double2 *__restrict__ X2,
+    const int64_t *__restrict__ nbase, const double2 *__restrict__ phase) {
+  extern __shared__ double2 lds[];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int chunks = (g.nA + kColWaves - 1) / kColWaves;
+  const int nslot = blockIdx.x / chunks;
+  const int ms0 = (blockIdx.x % chunks) * kColWaves;
+  const int n = n_of_slot(g, nslot);
+  const bool active = nbase[n + g.N - 1] != kNoBlock;
+  const int Pb = g.Pb, L = g.L;
+  const int mslot = ms0 + wv;
+  const bool mok = active && mslot < g.nA;
+  const int m = m_of_slot(g, mslot);
+  const double2 *row = X1 + ((int64_t)nslot * g.nA + mslot) * L;
+  const bool flip = ((m + n) & 1) != 0;
+  double2 v[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int k = lane + 64 * q;
+    double2 x = make_double2(0.0, 0.0);
+    if (mok && k < Pb && m >= -(L - 1)) {
+      const int mp = k <= L - 1 ? k : k - Pb;
+      if (mp >= -(L - 1)) {
+        x = row[mp < 0 ? -mp : mp];
+        if (mp < 0 && flip) x = make_double2(-x.x, -x.y);
+        x = cmul(x, phase[k]);
+      }
+    }
+    v[q] = x;
+  }
+  blue_reg<+1>(v, fd, lds + wv * kRegLds, lane);
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int b = lane + 64 * q;
+    if (b < g.nB) lds[b * kTileLd + wv] = v[q];
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < g.nB * kColWaves; idx += kColThreads) {
+    const int b = idx / kColWaves, c = idx - b * kColWaves;
+    if (ms0 + c < g.nA) X2[((int64_t)nslot * g.nB + b) * g.nA + ms0 + c] = lds[b * kTileLd + c];
   }
 }
 
@@ -250,7 +450,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_beta(Geo g, FftDesc fb, FftDes
     }
     const int am = m < 0 ? -m : m;
     const int sm = m < 0, sn = n < 0;
-    double2 *dst = Gp + (((int64_t)an * L + am) * L + mp) * C;
+    double2 *dst = Gp + ((int64_t)an * L + am) * C * L + mp;
     // m = 0 or n = 0 serve both sign combinations
     for (int cm = 0; cm < 2; ++cm) {
       if (cm != sm && am != 0) continue;
@@ -258,7 +458,98 @@ __global__ __launch_bounds__(kThreads) void k_fwd_beta(Geo g, FftDesc fb, FftDes
         if (cn != sn && an != 0) continue;
         // mixed-sign pairs carry D^l_{m',-|m|} = (-1)^{l+m'} D^l_{m',|m|}: the
         // (-1)^{m'} part is folded here, (-1)^l in the contraction
-        dst[cm + 2 * cn] = ((cm != cn) && (mp & 1)) ? make_double2(-G.x, -G.y) : G;
+        dst[(int64_t)(cm + 2 * cn) * L] = ((cm != cn) && (mp & 1)) ? make_double2(-G.x, -G.y) : G;
+      }
+    }
+  }
+}
+
+// forward beta, register path (MW: Pb Bluestein with inner length 1024,
+// Pw = 1024): sequence (n slot, m slot) per wavefront, 8 consecutive m
+// slots per workgroup
+__global__ __launch_bounds__(kColThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_fwd_beta_reg(
+    Geo g, FftDesc fb, FftDesc fw, const double2 *__restrict__ X2, double2 *__restrict__ Gp,
+    const int64_t *__restrict__ nbase, const double2 *__restrict__ phase, const double2 *__restrict__ wr,
+    int adjoint) {
+  extern __shared__ double2 lds[];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int chunks = (g.nA + kColWaves - 1) / kColWaves;
+  const int nslot = blockIdx.x / chunks;
+  const int ms0 = (blockIdx.x % chunks) * kColWaves;
+  const int n = n_of_slot(g, nslot);
+  if (nbase[n + g.N - 1] == kNoBlock) return;  // block not transformed (whole workgroup)
+  const int Pb = g.Pb, L = g.L;
+  // beta samples of the 8 sequences: tile[b][s]
+  for (int idx = threadIdx.x; idx < g.nB * kColWaves; idx += kColThreads) {
+    const int b = idx / kColWaves, c = idx - b * kColWaves;
+    lds[b * kTileLd + c] =
+        ms0 + c < g.nA ? X2[((int64_t)nslot * g.nB + b) * g.nA + ms0 + c] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  const int mslot = ms0 + wv;
+  const int m = m_of_slot(g, mslot);
+  const bool flip = ((m + n) & 1) != 0;
+  double2 v[16];
+  // extension F(2L-2-b) = (-1)^{m+n} F(b) (zero for the adjoint)
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int k = lane + 64 * q;
+    double2 x = make_double2(0.0, 0.0);
+    if (k < Pb) {
+      if (k < L) {
+        x = lds[k * kTileLd + wv];
+      } else if (!adjoint) {
+        x = lds[(2 * L - 2 - k) * kTileLd + wv];
+        if (flip) x = make_double2(-x.x, -x.y);
+      }
+    }
+    v[q] = x;
+  }
+  __syncthreads();
+  double2 *T = lds + wv * kRegLds;
+  blue_reg<-1>(v, fb, T, lane);
+  // F~(m'') times the phase (which carries 1/Pb) at m'' mod Pw, |m''| < L
+#pragma unroll
+  for (int q = 0; q < 16; ++q) T[lane + 64 * q] = v[q];
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int r = lane + 64 * q;
+    const int mpp = r <= L - 1 ? r : r - 1024;
+    double2 x = make_double2(0.0, 0.0);
+    if (mpp >= -(L - 1)) x = cmul(T[mpp < 0 ? mpp + Pb : mpp], phase[mpp + L - 1]);
+    v[q] = x;
+  }
+  __builtin_amdgcn_wave_barrier();
+  double sc = 1.0;
+  if (!adjoint) {
+    // quadrature-weight convolution as a product in the conjugate domain
+    fft_F<+1>(v, T, fw.tws, fw.tws + 1024, lane);
+#pragma unroll
+    for (int c = 0; c < 16; ++c) v[c] = cmul(v[c], wr[o_index(lane, c)]);
+    fft_G<-1>(v, T, fw.tws, fw.tws + 1024, lane);
+    sc = kTwoPi / 1024.0;
+  }
+  // fold to m' >= 0: G'(m') = G(m') + (-1)^{m+n} G(-m')
+#pragma unroll
+  for (int q = 0; q < 16; ++q) T[lane + 64 * q] = v[q];
+  __builtin_amdgcn_wave_barrier();
+  if (mslot >= g.nA || m < -(L - 1)) return;  // MW_SS Nyquist column / tail (whole wavefront)
+  const int C = 2 * g.NS;
+  const int an = n < 0 ? -n : n, am = m < 0 ? -m : m;
+  const int sm = m < 0, sn = n < 0;
+  double2 *dst = Gp + ((int64_t)an * L + am) * C * L;
+  for (int mp = lane; mp < L; mp += 64) {
+    double2 G0 = cscale(T[mp], sc);
+    if (mp > 0) {
+      const double2 Gn = cscale(T[1024 - mp], sc);
+      G0 = flip ? csub(G0, Gn) : cadd(G0, Gn);
+    }
+    for (int cm = 0; cm < 2; ++cm) {
+      if (cm != sm && am != 0) continue;
+      for (int cn = 0; cn < g.NS; ++cn) {
+        if (cn != sn && an != 0) continue;
+        dst[(int64_t)(cm + 2 * cn) * L + mp] = ((cm != cn) && (mp & 1)) ? make_double2(-G0.x, -G0.y) : G0;
       }
     }
   }
@@ -408,6 +699,13 @@ static void set_lds(Kern k, size_t bytes) {
 void launch_inv_beta(const Geo &g, const double2 *X1, double2 *X2, NBase nb, const double2 *phase,
                      hipStream_t st) {
   const FftDesc fd = fft_plan(g.Pb);
+  if (fd.len == 1024 && fd.blue && g.nB <= 1024) {
+    set_lds(k_inv_beta_reg, kColShared);
+    const int chunks = (g.nA + kColWaves - 1) / kColWaves;
+    k_inv_beta_reg<<<g.nG * chunks, kColThreads, kColShared, st>>>(g, fd, X1, X2, nb.base, phase);
+    SO3_LAUNCH_CHECK();
+    return;
+  }
   const int ld = fft_ld(fd);
   const int tb = fd.len * (int)sizeof(double2);
   const int K = choose_k(ld, fft_capacity(fd), fd.len, tb, 8);
@@ -418,26 +716,55 @@ void launch_inv_beta(const Geo &g, const double2 *X1, double2 *X2, NBase nb, con
   SO3_LAUNCH_CHECK();
 }
 
-void launch_alpha(const Geo &g, double2 *X2, int dir, double scale, hipStream_t st) {
-  const FftDesc fd = fft_plan(g.nA);
+void launch_rows(double2 *X, int64_t nrows, int n, int dir, double scale, hipStream_t st) {
+  const FftDesc fd = fft_plan(n);
+  if (nrows <= 0) return;
+  if (fd.len == 1024) {
+    const unsigned grid = (unsigned)((nrows + kRegWaves - 1) / kRegWaves);
+    auto go = [&](auto kern) {
+      set_lds(kern, kRegShared);
+      kern<<<grid, kThreads, kRegShared, st>>>(fd, X, nrows, n, scale);
+    };
+    if (fd.blue)
+      dir > 0 ? go(k_rows_reg<+1, 1>) : go(k_rows_reg<-1, 1>);
+    else
+      dir > 0 ? go(k_rows_reg<+1, 0>) : go(k_rows_reg<-1, 0>);
+    SO3_LAUNCH_CHECK();
+    return;
+  }
   const int ld = fft_ld(fd);
   const int tb = fd.len * (int)sizeof(double2);
   const int K = choose_k(ld, fft_capacity(fd), fd.len, tb, 8);
   const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
-  const int64_t nrows = (int64_t)g.nG * g.nB;
   const unsigned grid = (unsigned)((nrows + K - 1) / K);
   if (dir > 0) {
     set_lds(k_rows<+1>, sh);
-    k_rows<+1><<<grid, kThreads, sh, st>>>(fd, K, ld, X2, nrows, g.nA, scale);
+    k_rows<+1><<<grid, kThreads, sh, st>>>(fd, K, ld, X, nrows, n, scale);
   } else {
     set_lds(k_rows<-1>, sh);
-    k_rows<-1><<<grid, kThreads, sh, st>>>(fd, K, ld, X2, nrows, g.nA, scale);
+    k_rows<-1><<<grid, kThreads, sh, st>>>(fd, K, ld, X, nrows, n, scale);
   }
   SO3_LAUNCH_CHECK();
 }
 
+void launch_alpha(const Geo &g, double2 *X2, int dir, double scale, hipStream_t st) {
+  launch_rows(X2, (int64_t)g.nG * g.nB, g.nA, dir, scale, st);
+}
+
 void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, double scale, hipStream_t st) {
   const FftDesc fd = fft_plan(2 * g.N - 1);
+  if (fd.len == 1024) {
+    const int64_t S = (int64_t)g.nB * g.nA;
+    const unsigned grid = (unsigned)((S + kColWaves - 1) / kColWaves);
+    auto go = [&](auto kern) {
+      set_lds(kern, kColShared);
+      kern<<<grid, kColThreads, kColShared, st>>>(g, fd, X2, g.real ? nullptr : (double2 *)f,
+                                                   g.real ? (double *)f : nullptr, scale);
+    };
+    fd.blue ? go(k_inv_gamma_reg<1>) : go(k_inv_gamma_reg<0>);
+    SO3_LAUNCH_CHECK();
+    return;
+  }
   const int ld = fft_ld(fd);
   const int tb = fd.len * (int)sizeof(double2);
   const int K = choose_k(ld, fft_capacity(fd), fd.len, tb, 16);
@@ -451,6 +778,18 @@ void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, double scale, hi
 
 void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, double scale, hipStream_t st) {
   const FftDesc fd = fft_plan(2 * g.N - 1);
+  if (fd.len == 1024) {
+    const int64_t S = (int64_t)g.nB * g.nA;
+    const unsigned grid = (unsigned)((S + kColWaves - 1) / kColWaves);
+    auto go = [&](auto kern) {
+      set_lds(kern, kColShared);
+      kern<<<grid, kColThreads, kColShared, st>>>(g, fd, g.real ? nullptr : (const double2 *)f,
+                                                   g.real ? (const double *)f : nullptr, X2, scale);
+    };
+    fd.blue ? go(k_fwd_gamma_reg<1>) : go(k_fwd_gamma_reg<0>);
+    SO3_LAUNCH_CHECK();
+    return;
+  }
   const int ld = fft_ld(fd);
   const int tb = fd.len * (int)sizeof(double2);
   const int K = choose_k(ld, fft_capacity(fd), fd.len, tb, 16);
@@ -464,6 +803,15 @@ void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, double scale, hi
 
 void launch_fwd_beta(const Geo &g, const double2 *X2, double2 *Gp, NBase nb, const double2 *phase,
                      const double2 *wr, int Pw, int adjoint, hipStream_t st) {
+  if (!g.ss && Pw == 1024 && fft_plan(g.Pb).len == 1024 && fft_plan(g.Pb).blue && g.nB <= 1024) {
+    const FftDesc fb = fft_plan(g.Pb), fw = fft_plan(Pw);
+    set_lds(k_fwd_beta_reg, kColShared);
+    const int chunks = (g.nA + kColWaves - 1) / kColWaves;
+    k_fwd_beta_reg<<<g.nG * chunks, kColThreads, kColShared, st>>>(g, fb, fw, X2, Gp, nb.base, phase, wr,
+                                                                    adjoint);
+    SO3_LAUNCH_CHECK();
+    return;
+  }
   const FftDesc fb = fft_plan(g.Pb);
   const FftDesc fw = fft_plan(Pw);
   const int ld = std::max(fft_ld(fb), fft_ld(fw));

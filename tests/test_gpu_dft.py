@@ -9,7 +9,7 @@ from so3_amd._lib import lib
 
 pytestmark = pytest.mark.gpu
 
-LENGTHS = [1, 2, 3, 4, 5, 7, 8, 9, 11, 13, 15, 16, 17, 31, 61, 64, 73, 127, 255, 511, 512,
+LENGTHS = [1, 2, 3, 4, 5, 7, 8, 9, 11, 13, 15, 16, 17, 31, 61, 64, 73, 127, 255, 511, 512, 600,
            767, 1021, 1023, 1024, 1537, 2047, 4096]
 
 
