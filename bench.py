@@ -10,7 +10,9 @@ is K steps on one stream, bracketed by synchronisation.
 
 Prints one JSON line: value = fwd+inv pairs per second, with p50/p99 step
 latency, the round-trip error, a roofline entry for the dominant kernel
-(timed live with HIP events by the library's stage profiler) and the CPU
+(timed live with HIP events on the launch stream by the library's stage
+profiler; algorithmic bytes for the DFT stages, flops for the
+contractions) and one per stage and the CPU
 baseline (the oracle's single-thread restatement of the reference
 algorithm, timed on a bounded sample of the same workload).
 """
@@ -40,6 +42,51 @@ def contraction_terms(L, N, L0=0, real=False):
     return T
 
 
+def stage_model(p, A, T):
+    """Algorithmic work of each pipeline stage of one fwd+inv pair: bytes that
+    must cross HBM (read + write once) for the DFT stages, FP64 flops for the
+    contractions (5 per Delta-product term)."""
+    L, N, real = p.L, p.N, bool(p.reality)
+    ss = p.sampling_scheme == 1
+    nA, nB = (2 * L, L + 1) if ss else (2 * L - 1, L)
+    nG = N if real else 2 * N - 1          # n slots of the complex workspaces
+    C = 2 if real else 4                   # sign combinations per (|m|, |n|)
+    esz = 8 if real else 16
+    x1 = 16 * nG * nA * L                  # contraction output rows
+    x2 = 16 * nG * nB * nA                 # beta / alpha workspace
+    gp = 16 * N * L * L * C                # folded G' of the forward contraction
+    fb = esz * A.f_size(p)
+    return {
+        "contract_inv": ("mfma", 5.0 * T),
+        "beta_inv": ("hbm", x1 + x2),
+        "alpha_inv": ("hbm", 2 * x2),
+        "gamma_inv": ("hbm", x2 + fb),
+        "gamma_fwd": ("hbm", fb + x2),
+        "alpha_fwd": ("hbm", 2 * x2),
+        "beta_fwd": ("hbm", x2 + gp),
+        "contract_fwd": ("mfma", 5.0 * T),
+    }
+
+
+# kernel that carries each stage (names as rocprofv3 reports them)
+STAGE_KERNEL = {"contract_inv": "k_contract_inv", "beta_inv": "k_inv_beta", "alpha_inv": "k_rows",
+                "gamma_inv": "k_inv_gamma", "gamma_fwd": "k_fwd_gamma", "alpha_fwd": "k_rows",
+                "beta_fwd": "k_fwd_beta", "contract_fwd": "k_contract_fwd"}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_l256.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 3 and not line.startswith("#") and parts[0].startswith(kernel):
+            return float(parts[1]) + float(parts[2])
+    return None
+
+
 def cpu_baseline(p, n_slices):
     """Single-thread oracle time of one fwd+inv pair, from a bounded sample:
     n_slices evenly spaced n slices timed in full (spin-SHT inverse and
@@ -64,6 +111,84 @@ def cpu_baseline(p, n_slices):
     }
 
 
+def batched_workload(P, seed=4, count=96):
+    """The uneven batch: L uniform on {32, 48, ..., 384}, N uniform on
+    [1, min(L, 64)], MW / MW_SS and complex / real 50/50 each."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        L = int(rng.choice(np.arange(32, 385, 16)))
+        N = int(rng.integers(1, min(L, 64) + 1))
+        ss = int(rng.integers(0, 2))
+        real = int(rng.integers(0, 2))
+        out.append(P.SO3Parameters(L=L, N=N, sampling_scheme=ss, reality=real,
+                                   n_order=P.SO3_N_ORDER_ZERO_FIRST))
+    return out
+
+
+def batched_bench(A, P, torch, reps=5):
+    """96 uneven fwd+inv pairs: plain calls one after another on one stream
+    versus one batch call (tuned lane count) and its captured graph."""
+    dev = torch.device("cuda:0")
+    ps = batched_workload(P)
+    g = torch.Generator(device=dev)
+    g.manual_seed(4)
+    specs, keep = [], []
+    for p in ps:
+        nfl = A.flmn_size(p)
+        flmn = torch.complex(torch.rand(nfl, generator=g, device=dev, dtype=torch.float64) * 2 - 1,
+                             torch.rand(nfl, generator=g, device=dev, dtype=torch.float64) * 2 - 1)
+        out = torch.zeros_like(flmn)
+        f = torch.empty(A.f_size(p), dtype=torch.float64 if p.reality else torch.complex128, device=dev)
+        keep += [flmn, out, f]
+        specs.append({"p": p, "kind": A.ITEM_PAIR, "f": f.data_ptr(), "flmn_in": flmn.data_ptr(),
+                      "flmn_out": out.data_ptr()})
+    items = A.make_items(specs)
+    stream = torch.cuda.current_stream()
+    sid = stream.cuda_stream
+
+    def timeit(fn):
+        fn()
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+        ev[0].record(stream)
+        for k in range(reps):
+            fn()
+            ev[k + 1].record(stream)
+        torch.cuda.synchronize()
+        return np.array([ev[k].elapsed_time(ev[k + 1]) for k in range(reps)])
+
+    def plain():
+        for sp in specs:
+            A.inverse_device(sp["f"], sp["flmn_in"], sp["p"], A.VIA_SSHT, sid)
+            A.forward_device(sp["flmn_out"], sp["f"], sp["p"], A.VIA_SSHT, sid)
+
+    t_plain = timeit(plain)
+    lanes = A.batch_tune(items, stream=sid)
+    t_batch = timeit(lambda: A.batch(items, lanes=lanes, stream=sid))
+    gh = A.graph_create(items, lanes=lanes)
+    try:
+        t_graph = timeit(lambda: A.graph_launch(gh, sid))
+    finally:
+        A.graph_destroy(gh)
+    n = len(items)
+    best = min(np.median(t_batch), np.median(t_graph))
+    return {
+        "workload": "96 uneven fwd+inv pairs: L in {32,48,...,384}, N in [1,min(L,64)], MW/MW_SS, "
+                    "complex/real (seed 4), via SSHT, Risbo, padded",
+        "items": n,
+        "lanes": lanes,
+        "plain_ms": float(np.median(t_plain)),
+        "batch_ms": float(np.median(t_batch)),
+        "graph_ms": float(np.median(t_graph)),
+        "plain_pairs_per_s": n / (np.median(t_plain) * 1e-3),
+        "batched_pairs_per_s": n / (best * 1e-3),
+        "speedup": float(np.median(t_plain) / best),
+        "p50_ms": float(np.percentile(np.concatenate([t_batch, t_graph]), 50)),
+        "p99_ms": float(np.percentile(np.concatenate([t_batch, t_graph]), 99)),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,12 +199,13 @@ def main():
     ap.add_argument("--dl", type=int, default=0, help="0 Risbo, 1 Trapani")
     ap.add_argument("--sampling", type=int, default=0, help="0 MW, 1 MW_SS")
     ap.add_argument("--real", type=int, default=0)
-    ap.add_argument("--cpu-slices", type=int, default=8)
+    ap.add_argument("--cpu-slices", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(see DESIGN.md for how the transform would shard over n).", file=sys.stderr)
+        print("bench.py measures one GPU: this build has no multi-GPU path (one GPU's "
+              "throughput is the reported value).", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -135,8 +261,27 @@ def main():
     L, N = p.L, p.N
     T = contraction_terms(L, N, real=bool(args.real))
     fl_contract = 5.0 * T  # 1 mul (Delta x Delta) + 2 FMA (complex accumulate) per term
-    dom = max(("contract_inv", "contract_fwd"), key=lambda k: per.get(k, 0.0))
-    achieved = fl_contract / (per[dom] * 1e-3) / 1e12
+    model = stage_model(p, A, T)
+    stage_roof = {}
+    for k, (bound, amount) in model.items():
+        if k not in per:
+            continue
+        if bound == "mfma":
+            ach = amount / (per[k] * 1e-3) / 1e12
+            stage_roof[k] = {"ms": round(per[k], 4), "bound": "mfma", "achieved": round(ach, 3),
+                             "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFLOPS, 4)}
+        else:
+            ach = amount / (per[k] * 1e-3) / 1e9
+            stage_roof[k] = {"ms": round(per[k], 4), "bound": "hbm", "achieved": round(ach, 1),
+                             "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4)}
+    dom = max(stage_roof, key=lambda k: stage_roof[k]["ms"])
+    bound, amount = model[dom]
+    ach = stage_roof[dom]["achieved"]
+    peak = FP64_PEAK_TFLOPS if bound == "mfma" else HBM_PEAK_GBS
+    roofline = {"bound": bound, "kernel": STAGE_KERNEL[dom], "stage": dom, "achieved": ach, "peak": peak,
+                "unit": stage_roof[dom]["unit"], "frac": ach / peak,
+                "traffic": measured_traffic(STAGE_KERNEL[dom]) if bound == "hbm" else None,
+                ("flops_per_launch" if bound == "mfma" else "bytes_per_launch"): amount}
     esz = 8 if args.real else 16
     bytes_alg = 2 * (16 * A.flmn_size(p) + esz * A.f_size(p))
     t_min = max(bytes_alg / (HBM_PEAK_GBS * 1e9), 2 * fl_contract / (FP64_PEAK_TFLOPS * 1e12))
@@ -163,12 +308,13 @@ def main():
         "wall_s": wall,
         "roundtrip_rel_err": err,
         "stage_ms": {k: round(v, 4) for k, v in per.items()},
-        "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "flops_per_launch": fl_contract},
+        "roofline": roofline,
+        "stage_roofline": stage_roof,
         "path_roofline": {"t_min_ms": t_min * 1e3, "frac": t_min * 1e3 / ms_step,
                           "bytes_alg": bytes_alg, "contraction_flops": 2 * fl_contract},
     }
+    if not args.no_batch:
+        out["batched"] = batched_bench(A, P, torch)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, args.cpu_slices)
     print(json.dumps(out))

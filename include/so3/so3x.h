@@ -59,6 +59,57 @@ int so3x_stage_reset(void);
 /* One-line description of the device and build ("gfx950, 256 CUs, ..."). */
 int so3x_device_info(char *buf, int len);
 
+/* ------------------------------------------------------------------ batches
+ * Many transforms of different sizes in one call.  The items run on a set
+ * of `lanes` HIP streams (default priority) forked from `stream` and joined
+ * back into it, each lane with its own workspace, larger items first
+ * (longest-processing-time assignment on a cost model); the call returns
+ * once everything is enqueued.  lanes <= 0 selects the lane count found best
+ * for this batch's shape by so3x_batch_tune (4 when never tuned).  Items must
+ * not alias each other's outputs.  Plans, tables and lane workspaces are
+ * built during the call when missing (a prepared batch allocates nothing).
+ */
+enum {
+  SO3X_ITEM_INVERSE = 0,        /* flmn_in -> f */
+  SO3X_ITEM_FORWARD = 1,        /* f -> flmn_out */
+  SO3X_ITEM_PAIR = 2,           /* flmn_in -> f -> flmn_out */
+  SO3X_ITEM_S2TOSO3_INVERSE = 3 /* (flm, glm) -> flmn_out = h_lmn -> f */
+};
+
+typedef struct {
+  so3_parameters_t parameters; /* of the SO(3) transform (h for s2toso3) */
+  int kind;                    /* SO3X_ITEM_* */
+  int method;                  /* SO3X_VIA_SSHT or SO3X_DIRECT */
+  void *f;                     /* signal on the SO(3) grid */
+  const void *flmn_in;         /* coefficients in (INVERSE, PAIR) or flm (S2TOSO3) */
+  void *flmn_out;              /* coefficients out (FORWARD, PAIR) or h_lmn (S2TOSO3) */
+  const void *glm;             /* S2TOSO3 only: the second S^2 signal's coefficients */
+} so3x_item_t;
+
+int so3x_batch(const so3x_item_t *items, int count, int lanes, void *stream);
+
+/* Time this batch (already-prepared items) with 1, 2, 4 and 8 lanes on
+ * `stream` and remember the fastest for batches of the same shape (the
+ * multiset of item kinds and parameter blocks).  Runs the batch several
+ * times, so outputs are overwritten.  Returns the chosen lane count. */
+int so3x_batch_tune(const so3x_item_t *items, int count, void *stream);
+
+/* Capture a batch into a HIP graph (the items are prepared first, outside
+ * the capture); launch it on any stream as often as wanted; destroy it.
+ * The graph holds the device pointers of the items. */
+int so3x_graph_create(const so3x_item_t *items, int count, int lanes, long long *graph);
+int so3x_graph_launch(long long graph, void *stream);
+int so3x_graph_destroy(long long graph);
+
+/* Completion tickets for asynchronous callers (the serving loop): a ticket
+ * marks the current end of `stream`; so3x_ticket_done returns 1 once every
+ * operation enqueued before it has finished, 0 before; so3x_ticket_wait
+ * blocks until then and releases the ticket (done tickets must be released
+ * with so3x_ticket_wait as well). */
+int so3x_ticket(void *stream, long long *ticket);
+int so3x_ticket_done(long long ticket);
+int so3x_ticket_wait(long long ticket);
+
 #ifdef __cplusplus
 }
 #endif

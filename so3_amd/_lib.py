@@ -48,6 +48,8 @@ SIGNATURES = {
     "so3_core_inverse_direct_real": (None, [_P, _P, _PP]),
     "so3_core_forward_direct": (None, [_P, _P, _PP]),
     "so3_core_forward_direct_real": (None, [_P, _P, _PP]),
+    # convolution (so3_conv.h)
+    "so3_conv_s2toso3_harmonic_convolution": (None, [_P, _PP, _P, _P]),
     # extensions (so3x.h)
     "so3x_last_error": (ctypes.c_char_p, []),
     "so3x_inverse": (_I, [_P, _P, _PP, _I, _P]),
@@ -61,6 +63,27 @@ SIGNATURES = {
     "so3x_stage_times": (_I, [ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_double), _PI, _I]),
     "so3x_stage_reset": (_I, []),
 }
+
+
+class so3x_item_t(ctypes.Structure):
+    """One item of a batch (so3x.h): parameters, kind, method, device pointers."""
+    _fields_ = [("parameters", so3_parameters_t), ("kind", ctypes.c_int), ("method", ctypes.c_int),
+                ("f", ctypes.c_void_p), ("flmn_in", ctypes.c_void_p), ("flmn_out", ctypes.c_void_p),
+                ("glm", ctypes.c_void_p)]
+
+
+_PITEM = ctypes.POINTER(so3x_item_t)
+_PLL = ctypes.POINTER(ctypes.c_longlong)
+SIGNATURES.update({
+    "so3x_batch": (_I, [_PITEM, _I, _I, _P]),
+    "so3x_batch_tune": (_I, [_PITEM, _I, _P]),
+    "so3x_graph_create": (_I, [_PITEM, _I, _I, _PLL]),
+    "so3x_graph_launch": (_I, [ctypes.c_longlong, _P]),
+    "so3x_graph_destroy": (_I, [ctypes.c_longlong]),
+    "so3x_ticket": (_I, [_P, _PLL]),
+    "so3x_ticket_done": (_I, [ctypes.c_longlong]),
+    "so3x_ticket_wait": (_I, [ctypes.c_longlong]),
+})
 
 
 class _Complex(ctypes.Structure):

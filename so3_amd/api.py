@@ -88,6 +88,18 @@ def forward(f, p, method=VIA_SSHT, flmn=None):
     return out
 
 
+def s2toso3(flm, glm, p):
+    """h_lmn = s2toso3_harmonic_convolution(flm, glm) for the SO(3) parameters p
+    (host arrays of L^2 coefficients; reference src/so3/bindings.pyx:437-448)."""
+    flm = np.ascontiguousarray(flm, dtype=np.complex128)
+    glm = np.ascontiguousarray(glm, dtype=np.complex128)
+    if flm.size < p.L * p.L or glm.size < p.L * p.L:
+        raise ValueError("flm and glm need L^2 = %d entries" % (p.L * p.L))
+    h = np.zeros(flmn_size(p), dtype=np.complex128)
+    lib().so3_conv_s2toso3_harmonic_convolution(_ptr(h), _s(p), _ptr(flm), _ptr(glm))
+    return h
+
+
 def inverse_device(f_ptr, flmn_ptr, p, method=VIA_SSHT, stream=None):
     """Device-memory inverse on `stream` (hipStream_t as int or None)."""
     check(lib().so3x_inverse(ctypes.c_void_p(f_ptr), ctypes.c_void_p(flmn_ptr), _s(p), method,
@@ -135,3 +147,68 @@ def stage_reset():
 
 def dft_device(x_ptr, n, howmany, direction, stream=None):
     check(lib().so3x_dft(ctypes.c_void_p(x_ptr), n, howmany, direction, ctypes.c_void_p(stream or 0)))
+
+
+# ---------------------------------------------------------------- batches
+ITEM_INVERSE, ITEM_FORWARD, ITEM_PAIR, ITEM_S2TOSO3_INVERSE = 0, 1, 2, 3
+
+
+def make_items(specs):
+    """ctypes array of so3x_item_t from dicts with keys p (SO3Parameters),
+    kind, method (default VIA_SSHT) and the device pointers f, flmn_in,
+    flmn_out, glm (ints, 0 when unused)."""
+    from ._lib import so3x_item_t
+    arr = (so3x_item_t * len(specs))()
+    for it, sp in zip(arr, specs):
+        it.parameters = sp["p"].struct()
+        it.kind = sp["kind"]
+        it.method = sp.get("method", VIA_SSHT)
+        it.f = sp.get("f", 0) or None
+        it.flmn_in = sp.get("flmn_in", 0) or None
+        it.flmn_out = sp.get("flmn_out", 0) or None
+        it.glm = sp.get("glm", 0) or None
+    return arr
+
+
+def batch(items, lanes=0, stream=None):
+    """Enqueue a batch (array from make_items) on `stream`; lanes <= 0: tuned or 4."""
+    check(lib().so3x_batch(items, len(items), lanes, ctypes.c_void_p(stream or 0)))
+
+
+def batch_tune(items, stream=None):
+    r = lib().so3x_batch_tune(items, len(items), ctypes.c_void_p(stream or 0))
+    if r < 0:
+        check(r)
+    return r
+
+
+def graph_create(items, lanes=0):
+    h = ctypes.c_longlong()
+    check(lib().so3x_graph_create(items, len(items), lanes, ctypes.byref(h)))
+    return h.value
+
+
+def graph_launch(handle, stream=None):
+    check(lib().so3x_graph_launch(handle, ctypes.c_void_p(stream or 0)))
+
+
+def graph_destroy(handle):
+    check(lib().so3x_graph_destroy(handle))
+
+
+def ticket(stream=None):
+    t = ctypes.c_longlong()
+    check(lib().so3x_ticket(ctypes.c_void_p(stream or 0), ctypes.byref(t)))
+    return t.value
+
+
+def ticket_done(t):
+    r = lib().so3x_ticket_done(t)
+    if r < 0:
+        check(r)
+    return bool(r)
+
+
+def ticket_wait(t):
+    check(lib().so3x_ticket_wait(t))
+

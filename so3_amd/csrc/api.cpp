@@ -14,6 +14,7 @@
 #include <mutex>
 #include <string>
 
+#include "batch.hpp"
 #include "conv.hpp"
 #include "fft_plan.hpp"
 #include "so3/so3_adjoint.h"
@@ -379,7 +380,9 @@ long long so3x_workspace_bytes(const so3_parameters_t *parameters) {
 
 int so3x_release_all(void) {
   try {
+    so3g::release_batch();
     so3g::release_all();
+    so3g::release_conv_caches();
     return 0;
   } catch (const std::exception &e) {
     return status(e);
@@ -417,6 +420,79 @@ int so3x_device_info(char *buf, int len) {
     snprintf(buf, (size_t)len, "%s, %d CUs, %.0f MHz, LDS/block %zu B, HBM %.1f GiB",
              pr.gcnArchName, pr.multiProcessorCount, pr.clockRate / 1000.0, (size_t)pr.sharedMemPerBlock,
              pr.totalGlobalMem / 1073741824.0);
+    return 0;
+  } catch (const std::exception &e) {
+    return status(e);
+  }
+}
+
+// ---------------------------------------------------------------- batches
+int so3x_batch(const so3x_item_t *items, int count, int lanes, void *stream) {
+  try {
+    so3g::batch_enqueue(items, count, lanes, (hipStream_t)stream);
+    return 0;
+  } catch (const std::exception &e) {
+    return status(e);
+  }
+}
+
+int so3x_batch_tune(const so3x_item_t *items, int count, void *stream) {
+  try {
+    return so3g::batch_tune(items, count, (hipStream_t)stream);
+  } catch (const std::exception &e) {
+    return status(e);
+  }
+}
+
+int so3x_graph_create(const so3x_item_t *items, int count, int lanes, long long *graph) {
+  try {
+    if (!graph) throw HipError("graph pointer is NULL");
+    *graph = so3g::graph_create(items, count, lanes);
+    return 0;
+  } catch (const std::exception &e) {
+    return status(e);
+  }
+}
+
+int so3x_graph_launch(long long graph, void *stream) {
+  try {
+    so3g::graph_launch(graph, (hipStream_t)stream);
+    return 0;
+  } catch (const std::exception &e) {
+    return status(e);
+  }
+}
+
+int so3x_graph_destroy(long long graph) {
+  try {
+    so3g::graph_destroy(graph);
+    return 0;
+  } catch (const std::exception &e) {
+    return status(e);
+  }
+}
+
+int so3x_ticket(void *stream, long long *ticket) {
+  try {
+    if (!ticket) throw HipError("ticket pointer is NULL");
+    *ticket = so3g::ticket_new((hipStream_t)stream);
+    return 0;
+  } catch (const std::exception &e) {
+    return status(e);
+  }
+}
+
+int so3x_ticket_done(long long ticket) {
+  try {
+    return so3g::ticket_done(ticket);
+  } catch (const std::exception &e) {
+    return status(e);
+  }
+}
+
+int so3x_ticket_wait(long long ticket) {
+  try {
+    so3g::ticket_wait(ticket);
     return 0;
   } catch (const std::exception &e) {
     return status(e);

@@ -1,0 +1,318 @@
+// batch.cpp -- batched execution of uneven transforms on one GPU.
+//
+// A batch of transforms of different sizes is spread over "lanes": HIP
+// streams at the default priority, each with its own workspace slab, forked
+// from the caller's stream by an event and joined back into it.  Items are
+// assigned longest-first to the least-loaded lane (LPT on a cost model of
+// contraction terms plus grid samples), so that the small transforms of a
+// batch fill the machine around the large ones instead of running one after
+// another.  Everything a batch needs (plans, Delta tables, DFT plans, index
+// tables, lane slabs) is built before the first launch, so a batch can be
+// captured into a HIP graph and replayed with no host work per item.
+#include "batch.hpp"
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "conv.hpp"
+#include "transform.hpp"
+
+namespace so3g {
+
+namespace {
+
+constexpr int kMaxLanes = 8;
+
+struct Lane {
+  hipStream_t s = nullptr;
+  hipEvent_t done = nullptr;
+  void *ws = nullptr;
+  size_t cap = 0;  // double2 elements
+};
+
+struct DevLanes {
+  hipEvent_t fork = nullptr;
+  Lane lane[kMaxLanes];
+};
+
+std::mutex g_mu;
+std::map<int, DevLanes> g_dev;
+std::map<std::string, int> g_tuned;  // batch shape -> lane count
+struct GraphRec {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t x = nullptr;
+};
+std::map<long long, GraphRec> g_graphs;
+std::map<long long, hipEvent_t> g_tickets;
+long long g_next_id = 1;
+
+Method meth(int m) { return m == SO3X_DIRECT ? kDirect : kViaSsht; }
+
+double item_cost(const so3x_item_t &it) {
+  const so3_parameters_t &p = it.parameters;
+  const double L = p.L, ns = p.reality ? p.N : 2.0 * p.N - 1.0;
+  const double an = std::min<double>(p.N, p.L);
+  // Delta-product terms: sum over n of sum_{l >= |n|} (2l+1)(l+1) ~ 2/3 (L^3 - |n|^3)
+  const double terms = (ns * L * L * L - (p.reality ? 1.0 : 2.0) * an * an * an * an / 4.0) * 2.0 / 3.0;
+  const double grid = ns * 2.0 * L * L;  // samples per DFT stage
+  double c = terms + 60.0 * grid;
+  if (it.kind == SO3X_ITEM_PAIR) c *= 2.0;
+  return c;
+}
+
+std::string shape_key(const so3x_item_t *items, int count) {
+  std::vector<std::string> v;
+  for (int i = 0; i < count; ++i) {
+    const so3_parameters_t &p = items[i].parameters;
+    char b[160];
+    snprintf(b, sizeof b, "%d/%d/%d/%d/%d/%d/%d/%d/%d/%d/%d/%d/%d", items[i].kind, items[i].method, p.reality,
+             p.L0, p.L, p.N, (int)p.sampling_scheme, (int)p.n_order, (int)p.storage, (int)p.n_mode,
+             (int)p.dl_method, p.steerable, 0);
+    v.emplace_back(b);
+  }
+  std::sort(v.begin(), v.end());
+  std::string k;
+  for (auto &s : v) k += s + ";";
+  return k;
+}
+
+DevLanes &dev_lanes() {
+  int dev = 0;
+  SO3_HIP(hipGetDevice(&dev));
+  DevLanes &d = g_dev[dev];
+  if (!d.fork) SO3_HIP(hipEventCreateWithFlags(&d.fork, hipEventDisableTiming));
+  return d;
+}
+
+// plans, tables and workspace sizes of every item (allocates what is missing)
+std::vector<size_t> prepare_items(const so3x_item_t *items, int count) {
+  std::vector<size_t> ws(count);
+  for (int i = 0; i < count; ++i) {
+    const so3x_item_t &it = items[i];
+    if (it.kind < SO3X_ITEM_INVERSE || it.kind > SO3X_ITEM_S2TOSO3_INVERSE) throw HipError("invalid item kind");
+    if (it.method != SO3X_VIA_SSHT && it.method != SO3X_DIRECT) throw HipError("invalid item method");
+    ws[i] = workspace_elems(&it.parameters, meth(it.method));
+    if (it.kind == SO3X_ITEM_S2TOSO3_INVERSE) {
+      if (it.parameters.steerable) throw HipError("s2toso3 needs steerable = 0");
+      cached_bases(&it.parameters);
+    }
+  }
+  return ws;
+}
+
+void run_item(const so3x_item_t &it, void *ws, hipStream_t s) {
+  const so3_parameters_t *p = &it.parameters;
+  const Method m = meth(it.method);
+  switch (it.kind) {
+    case SO3X_ITEM_INVERSE:
+      enqueue_inverse(p, m, it.flmn_in, it.f, ws, s);
+      break;
+    case SO3X_ITEM_FORWARD:
+      enqueue_forward(p, m, it.f, it.flmn_out, ws, s);
+      break;
+    case SO3X_ITEM_PAIR:
+      enqueue_inverse(p, m, it.flmn_in, it.f, ws, s);
+      enqueue_forward(p, m, it.f, it.flmn_out, ws, s);
+      break;
+    default:  // SO3X_ITEM_S2TOSO3_INVERSE
+      s2toso3(p, it.flmn_in, it.glm, it.flmn_out, s);
+      enqueue_inverse(p, m, it.flmn_out, it.f, ws, s);
+      break;
+  }
+}
+
+// the batch on `lanes` lanes, caller holds g_mu
+void enqueue_locked(const so3x_item_t *items, int count, int lanes, hipStream_t st) {
+  if (count <= 0) return;
+  if (!items) throw HipError("items pointer is NULL");
+  const std::vector<size_t> ws = prepare_items(items, count);
+  if (lanes <= 0) {
+    auto it = g_tuned.find(shape_key(items, count));
+    lanes = it != g_tuned.end() ? it->second : 4;
+  }
+  lanes = std::max(1, std::min({lanes, kMaxLanes, count}));
+  // longest processing time first onto the least-loaded lane
+  std::vector<int> order(count);
+  std::iota(order.begin(), order.end(), 0);
+  std::vector<double> cost(count);
+  for (int i = 0; i < count; ++i) cost[i] = item_cost(items[i]);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+  std::vector<double> load(lanes, 0.0);
+  std::vector<int> lane_of(count);
+  std::vector<size_t> need(lanes, 0);
+  for (int i : order) {
+    const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    lane_of[i] = l;
+    load[l] += cost[i];
+    need[l] = std::max(need[l], ws[i]);
+  }
+  DevLanes &d = dev_lanes();
+  for (int l = 0; l < lanes; ++l) {
+    Lane &ln = d.lane[l];
+    if (!ln.s) {
+      SO3_HIP(hipStreamCreateWithFlags(&ln.s, hipStreamNonBlocking));
+      SO3_HIP(hipEventCreateWithFlags(&ln.done, hipEventDisableTiming));
+    }
+    if (ln.cap < need[l]) {
+      SO3_HIP(hipStreamSynchronize(ln.s));
+      if (ln.ws) SO3_HIP(hipFree(ln.ws));
+      ln.ws = nullptr;
+      ln.cap = 0;
+      SO3_HIP(hipMalloc(&ln.ws, sizeof(double2) * need[l]));
+      ln.cap = need[l];
+    }
+  }
+  SO3_HIP(hipEventRecord(d.fork, st));
+  for (int l = 0; l < lanes; ++l) SO3_HIP(hipStreamWaitEvent(d.lane[l].s, d.fork, 0));
+  for (int i : order) run_item(items[i], d.lane[lane_of[i]].ws, d.lane[lane_of[i]].s);
+  for (int l = 0; l < lanes; ++l) {
+    SO3_HIP(hipEventRecord(d.lane[l].done, d.lane[l].s));
+    SO3_HIP(hipStreamWaitEvent(st, d.lane[l].done, 0));
+  }
+}
+
+}  // namespace
+
+void batch_enqueue(const so3x_item_t *items, int count, int lanes, hipStream_t st) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  enqueue_locked(items, count, lanes, st);
+}
+
+int batch_tune(const so3x_item_t *items, int count, hipStream_t st) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (count <= 0) return 1;
+  hipEvent_t a, b;
+  SO3_HIP(hipEventCreate(&a));
+  SO3_HIP(hipEventCreate(&b));
+  int best = 1;
+  float best_ms = 1e30f;
+  for (int lanes : {1, 2, 4, 8}) {
+    if (lanes > count && lanes > 1) break;
+    enqueue_locked(items, count, lanes, st);  // warm-up (allocates lane slabs)
+    SO3_HIP(hipEventRecord(a, st));
+    for (int r = 0; r < 3; ++r) enqueue_locked(items, count, lanes, st);
+    SO3_HIP(hipEventRecord(b, st));
+    SO3_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    SO3_HIP(hipEventElapsedTime(&ms, a, b));
+    if (ms < best_ms) best_ms = ms, best = lanes;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  g_tuned[shape_key(items, count)] = best;
+  return best;
+}
+
+long long graph_create(const so3x_item_t *items, int count, int lanes) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  hipStream_t cap = nullptr;
+  SO3_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+  GraphRec rec;
+  try {
+    // one eager run builds every plan, table and lane slab and sets the
+    // kernels' LDS attributes, so the capture records launches only
+    enqueue_locked(items, count, lanes, cap);
+    SO3_HIP(hipStreamSynchronize(cap));
+    SO3_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed));
+    try {
+      enqueue_locked(items, count, lanes, cap);
+    } catch (...) {
+      hipGraph_t dummy = nullptr;
+      (void)hipStreamEndCapture(cap, &dummy);
+      if (dummy) (void)hipGraphDestroy(dummy);
+      throw;
+    }
+    SO3_HIP(hipStreamEndCapture(cap, &rec.g));
+    SO3_HIP(hipGraphInstantiate(&rec.x, rec.g, nullptr, nullptr, 0));
+  } catch (...) {
+    (void)hipStreamDestroy(cap);
+    throw;
+  }
+  SO3_HIP(hipStreamDestroy(cap));
+  const long long id = g_next_id++;
+  g_graphs[id] = rec;
+  return id;
+}
+
+void graph_launch(long long graph, hipStream_t st) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_graphs.find(graph);
+  if (it == g_graphs.end()) throw HipError("unknown graph handle");
+  SO3_HIP(hipGraphLaunch(it->second.x, st));
+}
+
+void graph_destroy(long long graph) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_graphs.find(graph);
+  if (it == g_graphs.end()) throw HipError("unknown graph handle");
+  SO3_HIP(hipGraphExecDestroy(it->second.x));
+  SO3_HIP(hipGraphDestroy(it->second.g));
+  g_graphs.erase(it);
+}
+
+long long ticket_new(hipStream_t st) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  hipEvent_t e = nullptr;
+  SO3_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  SO3_HIP(hipEventRecord(e, st));
+  const long long id = g_next_id++;
+  g_tickets[id] = e;
+  return id;
+}
+
+int ticket_done(long long ticket) {
+  hipEvent_t e = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_tickets.find(ticket);
+    if (it == g_tickets.end()) throw HipError("unknown ticket");
+    e = it->second;
+  }
+  const hipError_t r = hipEventQuery(e);
+  if (r == hipSuccess) return 1;
+  if (r == hipErrorNotReady) return 0;
+  SO3_HIP(r);
+  return 0;
+}
+
+void ticket_wait(long long ticket) {
+  hipEvent_t e = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_tickets.find(ticket);
+    if (it == g_tickets.end()) throw HipError("unknown ticket");
+    e = it->second;
+    g_tickets.erase(it);
+  }
+  const hipError_t r = hipEventSynchronize(e);
+  (void)hipEventDestroy(e);
+  SO3_HIP(r);
+}
+
+void release_batch() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (auto &kv : g_graphs) {
+    (void)hipGraphExecDestroy(kv.second.x);
+    (void)hipGraphDestroy(kv.second.g);
+  }
+  g_graphs.clear();
+  for (auto &kv : g_tickets) (void)hipEventDestroy(kv.second);
+  g_tickets.clear();
+  for (auto &kv : g_dev) {
+    for (Lane &ln : kv.second.lane) {
+      if (ln.s) (void)hipStreamSynchronize(ln.s);
+      if (ln.ws) (void)hipFree(ln.ws);
+      if (ln.done) (void)hipEventDestroy(ln.done);
+      if (ln.s) (void)hipStreamDestroy(ln.s);
+      ln = Lane{};
+    }
+    if (kv.second.fork) (void)hipEventDestroy(kv.second.fork);
+  }
+  g_dev.clear();
+}
+
+}  // namespace so3g

@@ -9,6 +9,9 @@
 // Both are HBM-bound gathers: one thread per output coefficient.
 #include <algorithm>
 #include <cstdlib>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "conv.hpp"
@@ -121,13 +124,37 @@ int64_t *to_device(const std::vector<int64_t> &v, hipStream_t st) {
   return d;
 }
 
+// device copies of bases(p), kept per (device, L, N, reality, storage, order)
+std::mutex g_bmu;
+std::map<std::tuple<int, int, int, int, int, int>, int64_t *> g_bases;
+
 }  // namespace
+
+const int64_t *cached_bases(const so3_parameters_t *hp) {
+  int dev = 0;
+  SO3_HIP(hipGetDevice(&dev));
+  const auto key = std::make_tuple(dev, hp->L, hp->N, hp->reality, (int)hp->storage, (int)hp->n_order);
+  std::lock_guard<std::mutex> lk(g_bmu);
+  auto it = g_bases.find(key);
+  if (it != g_bases.end()) return it->second;
+  const std::vector<int64_t> b = bases(hp);
+  int64_t *d = nullptr;
+  SO3_HIP(hipMalloc((void **)&d, sizeof(int64_t) * b.size()));
+  SO3_HIP(hipMemcpy(d, b.data(), sizeof(int64_t) * b.size(), hipMemcpyHostToDevice));
+  g_bases[key] = d;
+  return d;
+}
+
+void release_conv_caches() {
+  std::lock_guard<std::mutex> lk(g_bmu);
+  for (auto &kv : g_bases) (void)hipFree(kv.second);
+  g_bases.clear();
+}
 
 void s2toso3(const so3_parameters_t *hp, const void *flm, const void *glm, void *hlmn, hipStream_t st) {
   if (hp->storage != SO3_STORAGE_PADDED && hp->storage != SO3_STORAGE_COMPACT)
     throw HipError("Invalid storage method.");
-  const std::vector<int64_t> b = bases(hp);
-  int64_t *db = to_device(b, st);
+  const int64_t *db = cached_bases(hp);
   const int nn = hp->reality ? hp->N : 2 * hp->N - 1;
   if (nn > 0 && hp->L > 0) {
     k_s2toso3<<<dim3(hp->L, nn), 64, 0, st>>>(hp->L, hp->N, hp->reality ? 1 : 0,
@@ -136,8 +163,6 @@ void s2toso3(const so3_parameters_t *hp, const void *flm, const void *glm, void 
                                                 (double2 *)hlmn);
     SO3_LAUNCH_CHECK();
   }
-  SO3_HIP(hipStreamSynchronize(st));
-  SO3_HIP(hipFreeAsync(db, st));
 }
 
 void harmonic_convolution(const so3_parameters_t *hp, const so3_parameters_t *fp,

@@ -77,7 +77,15 @@ FftDesc build(int n) {
     d.tw = upload(std::vector<cld>{cld(1, 0)});
     return d;
   }
-  if (factor(n, rad)) {
+  // a smooth length runs directly when one workgroup holds it in registers
+  // (radix 5 and 7 passes hold fewer points per thread), else via Bluestein
+  bool direct = factor(n, rad);
+  if (direct) {
+    int cap = kFftThreads * kFftElems * 4;
+    for (int R : rad) cap = std::min(cap, kFftThreads * fft_nb(R) * R);
+    direct = cap >= n;
+  }
+  if (direct) {
     d.blue = 0;
     d.len = n;
   } else {

@@ -669,6 +669,10 @@ __global__ void k_fwd_zero(Geo g, double2 *__restrict__ flmn, const int64_t *__r
 // sequences per workgroup: every pass must fit kFftElems values per thread,
 // and the LDS (data + twiddle tables) should leave room for two workgroups
 // per CU when it can
+// sequences per workgroup for a kernel that runs two descriptors (each must
+// fit its own register capacity)
+int choose_k2(int ld, const FftDesc &a, const FftDesc &b, int tables, int want);
+
 int choose_k(int ld, int cap, int len, int tables, int want) {
   int k = std::min(want, cap / len);
   const int budget = std::max(lds_limit() / 2, (int)(ld * sizeof(double2)) + tables);
@@ -676,6 +680,14 @@ int choose_k(int ld, int cap, int len, int tables, int want) {
   if (k < 1 || (size_t)k * ld * sizeof(double2) + tables > (size_t)lds_limit() || len > cap)
     throw HipError("DFT length " + std::to_string(len) + " exceeds the LDS / register budget");
   return k;
+}
+
+int choose_k2(int ld, const FftDesc &a, const FftDesc &b, int tables, int want) {
+  const int ka = fft_capacity(a) / a.len, kb = fft_capacity(b) / b.len;
+  if (ka < 1 || kb < 1)
+    throw HipError("DFT length " + std::to_string(std::max(a.len, b.len)) + " exceeds the register budget");
+  const int k = std::min(ka, kb);
+  return choose_k(ld, k * std::max(a.len, b.len), std::max(a.len, b.len), tables, want);
 }
 
 }  // namespace
@@ -816,7 +828,7 @@ void launch_fwd_beta(const Geo &g, const double2 *X2, double2 *Gp, NBase nb, con
   const FftDesc fw = fft_plan(Pw);
   const int ld = std::max(fft_ld(fb), fft_ld(fw));
   const int tb = (fb.len + fw.len) * (int)sizeof(double2);
-  const int K = choose_k(ld, std::min(fft_capacity(fb), fft_capacity(fw)), std::max(fb.len, fw.len), tb, 8);
+  const int K = choose_k2(ld, fb, fw, tb, 8);
   const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
   set_lds(k_fwd_beta, sh);
   const int chunks = (g.nA + K - 1) / K;
@@ -830,7 +842,7 @@ void launch_adjf_beta(const Geo &g, const double2 *X1, double2 *X2, NBase nb, co
   const FftDesc fw = fft_plan(Pw);
   const int ld = std::max(fft_ld(fb), fft_ld(fw));
   const int tb = (fb.len + fw.len) * (int)sizeof(double2);
-  const int K = choose_k(ld, std::min(fft_capacity(fb), fft_capacity(fw)), std::max(fb.len, fw.len), tb, 8);
+  const int K = choose_k2(ld, fb, fw, tb, 8);
   const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
   set_lds(k_adjf_beta, sh);
   const int chunks = (g.nA + K - 1) / K;

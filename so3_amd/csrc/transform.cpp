@@ -41,12 +41,15 @@ struct Plan {
   size_t x1 = 0, x2 = 0, x3 = 0; // workspace sizes (double2)
 };
 
-int smooth_at_least(int n) {
-  for (int m = n;; ++m) {
-    int k = m;
-    for (int p : {2, 3, 5, 7}) while (k % p == 0) k /= p;
-    if (k == 1) return m;
-  }
+// Length of the weight-convolution DFTs: the smallest 2^a or 3 * 2^a that is
+// at least n (any length >= 4L - 3 gives the exact linear convolution; these
+// run in one workgroup up to 4096 and keep 1024 for L = 256)
+int conv_length(int n) {
+  int best = 1;
+  while (best < n) best *= 2;
+  for (int m = 3; m < best; m *= 2)
+    if (m >= n && m <= 1536) return m;
+  return best;
 }
 
 double2 *upload(const std::vector<double2> &h) {
@@ -129,7 +132,7 @@ Plan build_plan(const so3_parameters_t *p, Method method) {
   }
   // weights w(p) = int_0^pi exp(-i p b) sin b db (so3_sampling_weight) at p mod Pw,
   // backward DFT of length Pw >= 4L-3 (any such length is exact)
-  P.Pw = smooth_at_least(4 * L - 3);
+  P.Pw = conv_length(4 * L - 3);
   {
     const int Pw = P.Pw;
     std::vector<cld> w(Pw, cld(0, 0));
@@ -324,12 +327,11 @@ void prepare(const so3_parameters_t *p, Method method) {
   dl_tables(p->L, (int)p->dl_method);
 }
 
-void run_inverse(const so3_parameters_t *p, Method method, const void *flmn, void *f, hipStream_t st) {
-  validate(p, method);
-  std::lock_guard<std::mutex> lk(g_mu);
-  const Plan &P = get_plan(p, method);
+namespace {
+
+void enqueue_inverse_locked(const Plan &P, const so3_parameters_t *p, const void *flmn, void *f, double2 *ws,
+                            hipStream_t st) {
   const DlTables &dl = dl_tables(p->L, (int)p->dl_method);
-  double2 *ws = arena(P.x1 + P.x2 + P.x3);
   double2 *X1 = ws, *X2 = ws + P.x1, *Gi = ws + P.x1 + P.x2;
   const Geo &g = P.g;
   NBase nb{P.nbase};
@@ -339,12 +341,9 @@ void run_inverse(const so3_parameters_t *p, Method method, const void *flmn, voi
   timed(3, st, [&] { launch_inv_gamma(g, X2, f, 1.0, st); });
 }
 
-void run_forward(const so3_parameters_t *p, Method method, const void *f, void *flmn, hipStream_t st) {
-  validate(p, method);
-  std::lock_guard<std::mutex> lk(g_mu);
-  const Plan &P = get_plan(p, method);
+void enqueue_forward_locked(const Plan &P, const so3_parameters_t *p, Method method, const void *f, void *flmn,
+                            double2 *ws, hipStream_t st) {
   const DlTables &dl = dl_tables(p->L, (int)p->dl_method);
-  double2 *ws = arena(P.x1 + P.x2);
   double2 *Gp = ws, *X2 = ws + P.x1;
   const Geo &g = P.g;
   NBase nb{P.nbase};
@@ -355,6 +354,42 @@ void run_forward(const so3_parameters_t *p, Method method, const void *f, void *
     launch_fwd_zero(g, (double2 *)flmn, nb, method == kDirect, 0, (int)p->storage, (int)p->n_order, st);
   });
   timed(8, st, [&] { launch_contract_fwd(g, Gp, nb, dl, (double2 *)flmn, st); });
+}
+
+}  // namespace
+
+void run_inverse(const so3_parameters_t *p, Method method, const void *flmn, void *f, hipStream_t st) {
+  validate(p, method);
+  std::lock_guard<std::mutex> lk(g_mu);
+  const Plan &P = get_plan(p, method);
+  enqueue_inverse_locked(P, p, flmn, f, arena(P.x1 + P.x2 + P.x3), st);
+}
+
+void run_forward(const so3_parameters_t *p, Method method, const void *f, void *flmn, hipStream_t st) {
+  validate(p, method);
+  std::lock_guard<std::mutex> lk(g_mu);
+  const Plan &P = get_plan(p, method);
+  enqueue_forward_locked(P, p, method, f, flmn, arena(P.x1 + P.x2), st);
+}
+
+size_t workspace_elems(const so3_parameters_t *p, Method method) {
+  validate(p, method);
+  std::lock_guard<std::mutex> lk(g_mu);
+  const Plan &P = get_plan(p, method);
+  dl_tables(p->L, (int)p->dl_method);
+  return P.x1 + P.x2 + P.x3;
+}
+
+void enqueue_inverse(const so3_parameters_t *p, Method method, const void *flmn, void *f, void *ws,
+                     hipStream_t st) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  enqueue_inverse_locked(get_plan(p, method), p, flmn, f, (double2 *)ws, st);
+}
+
+void enqueue_forward(const so3_parameters_t *p, Method method, const void *f, void *flmn, void *ws,
+                     hipStream_t st) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  enqueue_forward_locked(get_plan(p, method), p, method, f, flmn, (double2 *)ws, st);
 }
 
 void set_stage_timing(bool on) {

@@ -18,6 +18,17 @@ void run_forward(const so3_parameters_t *p, Method method, const void *f_dev, vo
 void run_adjoint_inverse(const so3_parameters_t *p, const void *f_dev, void *flmn_dev, hipStream_t st);
 void run_adjoint_forward(const so3_parameters_t *p, const void *flmn_dev, void *f_dev, hipStream_t st);
 void prepare(const so3_parameters_t *p, Method method);
+
+// Batched execution: plan, tables and the workspace size (double2 elements)
+// of one transform (builds what is missing, so call it outside graph
+// capture), then the stage sequences on a caller-provided workspace of at
+// least that size.  The enqueue functions allocate nothing and expect
+// validated parameters whose plan exists.
+size_t workspace_elems(const so3_parameters_t *p, Method method);
+void enqueue_inverse(const so3_parameters_t *p, Method method, const void *flmn_dev, void *f_dev, void *ws,
+                     hipStream_t st);
+void enqueue_forward(const so3_parameters_t *p, Method method, const void *f_dev, void *flmn_dev, void *ws,
+                     hipStream_t st);
 size_t workspace_bytes(const so3_parameters_t *p, Method method);
 void release_all();
 

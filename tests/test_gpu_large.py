@@ -78,3 +78,15 @@ def test_l256_sparse_inverse_and_round_trip(oracle, real):
     assert err <= 1e-12 * max(fmax, 1.0), (err, fmax)
     back = so3.forward(f, p)
     assert np.abs(back - flmn).max() < 1e-11
+
+
+# shapes of the batched workload's largest items (Bluestein inner lengths 2048,
+# Pw = 1536, N = 1) and a long-alpha case, checked by the round trip
+@pytest.mark.parametrize("L,N,ss,real", [(384, 64, 0, 0), (384, 64, 1, 1), (368, 17, 0, 1),
+                                         (320, 1, 1, 0), (512, 4, 0, 0), (32, 32, 1, 0)])
+def test_round_trip_batch_shapes(L, N, ss, real):
+    from so3_amd import synth
+    p = P.SO3Parameters(L=L, N=N, sampling_scheme=ss, reality=real, n_order=P.SO3_N_ORDER_ZERO_FIRST)
+    flmn = synth.gen_flmn(p, 5)
+    back = so3.forward(so3.inverse(flmn, p), p)
+    assert np.abs(back - flmn).max() < 1e-10 * max(1.0, np.abs(flmn).max())
