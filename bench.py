@@ -189,6 +189,27 @@ def batched_bench(A, P, torch, reps=5):
     }
 
 
+def serving_bench(A, rates=(50, 100, 200), count=120):
+    """The mixed request stream (Poisson arrivals) through the serving loop at
+    several rates: throughput and p50 / p99 latency per rate."""
+    from so3_amd import serve
+    A.cache_limit(96 << 30)
+    srv = serve.Server(hbm_budget=48 << 30, lanes=4, max_batch=32, streams=3)
+    fn = serve.random_inputs(5)
+    out = {"workload": "Poisson arrivals; 70%% fwd+inv pairs (L log-uniform [16,512], N in [1,min(L,32)]), "
+                       "30%% s2toso3 + inverse (L log-uniform [64,1024], N in {4,8}); MW complex (seed 5); "
+                       "%d requests per rate; tables and plans resident (built before timing)" % count,
+           "rates": {}}
+    for rate in rates:
+        reqs = serve.request_mix(count, rate)
+        srv.prepare(reqs)
+        warm = [serve.Request(r.rid, r.kind, r.p, 0.0) for r in reqs[:8]]
+        srv.run(warm, fn)
+        srv.run(reqs, fn)
+        out["rates"][str(rate)] = serve.latency_stats(reqs)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -202,6 +223,7 @@ def main():
     ap.add_argument("--cpu-slices", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one GPU's "
@@ -315,6 +337,8 @@ def main():
     }
     if not args.no_batch:
         out["batched"] = batched_bench(A, P, torch)
+    if not args.no_serve:
+        out["serving"] = serving_bench(A)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, args.cpu_slices)
     print(json.dumps(out))

@@ -43,6 +43,11 @@ long long so3x_workspace_bytes(const so3_parameters_t *parameters);
 /* Free every cached table, plan and workspace. */
 int so3x_release_all(void);
 
+/* Upper bound on the bytes of Delta tables kept resident (default 32 GiB);
+ * building a table beyond it waits for the device and evicts the least
+ * recently used tables. */
+int so3x_cache_limit(long long bytes);
+
 /* Batched 1-D DFTs of `howmany` contiguous sequences of length n (device
  * memory, in place), dir = -1 forward, +1 backward, unnormalised.  Exposes
  * the library's DFT kernels for testing. */

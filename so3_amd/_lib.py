@@ -57,6 +57,7 @@ SIGNATURES = {
     "so3x_prepare": (_I, [_PP]),
     "so3x_workspace_bytes": (ctypes.c_longlong, [_PP]),
     "so3x_release_all": (_I, []),
+    "so3x_cache_limit": (_I, [ctypes.c_longlong]),
     "so3x_dft": (_I, [_P, _I, _I, _I, _P]),
     "so3x_device_info": (_I, [ctypes.c_char_p, _I]),
     "so3x_stage_timing": (_I, [_I]),

@@ -212,3 +212,8 @@ def ticket_done(t):
 def ticket_wait(t):
     check(lib().so3x_ticket_wait(t))
 
+
+
+def cache_limit(nbytes):
+    """Bound the bytes of resident Delta tables (least recently used evicted)."""
+    check(lib().so3x_cache_limit(int(nbytes)))

@@ -389,6 +389,16 @@ int so3x_release_all(void) {
   }
 }
 
+int so3x_cache_limit(long long bytes) {
+  try {
+    if (bytes < 0) throw HipError("negative cache limit");
+    so3g::dl_tables_limit((size_t)bytes);
+    return 0;
+  } catch (const std::exception &e) {
+    return status(e);
+  }
+}
+
 int so3x_stage_timing(int on) {
   so3g::set_stage_timing(on != 0);
   return 0;

@@ -181,7 +181,51 @@ DlTables build(int L, int method) {
 
 std::mutex g_mu;
 std::map<std::tuple<int, int, int>, DlTables> g_cache;
+std::map<std::tuple<int, int, int>, unsigned long long> g_used;  // last use, per table
+unsigned long long g_clock = 0;
+size_t g_limit = (size_t)32 << 30;  // bytes of tables kept per process
 std::map<int, double *> g_zero;
+
+void free_tables(DlTables &t) {
+  (void)hipFree(t.tf);
+  (void)hipFree(t.ti);
+  (void)hipFree(t.offf);
+  (void)hipFree(t.offi);
+}
+
+// evict least-recently-used tables of other sizes until `incoming` more bytes
+// fit the limit; waits for the device first, since queued kernels may still
+// read the tables being dropped
+void make_room(size_t incoming, const std::tuple<int, int, int> &keep) {
+  size_t total = incoming;
+  for (auto &kv : g_cache) total += kv.second.bytes;
+  if (total <= g_limit) return;
+  SO3_HIP(hipDeviceSynchronize());
+  while (total > g_limit) {
+    auto victim = g_cache.end();
+    unsigned long long oldest = ~0ull;
+    for (auto it = g_cache.begin(); it != g_cache.end(); ++it) {
+      if (it->first == keep) continue;
+      const unsigned long long u = g_used[it->first];
+      if (u < oldest) oldest = u, victim = it;
+    }
+    if (victim == g_cache.end()) break;
+    total -= victim->second.bytes;
+    free_tables(victim->second);
+    g_used.erase(victim->first);
+    g_cache.erase(victim);
+  }
+}
+
+size_t table_bytes(int L) {
+  const int64_t nlt = (L + 15) / 16, nlc = (L + 3) / 4;
+  int64_t nf = 0, ni = 0;
+  for (int c = 0; c < L; ++c) {
+    nf += 256 * (tri(nlt) - tri(c >> 4));
+    for (int mt = 0; mt < nlt; ++mt) ni += 64 * (nlc - (std::max(c, 16 * mt) >> 2));
+  }
+  return sizeof(double) * (nf + ni);
+}
 
 }  // namespace
 
@@ -190,9 +234,16 @@ const DlTables &dl_tables(int L, int method) {
   SO3_HIP(hipGetDevice(&dev));
   std::lock_guard<std::mutex> g(g_mu);
   auto key = std::make_tuple(dev, L, method);
+  g_used[key] = ++g_clock;
   auto it = g_cache.find(key);
   if (it != g_cache.end()) return it->second;
+  make_room(table_bytes(L), key);
   return g_cache.emplace(key, build(L, method)).first->second;
+}
+
+void dl_tables_limit(size_t bytes) {
+  std::lock_guard<std::mutex> g(g_mu);
+  g_limit = bytes;
 }
 
 const double *zero_page() {
@@ -212,13 +263,9 @@ void dl_tables_clear() {
   std::lock_guard<std::mutex> g(g_mu);
   for (auto &kv : g_zero) (void)hipFree(kv.second);
   g_zero.clear();
-  for (auto &kv : g_cache) {
-    (void)hipFree(kv.second.tf);
-    (void)hipFree(kv.second.ti);
-    (void)hipFree(kv.second.offf);
-    (void)hipFree(kv.second.offi);
-  }
+  for (auto &kv : g_cache) free_tables(kv.second);
   g_cache.clear();
+  g_used.clear();
 }
 
 }  // namespace so3g

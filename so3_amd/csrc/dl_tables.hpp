@@ -31,8 +31,12 @@ struct DlTables {
   float build_ms = 0.f;
 };
 
-// Cached per (device, L, method); builds on first use (synchronously).
+// Cached per (device, L, method); builds on first use (synchronously).  The
+// cache keeps at most dl_tables_limit bytes of tables (32 GiB by default):
+// building a table that would exceed it first waits for the device and drops
+// the least recently used tables.
 const DlTables &dl_tables(int L, int method);
+void dl_tables_limit(size_t bytes);
 
 // Drop every cached table and zero page (frees device memory).
 void dl_tables_clear();
