@@ -224,6 +224,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="end with a 1 GiB device-to-device copy (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one GPU's "
@@ -342,6 +344,11 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, args.cpu_slices)
     print(json.dumps(out))
+    if args.copy_calibration:
+        a = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
 
 
 if __name__ == "__main__":

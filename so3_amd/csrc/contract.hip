@@ -10,9 +10,10 @@
 // The four sign combinations (+-m, +-n) of a pair (x, y) = (|m|, |n|) are
 // treated together: with D^l_{m',-m} = (-1)^{l+m'} D^l_{m',m}, one product
 // Q(l, m') = D^l_{m'x} D^l_{m'y} feeds all four.  The factor (-1)^l of the
-// mixed-sign combinations sits in the packed coefficients (inverse) or is
-// applied at the store (forward); (-1)^{m'} is applied at the store
-// (inverse) or folded into G' by the forward beta stage.
+// mixed-sign combinations (and c_l of the synthesis) multiplies the
+// coefficients as they enter the matrix instruction (inverse) or the result
+// at the store (forward); (-1)^{m'} is applied at the store (inverse) or
+// folded into G' by the forward beta stage.
 //
 // For one pair the contraction is a small matrix product over the reduction
 // index r (m' forward, l inverse) with the lane index s (l forward, m'
@@ -63,37 +64,6 @@ __device__ __forceinline__ int coef_col(int rho, bool &used) {
   return (q & 1) * 2 + (j & 1);
 }
 
-// ------------------------------------------------------------------ packing
-// Gi[(off(an,am) + (l - l0(an,am))) * C + c], l0 = max(an & ~3, am & ~3),
-// l < P4(L): c_l f_{l, +-am, +-an} with the (-1)^l of mixed signs folded in,
-// zero where the coefficient is not part of the transform.
-__global__ void k_pack_inv(Geo g, const double2 *__restrict__ flmn, const int64_t *__restrict__ nbase,
-                           const int64_t *__restrict__ poff, double2 *__restrict__ Gi) {
-  const int an = blockIdx.y;
-  const int am = blockIdx.x;
-  const int C = 2 * g.NS;
-  const int l0 = max(an & ~3, am & ~3);
-  const int64_t off = poff[(int64_t)an * g.L + am];
-  const int lend = (g.L + 3) & ~3;
-  for (int t = threadIdx.x; t < (lend - l0) * C; t += blockDim.x) {
-    const int l = l0 + t / C, c = t % C;
-    const int sm = c & 1, sn = c >> 1;
-    const int m = sm ? -am : am, n = sn ? -an : an;
-    double2 v = make_double2(0.0, 0.0);
-    const bool ok = an < g.N && l < g.L && l >= g.L0 && l >= am && l >= an && (!g.nmodeL || l == an);
-    if (ok) {
-      const int64_t b = nbase[n + g.N - 1];
-      if (b != kNoBlock) {
-        v = flmn[b + (int64_t)l * l + l + m];
-        double s = g.clmode == 1 ? (2.0 * l + 1.0) * kInv8Pi2 : 1.0;
-        if ((sm != sn) && (l & 1)) s = -s;
-        v = cscale(v, s);
-      }
-    }
-    Gi[(off + (l - l0)) * C + c] = v;
-  }
-}
-
 // ------------------------------------------------------------------ staging
 // Operand staging shared by both kernels.  A workgroup of four wavefronts
 // owns the pairs (x0 + a, y0 + w), a = 0..3, for wavefront w.  Per
@@ -142,8 +112,8 @@ __device__ __forceinline__ void wait_ring(int ahead, int n) {
 // Workgroup: x group x0 .. x0+3, y0 + w for wavefront w, m' block mpb ..
 // mpb+63 (four 16-wide tiles); reduction over l in chunks of 4.
 template <int NS>
-__global__ __launch_bounds__(256) void k_contract_inv(Geo g, const double *__restrict__ Gi,
-                                                      const int64_t *__restrict__ poff,
+__global__ __launch_bounds__(256) void k_contract_inv(Geo g, const double2 *__restrict__ flmn,
+                                                      const int64_t *__restrict__ nbase,
                                                       const double *__restrict__ ti,
                                                       const int64_t *__restrict__ offi, int nmt,
                                                       const double *__restrict__ zero,
@@ -188,20 +158,26 @@ __global__ __launch_bounds__(256) void k_contract_inv(Geo g, const double *__res
     dsrc[j] = ti + offi[(int64_t)cc * nmt + (mpc >> 4)] - 64 * lcmin + e2;
     dlc[j] = ok ? lcmin : 0x7fffffff;
   }
-  const double *csrc[St::kCvI];
-  bool cok[St::kCvI];
+  // coefficient regions: lane i of a region copies f_{l, +-m, +-n} for row
+  // l = l0 + i / C and sign combination i % C straight from the caller's
+  // flmn; entries outside the transform come from the zero page
+  const double *cbase[St::kCvI];  // flmn + 2 (base(n) + m): row l at + 2 l (l + 1)
+  int crow[St::kCvI], clmin[St::kCvI], cnj[St::kCvI];
 #pragma unroll
   for (int j = 0; j < St::kCvI; ++j) {
     const int lpr = St::kRegD / 2;                // lanes per region
     const int rho = j * (64 / lpr) + lane / lpr;  // region = 2 a + h
     const int a = rho >> 1, h = rho & 1;
     const int x = x0 + a;
+    const int li = lane % lpr, combo = li % C;
+    const int mi = h == 0 ? x : y, nj = h == 0 ? y : x;
+    const int m = (combo & 1) ? -mi : mi, n = (combo >> 1) ? -nj : nj;
     const bool ok = h == 0 ? pv[a] : pp[a];
-    const int l0p = max(x & ~3, y & ~3);
-    const int64_t o = ok ? (h == 0 ? poff[(int64_t)y * g.L + x] : poff[(int64_t)x * g.L + y]) : 0;
-    // rows l of the stream start at (o + l - l0p) * 2C
-    csrc[j] = Gi + (o - l0p) * (2 * C) + 2 * (lane % lpr);
-    cok[j] = ok;
+    const int64_t b = ok ? nbase[n + g.N - 1] : kNoBlock;
+    crow[j] = li / C;
+    cnj[j] = nj;
+    clmin[j] = b == kNoBlock ? 0x7fffffff : max(g.L0, max(mi, nj));
+    cbase[j] = (const double *)flmn + 2 * (b == kNoBlock ? 0 : b + m);
   }
   auto issue = [&](int l0, double *buf) {
     const int lc = l0 >> 2;
@@ -213,7 +189,9 @@ __global__ __launch_bounds__(256) void k_contract_inv(Geo g, const double *__res
     }
 #pragma unroll
     for (int j = 0; j < St::kCvI; ++j) {
-      const double *src = cok[j] ? csrc[j] + (int64_t)l0 * (2 * C) : zero + 2 * (lane % (St::kRegD / 2));
+      const int l = l0 + crow[j];
+      const bool ok = l >= clmin[j] && l < g.L && (!g.nmodeL || l == cnj[j]);
+      const double *src = ok ? cbase[j] + 2 * (int64_t)l * (l + 1) : zero + 2 * (lane % (St::kRegD / 2));
       dma16(src, buf + 2048 + w * 8 * St::kRegD + j * 128);
     }
   };
@@ -224,6 +202,7 @@ __global__ __launch_bounds__(256) void k_contract_inv(Geo g, const double *__res
   const int h = ((lane & 15) & 3) >> 1;
   const int k = lane >> 4;
   const int cvo = 2048 + (w * 8 + h) * St::kRegD + k * (2 * C) + col;  // + 2 a kRegD
+  const bool mixed = ((col >> 1) & 1) != ((col >> 1) >> 1);  // combination col / 2: sm != sn
 
   d4 acc[kXG][kNT];
 #pragma unroll
@@ -243,9 +222,14 @@ __global__ __launch_bounds__(256) void k_contract_inv(Geo g, const double *__res
     __builtin_amdgcn_s_barrier();
     const double *buf = sm + (q % kRing) * kBufD;
     const int l0 = lbeg + 4 * q;
+    // c_l and the (-1)^l of the mixed-sign combinations on this lane's row
+    const int lk = l0 + k;
+    double fac = (mixed && (lk & 1)) ? -1.0 : 1.0;
+    if (g.clmode == 1) fac *= (2.0 * lk + 1.0) * kInv8Pi2;
+    if (!used) fac = 0.0;
     double cv[kXG], dy[kNT];
 #pragma unroll
-    for (int a = 0; a < kXG; ++a) cv[a] = used ? buf[cvo + 2 * a * St::kRegD] : 0.0;
+    for (int a = 0; a < kXG; ++a) cv[a] = fac * buf[cvo + 2 * a * St::kRegD];
 #pragma unroll
     for (int t = 0; t < kNT; ++t) dy[t] = buf[1024 + (4 * w + t) * 64 + lane];
 #pragma unroll
@@ -454,28 +438,15 @@ __global__ __launch_bounds__(256) void k_contract_fwd(Geo g, const double *__res
 
 }  // namespace
 
-int64_t pack_inv_size(const Geo &g, int64_t *poff_host) {
-  int64_t acc = 0;
-  for (int an = 0; an < g.N; ++an)
-    for (int am = 0; am < g.L; ++am) {
-      if (poff_host) poff_host[(int64_t)an * g.L + am] = acc;
-      const int l0 = std::max(an & ~3, am & ~3);
-      acc += std::max(0, ((g.L + 3) & ~3) - l0);
-    }
-  return acc * 2 * g.NS;  // double2 elements
-}
-
-void launch_contract_inv(const Geo &g, const double2 *flmn, NBase nb, const DlTables &dl,
-                         const int64_t *poff, double2 *Gi, double2 *X1, hipStream_t st) {
-  k_pack_inv<<<dim3(g.L, g.N), 256, 0, st>>>(g, flmn, nb.base, poff, Gi);
-  SO3_LAUNCH_CHECK();
+void launch_contract_inv(const Geo &g, const double2 *flmn, NBase nb, const DlTables &dl, double2 *X1,
+                         hipStream_t st) {
   const int Np = std::min(g.N, g.L);
   const int nmb = (g.L + 63) / 64;
   dim3 grid(nmb * ((g.L + kXG - 1) / kXG), (Np + 3) / 4);
   if (g.NS == 2)
-    k_contract_inv<2><<<grid, 256, 0, st>>>(g, (const double *)Gi, poff, dl.ti, dl.offi, dl.nmt, zero_page(), X1);
+    k_contract_inv<2><<<grid, 256, 0, st>>>(g, flmn, nb.base, dl.ti, dl.offi, dl.nmt, zero_page(), X1);
   else
-    k_contract_inv<1><<<grid, 256, 0, st>>>(g, (const double *)Gi, poff, dl.ti, dl.offi, dl.nmt, zero_page(), X1);
+    k_contract_inv<1><<<grid, 256, 0, st>>>(g, flmn, nb.base, dl.ti, dl.offi, dl.nmt, zero_page(), X1);
   SO3_LAUNCH_CHECK();
   if (g.N > g.L) {
     k_zero_rows<<<dim3(64, g.N - g.L), 256, 0, st>>>(g, X1);

@@ -12,7 +12,7 @@ extern "C" int so3x_dft(void *x, int n, int howmany, int dir, void *stream) {
   using namespace so3g;
   try {
     if (n < 1 || howmany < 0 || (dir != 1 && dir != -1)) throw HipError("bad so3x_dft arguments");
-    launch_rows((double2 *)x, howmany, n, dir, 1.0, (hipStream_t)stream);
+    launch_rows((double2 *)x, howmany, n, n, dir, 1.0, (hipStream_t)stream);
     return 0;
   } catch (const std::exception &e) {
     fprintf(stderr, "so3x_dft: %s\n", e.what());

@@ -14,6 +14,7 @@ namespace so3g {
 struct Geo {
   int L, N, L0;
   int nA, nB, nG;  // alpha / beta samples, n slots
+  int ldA;         // row stride of the X2 workspace (>= nA)
   int Pb;          // beta DFT length: 2L-1 (MW) or 2L (MW_SS)
   int ss;          // MW_SS sampling
   int real;        // real signal: n >= 0 only, c2r / r2c over gamma
@@ -35,21 +36,17 @@ struct NBase {
 // Workspace layouts (double2):
 //   X1 rows  : [n_slot][m_slot][m'], m' < L          (inverse contraction out)
 //   Gp       : [|n|][|m|][2*NS sign combos][m']        (forward contraction in)
-//   X2       : [n_slot][b][a or m_slot]               (beta / alpha stages)
-// Packed inverse coefficient streams: pair (|n|, |m|) at poff[|n| L + |m|]
-// (in units of 2*NS double2), covering l from max(|n| & ~3, |m| & ~3).
-// Returns the packed size in double2 and fills poff_host when non-null.
-int64_t pack_inv_size(const Geo &g, int64_t *poff_host);
-void launch_contract_inv(const Geo &g, const double2 *flmn, NBase nb, const DlTables &dl,
-                         const int64_t *poff, double2 *Gi, double2 *X1, hipStream_t st);
+//   X2       : [n_slot][b][a or m_slot], rows of ldA  (beta / alpha stages)
+void launch_contract_inv(const Geo &g, const double2 *flmn, NBase nb, const DlTables &dl, double2 *X1,
+                         hipStream_t st);
 void launch_contract_fwd(const Geo &g, const double2 *Gp, NBase nb, const DlTables &dl,
                          double2 *flmn, hipStream_t st);
 
 void launch_inv_beta(const Geo &g, const double2 *X1, double2 *X2, NBase nb, const double2 *phase,
                      hipStream_t st);
 void launch_alpha(const Geo &g, double2 *X2, int dir, double scale, hipStream_t st);
-// nrows contiguous DFTs of length n (sign dir), results times scale
-void launch_rows(double2 *X, int64_t nrows, int n, int dir, double scale, hipStream_t st);
+// nrows DFTs of length n (sign dir) on rows ld apart, results times scale
+void launch_rows(double2 *X, int64_t nrows, int n, int ld, int dir, double scale, hipStream_t st);
 void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, double scale, hipStream_t st);
 void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, double scale, hipStream_t st);
 // adjoint != 0: zero-fill instead of the beta extension and no weights (the

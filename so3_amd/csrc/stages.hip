@@ -66,27 +66,27 @@ __global__ __launch_bounds__(kThreads) void k_inv_beta(Geo g, FftDesc fd, int K,
   fft_run<+1>(A, K, ld, fd, tw);
   for (int idx = threadIdx.x; idx < K * g.nB; idx += blockDim.x) {
     const int b = idx / K, s = idx - b * K;
-    if (ms0 + s < g.nA) X2[((int64_t)nslot * g.nB + b) * g.nA + ms0 + s] = A[s * ld + b];
+    if (ms0 + s < g.nA) X2[((int64_t)nslot * g.nB + b) * g.ldA + ms0 + s] = A[s * ld + b];
   }
 }
 
 // ---------------------------------------------------------------- rows (alpha)
 template <int DIR>
 __global__ __launch_bounds__(kThreads) void k_rows(FftDesc fd, int K, int ld, double2 *__restrict__ X,
-                                                   int64_t nrows, int n, double scale) {
+                                                   int64_t nrows, int n, int64_t rs, double scale) {
   extern __shared__ double2 lds[];
   double2 *A = lds, *tw = lds + K * ld;
   fft_load_twiddles(tw, fd);
   const int64_t r0 = (int64_t)blockIdx.x * K;
   for (int idx = threadIdx.x; idx < K * n; idx += blockDim.x) {
     const int s = idx / n, k = idx - s * n;
-    A[s * ld + k] = (r0 + s < nrows) ? X[(r0 + s) * n + k] : make_double2(0.0, 0.0);
+    A[s * ld + k] = (r0 + s < nrows) ? X[(r0 + s) * rs + k] : make_double2(0.0, 0.0);
   }
   __syncthreads();
   fft_run<DIR>(A, K, ld, fd, tw);
   for (int idx = threadIdx.x; idx < K * n; idx += blockDim.x) {
     const int s = idx / n, k = idx - s * n;
-    if (r0 + s < nrows) X[(r0 + s) * n + k] = cscale(A[s * ld + k], scale);
+    if (r0 + s < nrows) X[(r0 + s) * rs + k] = cscale(A[s * ld + k], scale);
   }
 }
 
@@ -95,17 +95,21 @@ __global__ __launch_bounds__(kThreads) void k_rows(FftDesc fd, int K, int ld, do
 // [513, 1023]): one row per wavefront in the register layouts of
 // fft_reg.hpp.
 constexpr int kRegWaves = kThreads / 64;
+// wavefronts per workgroup of the gamma (8: 128-byte HBM runs) and beta (4:
+// two workgroups per CU) kernels, measured best at L = 256
+constexpr int kGammaW = 8;
+constexpr int kBetaW = 4;
 constexpr size_t kRegShared = sizeof(double2) * kRegWaves * kRegLds;
 
 template <int DIR, int BLUE>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_rows_reg(
-    FftDesc fd, double2 *__restrict__ X, int64_t nrows, int n, double scale) {
+    FftDesc fd, double2 *__restrict__ X, int64_t nrows, int n, int64_t rs, double scale) {
   extern __shared__ double2 lds[];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t r = (int64_t)blockIdx.x * kRegWaves + wv;
   if (r >= nrows) return;  // the whole wavefront
   double2 *T = lds + wv * kRegLds;
-  double2 *row = X + r * n;
+  double2 *row = X + r * rs;
   double2 v[16];
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
@@ -133,39 +137,44 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))
 // consecutive m of the beta output), moved through an LDS tile with rows of
 // 9 entries so that HBM sees 128-byte runs and LDS no bank conflicts.  The
 // tile shares LDS with the transpose buffers (barriers separate the uses).
-constexpr int kColWaves = 8;
-constexpr int kColThreads = 64 * kColWaves;
-constexpr int kTileLd = kColWaves + 1;
-constexpr size_t kColShared =
-    sizeof(double2) * (1024 * kTileLd > kColWaves * kRegLds ? 1024 * kTileLd : kColWaves * kRegLds);
+// W: wavefronts (sequences) per workgroup; the tile rows hold W + 1 entries
+template <int W>
+constexpr int tile_ld() { return W + 1; }
+// LDS bytes of a workgroup whose tile has `rows` rows
+template <int W>
+size_t col_shared(int rows) {
+  return sizeof(double2) * std::max<size_t>((size_t)rows * tile_ld<W>(), (size_t)W * kRegLds);
+}
 
-template <int BLUE>
-__global__ __launch_bounds__(kColThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_inv_gamma_reg(
+template <int W, int BLUE>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_inv_gamma_reg(
     Geo g, FftDesc fd, const double2 *__restrict__ X2, double2 *__restrict__ fc, double *__restrict__ fr,
     double scale) {
   extern __shared__ double2 lds[];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t S = (int64_t)g.nB * g.nA;
-  const int64_t p0 = (int64_t)blockIdx.x * kColWaves;
+  // columns p2 of X2 (rows of ldA: 128-byte aligned tiles) map to columns
+  // b nA + a of f
+  const int64_t S = (int64_t)g.nB * g.nA, S2 = (int64_t)g.nB * g.ldA;
+  const int64_t p0 = (int64_t)blockIdx.x * W;
   const int nG = 2 * g.N - 1;
-  for (int idx = threadIdx.x; idx < nG * kColWaves; idx += kColThreads) {
-    const int k = idx / kColWaves, c = idx - k * kColWaves;
+  for (int idx = threadIdx.x; idx < nG * W; idx += (64 * W)) {
+    const int k = idx / W, c = idx - k * W;
     const int64_t p = p0 + c;
     double2 v = make_double2(0.0, 0.0);
-    if (p < S) {
-      if (!g.real) v = X2[k * S + p];
+    if (p < S2 && p % g.ldA < g.nA) {
+      if (!g.real) v = X2[k * S2 + p];
       else if (k == 0) v = make_double2(X2[p].x, 0.0);
-      else if (k < g.N) v = X2[k * S + p];
-      else v = cconj(X2[(nG - k) * S + p]);
+      else if (k < g.N) v = X2[k * S2 + p];
+      else v = cconj(X2[(nG - k) * S2 + p]);
     }
-    lds[k * kTileLd + c] = v;
+    lds[k * tile_ld<W>() + c] = v;
   }
   __syncthreads();
   double2 v[16];
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int k = lane + 64 * q;
-    v[q] = k < nG ? lds[k * kTileLd + wv] : make_double2(0.0, 0.0);
+    v[q] = k < nG ? lds[k * tile_ld<W>() + wv] : make_double2(0.0, 0.0);
   }
   __syncthreads();
   double2 *T = lds + wv * kRegLds;
@@ -175,47 +184,51 @@ __global__ __launch_bounds__(kColThreads) __attribute__((amdgpu_waves_per_eu(2, 
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int k = lane + 64 * q;
-      if (k < nG) lds[k * kTileLd + wv] = v[q];
+      if (k < nG) lds[k * tile_ld<W>() + wv] = v[q];
     }
   } else {
     fft_F<+1>(v, T, fd.tws, fd.tws + 1024, lane);
     __syncthreads();
 #pragma unroll
-    for (int c = 0; c < 16; ++c) lds[o_index(lane, c) * kTileLd + wv] = v[c];
+    for (int c = 0; c < 16; ++c) lds[o_index(lane, c) * tile_ld<W>() + wv] = v[c];
   }
   __syncthreads();
-  for (int idx = threadIdx.x; idx < nG * kColWaves; idx += kColThreads) {
-    const int k = idx / kColWaves, c = idx - k * kColWaves;
-    const int64_t p = p0 + c;
-    if (p >= S) continue;
-    const double2 v2 = lds[k * kTileLd + c];
+  for (int idx = threadIdx.x; idx < nG * W; idx += (64 * W)) {
+    const int k = idx / W, c = idx - k * W;
+    const int64_t p2 = p0 + c;
+    const int a = (int)(p2 % g.ldA);
+    if (p2 >= S2 || a >= g.nA) continue;
+    const int64_t p = (p2 / g.ldA) * g.nA + a;
+    const double2 v2 = lds[k * tile_ld<W>() + c];
     if (g.real) fr[k * S + p] = v2.x * scale;
     else fc[k * S + p] = cscale(v2, scale);
   }
 }
 
-template <int BLUE>
-__global__ __launch_bounds__(kColThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_fwd_gamma_reg(
+template <int W, int BLUE>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_fwd_gamma_reg(
     Geo g, FftDesc fd, const double2 *__restrict__ fc, const double *__restrict__ fr,
     double2 *__restrict__ X2, double sc) {
   extern __shared__ double2 lds[];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t S = (int64_t)g.nB * g.nA;
-  const int64_t p0 = (int64_t)blockIdx.x * kColWaves;
+  // columns p of f (aligned tiles of the flat f layout) map to columns
+  // (p / nA) ldA + p % nA of X2
+  const int64_t S = (int64_t)g.nB * g.nA, S2 = (int64_t)g.nB * g.ldA;
+  const int64_t p0 = (int64_t)blockIdx.x * W;
   const int nG = 2 * g.N - 1;
-  for (int idx = threadIdx.x; idx < nG * kColWaves; idx += kColThreads) {
-    const int k = idx / kColWaves, c = idx - k * kColWaves;
+  for (int idx = threadIdx.x; idx < nG * W; idx += (64 * W)) {
+    const int k = idx / W, c = idx - k * W;
     const int64_t p = p0 + c;
     double2 v = make_double2(0.0, 0.0);
     if (p < S) v = g.real ? make_double2(fr[k * S + p], 0.0) : fc[k * S + p];
-    lds[k * kTileLd + c] = v;
+    lds[k * tile_ld<W>() + c] = v;
   }
   __syncthreads();
   double2 v[16];
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int k = lane + 64 * q;
-    v[q] = k < nG ? lds[k * kTileLd + wv] : make_double2(0.0, 0.0);
+    v[q] = k < nG ? lds[k * tile_ld<W>() + wv] : make_double2(0.0, 0.0);
   }
   __syncthreads();
   double2 *T = lds + wv * kRegLds;
@@ -225,33 +238,34 @@ __global__ __launch_bounds__(kColThreads) __attribute__((amdgpu_waves_per_eu(2, 
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int k = lane + 64 * q;
-      if (k < nG) lds[k * kTileLd + wv] = v[q];
+      if (k < nG) lds[k * tile_ld<W>() + wv] = v[q];
     }
   } else {
     fft_F<-1>(v, T, fd.tws, fd.tws + 1024, lane);
     __syncthreads();
 #pragma unroll
-    for (int c = 0; c < 16; ++c) lds[o_index(lane, c) * kTileLd + wv] = v[c];
+    for (int c = 0; c < 16; ++c) lds[o_index(lane, c) * tile_ld<W>() + wv] = v[c];
   }
   __syncthreads();
   const int nout = g.real ? g.N : nG;
-  for (int idx = threadIdx.x; idx < nout * kColWaves; idx += kColThreads) {
-    const int k = idx / kColWaves, c = idx - k * kColWaves;
+  for (int idx = threadIdx.x; idx < nout * W; idx += (64 * W)) {
+    const int k = idx / W, c = idx - k * W;
     const int64_t p = p0 + c;
-    if (p < S) X2[k * S + p] = cscale(lds[k * kTileLd + c], sc);
+    if (p < S) X2[k * S2 + (p / g.nA) * g.ldA + p % g.nA] = cscale(lds[k * tile_ld<W>() + c], sc);
   }
 }
 
 // inverse beta: sequence (n slot, m slot) per wavefront, 8 consecutive m
 // slots per workgroup; Pb is a Bluestein length with inner length 1024
-__global__ __launch_bounds__(kColThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_inv_beta_reg(
+template <int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_inv_beta_reg(
     Geo g, FftDesc fd, const double2 *__restrict__ X1, double2 *__restrict__ X2,
     const int64_t *__restrict__ nbase, const double2 *__restrict__ phase) {
   extern __shared__ double2 lds[];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int chunks = (g.nA + kColWaves - 1) / kColWaves;
+  const int chunks = (g.nA + W - 1) / W;
   const int nslot = blockIdx.x / chunks;
-  const int ms0 = (blockIdx.x % chunks) * kColWaves;
+  const int ms0 = (blockIdx.x % chunks) * W;
   const int n = n_of_slot(g, nslot);
   const bool active = nbase[n + g.N - 1] != kNoBlock;
   const int Pb = g.Pb, L = g.L;
@@ -280,12 +294,12 @@ __global__ __launch_bounds__(kColThreads) __attribute__((amdgpu_waves_per_eu(2, 
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int b = lane + 64 * q;
-    if (b < g.nB) lds[b * kTileLd + wv] = v[q];
+    if (b < g.nB) lds[b * tile_ld<W>() + wv] = v[q];
   }
   __syncthreads();
-  for (int idx = threadIdx.x; idx < g.nB * kColWaves; idx += kColThreads) {
-    const int b = idx / kColWaves, c = idx - b * kColWaves;
-    if (ms0 + c < g.nA) X2[((int64_t)nslot * g.nB + b) * g.nA + ms0 + c] = lds[b * kTileLd + c];
+  for (int idx = threadIdx.x; idx < g.nB * W; idx += (64 * W)) {
+    const int b = idx / W, c = idx - b * W;
+    if (ms0 + c < g.nA) X2[((int64_t)nslot * g.nB + b) * g.ldA + ms0 + c] = lds[b * tile_ld<W>() + c];
   }
 }
 
@@ -297,22 +311,23 @@ __global__ __launch_bounds__(kThreads) void k_inv_gamma(Geo g, FftDesc fd, int K
   extern __shared__ double2 lds[];
   double2 *A = lds, *tw = lds + K * ld;
   fft_load_twiddles(tw, fd);
-  const int64_t S = (int64_t)g.nB * g.nA;
+  // columns p2 of X2 (rows of ldA) map to columns b nA + a of f
+  const int64_t S = (int64_t)g.nB * g.nA, S2 = (int64_t)g.nB * g.ldA;
   const int64_t p0 = (int64_t)blockIdx.x * K;
   const int nG = 2 * g.N - 1;
   for (int idx = threadIdx.x; idx < K * nG; idx += blockDim.x) {
     const int k = idx / K, s = idx - k * K;
     const int64_t p = p0 + s;
     double2 v = make_double2(0.0, 0.0);
-    if (p < S) {
+    if (p < S2 && p % g.ldA < g.nA) {
       if (!g.real) {
-        v = X2[k * S + p];
+        v = X2[k * S2 + p];
       } else if (k == 0) {
         v = make_double2(X2[p].x, 0.0);
       } else if (k < g.N) {
-        v = X2[k * S + p];
+        v = X2[k * S2 + p];
       } else {
-        v = cconj(X2[(nG - k) * S + p]);
+        v = cconj(X2[(nG - k) * S2 + p]);
       }
     }
     A[s * ld + k] = v;
@@ -321,8 +336,10 @@ __global__ __launch_bounds__(kThreads) void k_inv_gamma(Geo g, FftDesc fd, int K
   fft_run<+1>(A, K, ld, fd, tw);
   for (int idx = threadIdx.x; idx < K * nG; idx += blockDim.x) {
     const int k = idx / K, s = idx - k * K;
-    const int64_t p = p0 + s;
-    if (p >= S) continue;
+    const int64_t p2 = p0 + s;
+    const int a = (int)(p2 % g.ldA);
+    if (p2 >= S2 || a >= g.nA) continue;
+    const int64_t p = (p2 / g.ldA) * g.nA + a;
     if (g.real)
       fr[k * S + p] = A[s * ld + k].x * scale;
     else
@@ -338,7 +355,8 @@ __global__ __launch_bounds__(kThreads) void k_fwd_gamma(Geo g, FftDesc fd, int K
   extern __shared__ double2 lds[];
   double2 *A = lds, *tw = lds + K * ld;
   fft_load_twiddles(tw, fd);
-  const int64_t S = (int64_t)g.nB * g.nA;
+  // columns p of f map to columns (p / nA) ldA + p % nA of X2
+  const int64_t S = (int64_t)g.nB * g.nA, S2 = (int64_t)g.nB * g.ldA;
   const int64_t p0 = (int64_t)blockIdx.x * K;
   const int nG = 2 * g.N - 1;
   for (int idx = threadIdx.x; idx < K * nG; idx += blockDim.x) {
@@ -354,7 +372,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_gamma(Geo g, FftDesc fd, int K
   for (int idx = threadIdx.x; idx < K * nout; idx += blockDim.x) {
     const int k = idx / K, s = idx - k * K;
     const int64_t p = p0 + s;
-    if (p < S) X2[k * S + p] = cscale(A[s * ld + k], sc);
+    if (p < S) X2[k * S2 + (p / g.nA) * g.ldA + p % g.nA] = cscale(A[s * ld + k], sc);
   }
 }
 
@@ -389,7 +407,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_beta(Geo g, FftDesc fb, FftDes
       } else {
         if (b > L) { b = 2 * L - b; neg = ((m + n) & 1); zero = adjoint; }
       }
-      v = zero ? make_double2(0.0, 0.0) : X2[((int64_t)nslot * g.nB + b) * g.nA + mslot];
+      v = zero ? make_double2(0.0, 0.0) : X2[((int64_t)nslot * g.nB + b) * g.ldA + mslot];
       if (neg) v = make_double2(-v.x, -v.y);
     }
     A[s * ld + k] = v;
@@ -467,23 +485,24 @@ __global__ __launch_bounds__(kThreads) void k_fwd_beta(Geo g, FftDesc fb, FftDes
 // forward beta, register path (MW: Pb Bluestein with inner length 1024,
 // Pw = 1024): sequence (n slot, m slot) per wavefront, 8 consecutive m
 // slots per workgroup
-__global__ __launch_bounds__(kColThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_fwd_beta_reg(
+template <int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_fwd_beta_reg(
     Geo g, FftDesc fb, FftDesc fw, const double2 *__restrict__ X2, double2 *__restrict__ Gp,
     const int64_t *__restrict__ nbase, const double2 *__restrict__ phase, const double2 *__restrict__ wr,
     int adjoint) {
   extern __shared__ double2 lds[];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int chunks = (g.nA + kColWaves - 1) / kColWaves;
+  const int chunks = (g.nA + W - 1) / W;
   const int nslot = blockIdx.x / chunks;
-  const int ms0 = (blockIdx.x % chunks) * kColWaves;
+  const int ms0 = (blockIdx.x % chunks) * W;
   const int n = n_of_slot(g, nslot);
   if (nbase[n + g.N - 1] == kNoBlock) return;  // block not transformed (whole workgroup)
   const int Pb = g.Pb, L = g.L;
   // beta samples of the 8 sequences: tile[b][s]
-  for (int idx = threadIdx.x; idx < g.nB * kColWaves; idx += kColThreads) {
-    const int b = idx / kColWaves, c = idx - b * kColWaves;
-    lds[b * kTileLd + c] =
-        ms0 + c < g.nA ? X2[((int64_t)nslot * g.nB + b) * g.nA + ms0 + c] : make_double2(0.0, 0.0);
+  for (int idx = threadIdx.x; idx < g.nB * W; idx += (64 * W)) {
+    const int b = idx / W, c = idx - b * W;
+    lds[b * tile_ld<W>() + c] =
+        ms0 + c < g.nA ? X2[((int64_t)nslot * g.nB + b) * g.ldA + ms0 + c] : make_double2(0.0, 0.0);
   }
   __syncthreads();
   const int mslot = ms0 + wv;
@@ -497,9 +516,9 @@ __global__ __launch_bounds__(kColThreads) __attribute__((amdgpu_waves_per_eu(2, 
     double2 x = make_double2(0.0, 0.0);
     if (k < Pb) {
       if (k < L) {
-        x = lds[k * kTileLd + wv];
+        x = lds[k * tile_ld<W>() + wv];
       } else if (!adjoint) {
-        x = lds[(2 * L - 2 - k) * kTileLd + wv];
+        x = lds[(2 * L - 2 - k) * tile_ld<W>() + wv];
         if (flip) x = make_double2(-x.x, -x.y);
       }
     }
@@ -633,7 +652,7 @@ __global__ __launch_bounds__(kThreads) void k_adjf_beta(Geo g, FftDesc fb, FftDe
       const double2 e = A[s * ld + (2 * L - 2 - b)];
       v = ((m + n) & 1) ? csub(v, e) : cadd(v, e);
     }
-    X2[((int64_t)nslot * g.nB + b) * g.nA + mslot] = v;
+    X2[((int64_t)nslot * g.nB + b) * g.ldA + mslot] = v;
   }
 }
 
@@ -712,9 +731,11 @@ void launch_inv_beta(const Geo &g, const double2 *X1, double2 *X2, NBase nb, con
                      hipStream_t st) {
   const FftDesc fd = fft_plan(g.Pb);
   if (fd.len == 1024 && fd.blue && g.nB <= 1024) {
-    set_lds(k_inv_beta_reg, kColShared);
-    const int chunks = (g.nA + kColWaves - 1) / kColWaves;
-    k_inv_beta_reg<<<g.nG * chunks, kColThreads, kColShared, st>>>(g, fd, X1, X2, nb.base, phase);
+    constexpr int W = kBetaW;
+    const size_t sh = col_shared<W>(g.nB);
+    set_lds(k_inv_beta_reg<W>, sh);
+    const int chunks = (g.nA + W - 1) / W;
+    k_inv_beta_reg<W><<<g.nG * chunks, 64 * W, sh, st>>>(g, fd, X1, X2, nb.base, phase);
     SO3_LAUNCH_CHECK();
     return;
   }
@@ -728,14 +749,14 @@ void launch_inv_beta(const Geo &g, const double2 *X1, double2 *X2, NBase nb, con
   SO3_LAUNCH_CHECK();
 }
 
-void launch_rows(double2 *X, int64_t nrows, int n, int dir, double scale, hipStream_t st) {
+void launch_rows(double2 *X, int64_t nrows, int n, int ldr, int dir, double scale, hipStream_t st) {
   const FftDesc fd = fft_plan(n);
   if (nrows <= 0) return;
   if (fd.len == 1024) {
     const unsigned grid = (unsigned)((nrows + kRegWaves - 1) / kRegWaves);
     auto go = [&](auto kern) {
       set_lds(kern, kRegShared);
-      kern<<<grid, kThreads, kRegShared, st>>>(fd, X, nrows, n, scale);
+      kern<<<grid, kThreads, kRegShared, st>>>(fd, X, nrows, n, (int64_t)ldr, scale);
     };
     if (fd.blue)
       dir > 0 ? go(k_rows_reg<+1, 1>) : go(k_rows_reg<-1, 1>);
@@ -751,29 +772,31 @@ void launch_rows(double2 *X, int64_t nrows, int n, int dir, double scale, hipStr
   const unsigned grid = (unsigned)((nrows + K - 1) / K);
   if (dir > 0) {
     set_lds(k_rows<+1>, sh);
-    k_rows<+1><<<grid, kThreads, sh, st>>>(fd, K, ld, X, nrows, n, scale);
+    k_rows<+1><<<grid, kThreads, sh, st>>>(fd, K, ld, X, nrows, n, (int64_t)ldr, scale);
   } else {
     set_lds(k_rows<-1>, sh);
-    k_rows<-1><<<grid, kThreads, sh, st>>>(fd, K, ld, X, nrows, n, scale);
+    k_rows<-1><<<grid, kThreads, sh, st>>>(fd, K, ld, X, nrows, n, (int64_t)ldr, scale);
   }
   SO3_LAUNCH_CHECK();
 }
 
 void launch_alpha(const Geo &g, double2 *X2, int dir, double scale, hipStream_t st) {
-  launch_rows(X2, (int64_t)g.nG * g.nB, g.nA, dir, scale, st);
+  launch_rows(X2, (int64_t)g.nG * g.nB, g.nA, g.ldA, dir, scale, st);
 }
 
 void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, double scale, hipStream_t st) {
   const FftDesc fd = fft_plan(2 * g.N - 1);
   if (fd.len == 1024) {
-    const int64_t S = (int64_t)g.nB * g.nA;
-    const unsigned grid = (unsigned)((S + kColWaves - 1) / kColWaves);
+    constexpr int W = kGammaW;
+    const int64_t S2 = (int64_t)g.nB * g.ldA;
+    const unsigned grid = (unsigned)((S2 + W - 1) / W);
+    const size_t sh = col_shared<W>(fd.blue ? fd.n : 1024);
     auto go = [&](auto kern) {
-      set_lds(kern, kColShared);
-      kern<<<grid, kColThreads, kColShared, st>>>(g, fd, X2, g.real ? nullptr : (double2 *)f,
+      set_lds(kern, sh);
+      kern<<<grid, 64 * W, sh, st>>>(g, fd, X2, g.real ? nullptr : (double2 *)f,
                                                    g.real ? (double *)f : nullptr, scale);
     };
-    fd.blue ? go(k_inv_gamma_reg<1>) : go(k_inv_gamma_reg<0>);
+    fd.blue ? go(k_inv_gamma_reg<W, 1>) : go(k_inv_gamma_reg<W, 0>);
     SO3_LAUNCH_CHECK();
     return;
   }
@@ -782,8 +805,8 @@ void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, double scale, hi
   const int K = choose_k(ld, fft_capacity(fd), fd.len, tb, 16);
   const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
   set_lds(k_inv_gamma, sh);
-  const int64_t S = (int64_t)g.nB * g.nA;
-  k_inv_gamma<<<(unsigned)((S + K - 1) / K), kThreads, sh, st>>>(
+  const int64_t S2 = (int64_t)g.nB * g.ldA;
+  k_inv_gamma<<<(unsigned)((S2 + K - 1) / K), kThreads, sh, st>>>(
       g, fd, K, ld, X2, g.real ? nullptr : (double2 *)f, g.real ? (double *)f : nullptr, scale);
   SO3_LAUNCH_CHECK();
 }
@@ -791,14 +814,16 @@ void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, double scale, hi
 void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, double scale, hipStream_t st) {
   const FftDesc fd = fft_plan(2 * g.N - 1);
   if (fd.len == 1024) {
+    constexpr int W = kGammaW;
     const int64_t S = (int64_t)g.nB * g.nA;
-    const unsigned grid = (unsigned)((S + kColWaves - 1) / kColWaves);
+    const unsigned grid = (unsigned)((S + W - 1) / W);
+    const size_t sh = col_shared<W>(fd.blue ? fd.n : 1024);
     auto go = [&](auto kern) {
-      set_lds(kern, kColShared);
-      kern<<<grid, kColThreads, kColShared, st>>>(g, fd, g.real ? nullptr : (const double2 *)f,
+      set_lds(kern, sh);
+      kern<<<grid, 64 * W, sh, st>>>(g, fd, g.real ? nullptr : (const double2 *)f,
                                                    g.real ? (const double *)f : nullptr, X2, scale);
     };
-    fd.blue ? go(k_fwd_gamma_reg<1>) : go(k_fwd_gamma_reg<0>);
+    fd.blue ? go(k_fwd_gamma_reg<W, 1>) : go(k_fwd_gamma_reg<W, 0>);
     SO3_LAUNCH_CHECK();
     return;
   }
@@ -817,10 +842,11 @@ void launch_fwd_beta(const Geo &g, const double2 *X2, double2 *Gp, NBase nb, con
                      const double2 *wr, int Pw, int adjoint, hipStream_t st) {
   if (!g.ss && Pw == 1024 && fft_plan(g.Pb).len == 1024 && fft_plan(g.Pb).blue && g.nB <= 1024) {
     const FftDesc fb = fft_plan(g.Pb), fw = fft_plan(Pw);
-    set_lds(k_fwd_beta_reg, kColShared);
-    const int chunks = (g.nA + kColWaves - 1) / kColWaves;
-    k_fwd_beta_reg<<<g.nG * chunks, kColThreads, kColShared, st>>>(g, fb, fw, X2, Gp, nb.base, phase, wr,
-                                                                    adjoint);
+    constexpr int W = kBetaW;
+    const size_t sh = col_shared<W>(g.nB);
+    set_lds(k_fwd_beta_reg<W>, sh);
+    const int chunks = (g.nA + W - 1) / W;
+    k_fwd_beta_reg<W><<<g.nG * chunks, 64 * W, sh, st>>>(g, fb, fw, X2, Gp, nb.base, phase, wr, adjoint);
     SO3_LAUNCH_CHECK();
     return;
   }

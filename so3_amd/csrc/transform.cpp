@@ -37,8 +37,7 @@ struct Plan {
   double2 *phase_af = nullptr;   // 2L-1: exp(+i m' pi/(2L-1)) / Pb, adjoint of the forward
   double2 *wr = nullptr;         // Pw entries
   int64_t *nbase = nullptr;      // 2*(2N-1): masked, then unmasked
-  int64_t *poff = nullptr;       // packed inverse stream offsets, N*L
-  size_t x1 = 0, x2 = 0, x3 = 0; // workspace sizes (double2)
+  size_t x1 = 0, x2 = 0;         // workspace sizes (double2)
 };
 
 // Length of the weight-convolution DFTs: the smallest 2^a or 3 * 2^a that is
@@ -74,6 +73,9 @@ Geo make_geo(const so3_parameters_t *p, Method method) {
   g.ss = (method == kViaSsht && p->sampling_scheme == SO3_SAMPLING_MW_SS) ? 1 : 0;
   g.real = p->reality ? 1 : 0;
   g.nA = g.ss ? 2 * g.L : 2 * g.L - 1;
+  // X2 rows unpadded: the gamma stages then read and write 128-byte runs of
+  // the flat layout on both sides (padding to 8 was measured slower overall)
+  g.ldA = g.nA;
   g.nB = g.ss ? g.L + 1 : g.L;
   g.nG = g.real ? g.N : 2 * g.N - 1;
   g.Pb = g.ss ? 2 * g.L : 2 * g.L - 1;
@@ -189,15 +191,7 @@ Plan build_plan(const so3_parameters_t *p, Method method) {
   // the forward contraction copies G' in 4-row chunks: up to 3 rows past the
   // last pair's stream
   P.x1 = std::max<size_t>((size_t)g.nG * g.nA * L, (size_t)N * L * L * 2 * g.NS + 8 * g.NS);
-  P.x2 = (size_t)g.nG * g.nB * g.nA;
-  {
-    std::vector<int64_t> po((size_t)N * L);
-    P.x3 = (size_t)pack_inv_size(g, po.data());
-    int64_t *d = nullptr;
-    SO3_HIP(hipMalloc(&d, sizeof(int64_t) * po.size()));
-    SO3_HIP(hipMemcpy(d, po.data(), sizeof(int64_t) * po.size(), hipMemcpyHostToDevice));
-    P.poff = d;
-  }
+  P.x2 = (size_t)g.nG * g.nB * g.ldA;
   // DFT plans and the zero page are made now so that a timed (or captured)
   // call does not allocate
   zero_page();
@@ -317,7 +311,7 @@ size_t workspace_bytes(const so3_parameters_t *p, Method method) {
   validate(p, method);
   std::lock_guard<std::mutex> lk(g_mu);
   const Plan &P = get_plan(p, method);
-  return sizeof(double2) * (P.x1 + P.x2 + P.x3);
+  return sizeof(double2) * (P.x1 + P.x2);
 }
 
 void prepare(const so3_parameters_t *p, Method method) {
@@ -332,10 +326,10 @@ namespace {
 void enqueue_inverse_locked(const Plan &P, const so3_parameters_t *p, const void *flmn, void *f, double2 *ws,
                             hipStream_t st) {
   const DlTables &dl = dl_tables(p->L, (int)p->dl_method);
-  double2 *X1 = ws, *X2 = ws + P.x1, *Gi = ws + P.x1 + P.x2;
+  double2 *X1 = ws, *X2 = ws + P.x1;
   const Geo &g = P.g;
   NBase nb{P.nbase};
-  timed(0, st, [&] { launch_contract_inv(g, (const double2 *)flmn, nb, dl, P.poff, Gi, X1, st); });
+  timed(0, st, [&] { launch_contract_inv(g, (const double2 *)flmn, nb, dl, X1, st); });
   timed(1, st, [&] { launch_inv_beta(g, X1, X2, nb, P.phase_inv, st); });
   timed(2, st, [&] { launch_alpha(g, X2, +1, 1.0, st); });
   timed(3, st, [&] { launch_inv_gamma(g, X2, f, 1.0, st); });
@@ -362,7 +356,7 @@ void run_inverse(const so3_parameters_t *p, Method method, const void *flmn, voi
   validate(p, method);
   std::lock_guard<std::mutex> lk(g_mu);
   const Plan &P = get_plan(p, method);
-  enqueue_inverse_locked(P, p, flmn, f, arena(P.x1 + P.x2 + P.x3), st);
+  enqueue_inverse_locked(P, p, flmn, f, arena(P.x1 + P.x2), st);
 }
 
 void run_forward(const so3_parameters_t *p, Method method, const void *f, void *flmn, hipStream_t st) {
@@ -377,7 +371,7 @@ size_t workspace_elems(const so3_parameters_t *p, Method method) {
   std::lock_guard<std::mutex> lk(g_mu);
   const Plan &P = get_plan(p, method);
   dl_tables(p->L, (int)p->dl_method);
-  return P.x1 + P.x2 + P.x3;
+  return P.x1 + P.x2;
 }
 
 void enqueue_inverse(const so3_parameters_t *p, Method method, const void *flmn, void *f, void *ws,
@@ -421,7 +415,7 @@ void run_adjoint_inverse(const so3_parameters_t *p, const void *f, void *flmn, h
   std::lock_guard<std::mutex> lk(g_mu);
   const Plan &P = get_plan(p, kDirect);
   const DlTables &dl = dl_tables(p->L, (int)p->dl_method);
-  double2 *ws = arena(P.x1 + P.x2 + P.x3);
+  double2 *ws = arena(P.x1 + P.x2);
   double2 *Gp = ws, *X2 = ws + P.x1;
   Geo g = P.g;
   NBase nb{P.nbase};
@@ -439,12 +433,12 @@ void run_adjoint_forward(const so3_parameters_t *p, const void *flmn, void *f, h
   std::lock_guard<std::mutex> lk(g_mu);
   const Plan &P = get_plan(p, kDirect);
   const DlTables &dl = dl_tables(p->L, (int)p->dl_method);
-  double2 *ws = arena(P.x1 + P.x2 + P.x3);
-  double2 *X1 = ws, *X2 = ws + P.x1, *Gi = ws + P.x1 + P.x2;
+  double2 *ws = arena(P.x1 + P.x2);
+  double2 *X1 = ws, *X2 = ws + P.x1;
   Geo g = P.g;
   g.clmode = 0;
   NBase nb{P.nbase};
-  launch_contract_inv(g, (const double2 *)flmn, nb, dl, P.poff, Gi, X1, st);
+  launch_contract_inv(g, (const double2 *)flmn, nb, dl, X1, st);
   launch_adjf_beta(g, X1, X2, nb, P.phase_af, P.wr, P.Pw, st);
   launch_alpha(g, X2, +1, 1.0 / g.nA, st);
   launch_inv_gamma(g, X2, f, 1.0 / (2.0 * g.N - 1.0), st);
@@ -460,7 +454,6 @@ void release_all() {
     (void)hipFree(kv.second.phase_af);
     (void)hipFree(kv.second.wr);
     (void)hipFree(kv.second.nbase);
-    (void)hipFree(kv.second.poff);
   }
   g_plans.clear();
   for (auto &kv : g_arena) (void)hipFree(kv.second.buf);
