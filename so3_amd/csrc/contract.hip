@@ -304,8 +304,10 @@ __global__ __launch_bounds__(256) void k_contract_fwd(Geo g, const double *__res
   __shared__ __attribute__((aligned(16))) double sm[kRing * kBufD];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  // l blocks are the slowest grid dimension, highest first: their work grows
+  // with l, so the longest workgroups start first and the tail is short
   const int nlb = (g.L + 63) >> 6;
-  const int X = blockIdx.x / nlb, lb = blockIdx.x % nlb;
+  const int X = blockIdx.x, lb = nlb - 1 - (int)blockIdx.z;
   const int Np = min(g.N, g.L);
   const int x0 = X * kXG, y0 = blockIdx.y * 4;
   const int lo = lb * 64;
@@ -458,7 +460,7 @@ void launch_contract_fwd(const Geo &g, const double2 *Gp, NBase nb, const DlTabl
                          double2 *flmn, hipStream_t st) {
   const int Np = std::min(g.N, g.L);
   const int nlb = (g.L + 63) / 64;
-  dim3 grid(nlb * ((g.L + kXG - 1) / kXG), (Np + 3) / 4);
+  dim3 grid((g.L + kXG - 1) / kXG, (Np + 3) / 4, nlb);
   if (g.NS == 2)
     k_contract_fwd<2><<<grid, 256, 0, st>>>(g, (const double *)Gp, nb.base, dl.tf, dl.offf, zero_page(), flmn);
   else
