@@ -78,8 +78,11 @@ FftDesc build(int n) {
     return d;
   }
   // a smooth length runs directly when one workgroup holds it in registers
-  // (radix 5 and 7 passes hold fewer points per thread), else via Bluestein
-  bool direct = factor(n, rad);
+  // (radix 5 and 7 passes hold fewer points per thread), else via Bluestein;
+  // every n in [257, 512] takes Bluestein with M = 1024, which runs on the
+  // register-resident FFT (fft_reg.hpp) faster than the workgroup engine
+  // runs even a smooth n directly
+  bool direct = factor(n, rad) && !(n >= 257 && n <= 512);
   if (direct) {
     int cap = kFftThreads * kFftElems * 4;
     for (int R : rad) cap = std::min(cap, kFftThreads * fft_nb(R) * R);

@@ -482,9 +482,9 @@ __global__ __launch_bounds__(kThreads) void k_fwd_beta(Geo g, FftDesc fb, FftDes
   }
 }
 
-// forward beta, register path (MW: Pb Bluestein with inner length 1024,
-// Pw = 1024): sequence (n slot, m slot) per wavefront, 8 consecutive m
-// slots per workgroup
+// forward beta, register path (Pb Bluestein with inner length 1024,
+// Pw = 1024; MW and MW_SS): sequence (n slot, m slot) per wavefront, W
+// consecutive m slots per workgroup
 template <int W>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_fwd_beta_reg(
     Geo g, FftDesc fb, FftDesc fw, const double2 *__restrict__ X2, double2 *__restrict__ Gp,
@@ -515,10 +515,12 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) 
     const int k = lane + 64 * q;
     double2 x = make_double2(0.0, 0.0);
     if (k < Pb) {
-      if (k < L) {
+      // MW: F(2L-2-b) for b >= L; MW_SS: F(2L-b) for b > L
+      const int kr = g.ss ? 2 * L - k : 2 * L - 2 - k;
+      if (g.ss ? k <= L : k < L) {
         x = lds[k * tile_ld<W>() + wv];
       } else if (!adjoint) {
-        x = lds[(2 * L - 2 - k) * tile_ld<W>() + wv];
+        x = lds[kr * tile_ld<W>() + wv];
         if (flip) x = make_double2(-x.x, -x.y);
       }
     }
@@ -840,7 +842,7 @@ void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, double scale, hi
 
 void launch_fwd_beta(const Geo &g, const double2 *X2, double2 *Gp, NBase nb, const double2 *phase,
                      const double2 *wr, int Pw, int adjoint, hipStream_t st) {
-  if (!g.ss && Pw == 1024 && fft_plan(g.Pb).len == 1024 && fft_plan(g.Pb).blue && g.nB <= 1024) {
+  if (Pw == 1024 && fft_plan(g.Pb).len == 1024 && fft_plan(g.Pb).blue && g.nB <= 1024) {
     const FftDesc fb = fft_plan(g.Pb), fw = fft_plan(Pw);
     constexpr int W = kBetaW;
     const size_t sh = col_shared<W>(g.nB);

@@ -41,10 +41,10 @@ REAL_COEFS = [(255, 255, 255), (255, -3, 254), (200, -17, 33), (128, 0, 0), (97,
               (1, 1, 1), (254, -200, 7), (60, 5, 60)]
 
 
-@pytest.mark.parametrize("real", [0, 1])
-def test_l256_sparse_inverse_and_round_trip(oracle, real):
+@pytest.mark.parametrize("real,ss", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_l256_sparse_inverse_and_round_trip(oracle, real, ss):
     L = N = 256
-    p = P.SO3Parameters(L=L, N=N, reality=real, n_order=P.SO3_N_ORDER_ZERO_FIRST)
+    p = P.SO3Parameters(L=L, N=N, reality=real, sampling_scheme=ss, n_order=P.SO3_N_ORDER_ZERO_FIRST)
     rng = np.random.default_rng(7 + real)
     flmn = np.zeros(P.flmn_size(p), dtype=np.complex128)
     coefs = REAL_COEFS if real else COEFS
@@ -56,7 +56,7 @@ def test_l256_sparse_inverse_and_round_trip(oracle, real):
         vals[(l, m, n)] = v
         flmn[P.elmn2ind_vec(np.array([l]), np.array([m]), np.array([n]), p)[0]] = v
     f = so3.inverse(flmn, p)
-    na, nb, ng = 2 * L - 1, L, 2 * N - 1
+    na, nb, ng = (2 * L, L + 1, 2 * N - 1) if ss else (2 * L - 1, L, 2 * N - 1)
     assert f.size == na * nb * ng
     T = oracle.dl_quarter(L, 0)
     Ds = {l: _full(T[l], l) for l in {c[0] for c in coefs}}
@@ -64,7 +64,8 @@ def test_l256_sparse_inverse_and_round_trip(oracle, real):
     fmax = 0.0
     err = 0.0
     for a, b, g in pts:
-        al, be, ga = 2 * np.pi * a / na, (2 * b + 1) * np.pi / (2 * L - 1), 2 * np.pi * g / ng
+        be = np.pi * b / L if ss else (2 * b + 1) * np.pi / (2 * L - 1)
+        al, ga = 2 * np.pi * a / na, 2 * np.pi * g / ng
         ref = 0j
         for (l, m, n), c in vals.items():
             t = c * (2 * l + 1) / (8 * np.pi ** 2) * np.exp(1j * m * al) * \
@@ -83,7 +84,8 @@ def test_l256_sparse_inverse_and_round_trip(oracle, real):
 # shapes of the batched workload's largest items (Bluestein inner lengths 2048,
 # Pw = 1536, N = 1) and a long-alpha case, checked by the round trip
 @pytest.mark.parametrize("L,N,ss,real", [(384, 64, 0, 0), (384, 64, 1, 1), (368, 17, 0, 1),
-                                         (320, 1, 1, 0), (512, 4, 0, 0), (32, 32, 1, 0)])
+                                         (320, 1, 1, 0), (512, 4, 0, 0), (32, 32, 1, 0),
+                                         (256, 256, 1, 0), (200, 200, 1, 1), (129, 150, 0, 0)])
 def test_round_trip_batch_shapes(L, N, ss, real):
     from so3_amd import synth
     p = P.SO3Parameters(L=L, N=N, sampling_scheme=ss, reality=real, n_order=P.SO3_N_ORDER_ZERO_FIRST)
