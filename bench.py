@@ -210,6 +210,65 @@ def serving_bench(A, rates=(50, 100, 200), count=120):
     return out
 
 
+def variants_bench(A, P, torch, steps=5):
+    """The other configurations the metric is quoted with: C1 variants
+    (Trapani Delta, MW_SS grid, real signal) as fwd+inv pairs, and at L=1024
+    the s2toso3 convolution followed by the inverse (N=8) and a fwd+inv pair
+    (N=4).  Median milliseconds per item over `steps` runs after a warm-up."""
+    from so3_amd import synth
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    sid = stream.cuda_stream
+    out = {}
+
+    def timeit(fn):
+        fn()
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+        ev[0].record(stream)
+        for k in range(steps):
+            fn()
+            ev[k + 1].record(stream)
+        torch.cuda.synchronize()
+        return float(np.median([ev[k].elapsed_time(ev[k + 1]) for k in range(steps)]))
+
+    def pair(name, p):
+        A.prepare(p)
+        flmn = torch.from_numpy(synth.gen_flmn(p, 1)).to(dev)
+        back = torch.zeros_like(flmn)
+        f = torch.empty(A.f_size(p), dtype=torch.float64 if p.reality else torch.complex128, device=dev)
+
+        def step():
+            A.inverse_device(f.data_ptr(), flmn.data_ptr(), p, A.VIA_SSHT, sid)
+            A.forward_device(back.data_ptr(), f.data_ptr(), p, A.VIA_SSHT, sid)
+        ms = timeit(step)
+        err = (back - flmn).abs().max().item() / flmn.abs().max().item()
+        out[name] = {"ms_per_pair": ms, "pairs_per_s": 1000.0 / ms, "roundtrip_rel_err": err}
+
+    z = P.SO3_N_ORDER_ZERO_FIRST
+    pair("C1_trapani", P.SO3Parameters(L=256, N=256, dl_method=P.SSHT_DL_TRAPANI, n_order=z))
+    pair("C1_mw_ss", P.SO3Parameters(L=256, N=256, sampling_scheme=P.SO3_SAMPLING_MW_SS, n_order=z))
+    pair("C1_real", P.SO3Parameters(L=256, N=256, reality=1, n_order=z))
+    pair("C2b_L1024_N4", P.SO3Parameters(L=1024, N=4, n_order=z))
+    # C2: h = s2toso3(flm, glm) at L=1024, N=8, then the inverse transform of h
+    hp = P.SO3Parameters(L=1024, N=8, n_order=z)
+    A.prepare(hp)
+    g = torch.Generator(device=dev)
+    g.manual_seed(2)
+    L2 = hp.L * hp.L
+    flm = torch.complex(torch.rand(L2, generator=g, device=dev, dtype=torch.float64),
+                        torch.rand(L2, generator=g, device=dev, dtype=torch.float64))
+    glm = torch.complex(torch.rand(L2, generator=g, device=dev, dtype=torch.float64),
+                        torch.rand(L2, generator=g, device=dev, dtype=torch.float64))
+    h = torch.zeros(A.flmn_size(hp), dtype=torch.complex128, device=dev)
+    f = torch.empty(A.f_size(hp), dtype=torch.complex128, device=dev)
+    items = A.make_items([{"p": hp, "kind": A.ITEM_S2TOSO3_INVERSE, "f": f.data_ptr(),
+                           "flmn_in": flm.data_ptr(), "glm": glm.data_ptr(), "flmn_out": h.data_ptr()}])
+    ms = timeit(lambda: A.batch(items, lanes=1, stream=sid))
+    out["C2_conv_inverse_L1024_N8"] = {"ms": ms, "per_s": 1000.0 / ms}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -224,6 +283,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with a 1 GiB device-to-device copy (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -337,6 +397,8 @@ def main():
         "path_roofline": {"t_min_ms": t_min * 1e3, "frac": t_min * 1e3 / ms_step,
                           "bytes_alg": bytes_alg, "contraction_flops": 2 * fl_contract},
     }
+    if not args.no_variants:
+        out["variants"] = variants_bench(A, P, torch)
     if not args.no_batch:
         out["batched"] = batched_bench(A, P, torch)
     if not args.no_serve:

@@ -385,14 +385,15 @@ __global__ __launch_bounds__(kThreads) void k_fwd_beta(Geo g, FftDesc fb, FftDes
                                                        const double2 *__restrict__ wr, int Pw,
                                                        int adjoint) {
   extern __shared__ double2 lds[];
-  double2 *A = lds, *twb = lds + K * ld, *tww = twb + fb.len;
+  // one twiddle table serves both transforms when their lengths agree
+  double2 *A = lds, *twb = lds + K * ld, *tww = fw.len == fb.len ? twb : twb + fb.len;
   const int chunks = (g.nA + K - 1) / K;
   const int nslot = blockIdx.x / chunks;
   const int ms0 = (blockIdx.x % chunks) * K;
   const int n = n_of_slot(g, nslot);
   if (nbase[n + g.N - 1] == kNoBlock) return;  // block not transformed
   fft_load_twiddles(twb, fb);
-  fft_load_twiddles(tww, fw);
+  if (tww != twb) fft_load_twiddles(tww, fw);
   const int Pb = g.Pb, L = g.L;
   for (int idx = threadIdx.x; idx < K * Pb; idx += blockDim.x) {
     const int k = idx / K, s = idx - k * K;
@@ -584,7 +585,8 @@ __global__ __launch_bounds__(kThreads) void k_adjf_beta(Geo g, FftDesc fb, FftDe
                                                         const double2 *__restrict__ phase,
                                                         const double2 *__restrict__ wr, int Pw) {
   extern __shared__ double2 lds[];
-  double2 *A = lds, *twb = lds + K * ld, *tww = twb + fb.len;
+  // one twiddle table serves both transforms when their lengths agree
+  double2 *A = lds, *twb = lds + K * ld, *tww = fw.len == fb.len ? twb : twb + fb.len;
   const int chunks = (g.nA + K - 1) / K;
   const int nslot = blockIdx.x / chunks;
   const int ms0 = (blockIdx.x % chunks) * K;
@@ -592,7 +594,7 @@ __global__ __launch_bounds__(kThreads) void k_adjf_beta(Geo g, FftDesc fb, FftDe
   const bool active = nbase[n + g.N - 1] != kNoBlock;
   const int Pb = g.Pb, L = g.L;
   fft_load_twiddles(twb, fb);
-  fft_load_twiddles(tww, fw);
+  if (tww != twb) fft_load_twiddles(tww, fw);
   // G(m') for |m'| <= L-1 at m' mod Pw, F(-m') = (-1)^{m+n} F(m')
   for (int idx = threadIdx.x; idx < K * Pw; idx += blockDim.x) {
     const int s = idx / Pw, r = idx - s * Pw;
@@ -855,7 +857,7 @@ void launch_fwd_beta(const Geo &g, const double2 *X2, double2 *Gp, NBase nb, con
   const FftDesc fb = fft_plan(g.Pb);
   const FftDesc fw = fft_plan(Pw);
   const int ld = std::max(fft_ld(fb), fft_ld(fw));
-  const int tb = (fb.len + fw.len) * (int)sizeof(double2);
+  const int tb = (fb.len + (fw.len == fb.len ? 0 : fw.len)) * (int)sizeof(double2);
   const int K = choose_k2(ld, fb, fw, tb, 8);
   const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
   set_lds(k_fwd_beta, sh);
@@ -869,7 +871,7 @@ void launch_adjf_beta(const Geo &g, const double2 *X1, double2 *X2, NBase nb, co
   const FftDesc fb = fft_plan(g.Pb);
   const FftDesc fw = fft_plan(Pw);
   const int ld = std::max(fft_ld(fb), fft_ld(fw));
-  const int tb = (fb.len + fw.len) * (int)sizeof(double2);
+  const int tb = (fb.len + (fw.len == fb.len ? 0 : fw.len)) * (int)sizeof(double2);
   const int K = choose_k2(ld, fb, fw, tb, 8);
   const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
   set_lds(k_adjf_beta, sh);

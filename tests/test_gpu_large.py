@@ -85,7 +85,8 @@ def test_l256_sparse_inverse_and_round_trip(oracle, real, ss):
 # Pw = 1536, N = 1) and a long-alpha case, checked by the round trip
 @pytest.mark.parametrize("L,N,ss,real", [(384, 64, 0, 0), (384, 64, 1, 1), (368, 17, 0, 1),
                                          (320, 1, 1, 0), (512, 4, 0, 0), (32, 32, 1, 0),
-                                         (256, 256, 1, 0), (200, 200, 1, 1), (129, 150, 0, 0)])
+                                         (256, 256, 1, 0), (200, 200, 1, 1), (129, 150, 0, 0),
+                                         (1024, 4, 0, 0)])
 def test_round_trip_batch_shapes(L, N, ss, real):
     from so3_amd import synth
     p = P.SO3Parameters(L=L, N=N, sampling_scheme=ss, reality=real, n_order=P.SO3_N_ORDER_ZERO_FIRST)
