@@ -269,6 +269,42 @@ def variants_bench(A, P, torch, steps=5):
     return out
 
 
+def cpu_baseline_mt(p, per_thread=3):
+    """The same oracle work spread over the host's cores: one thread per core
+    (ctypes releases the GIL; every oracle call has its own workspace), each
+    timing `per_thread` n slices; throughput = slices / wall time, scaled to
+    all 2N-1 slices, plus the Delta table (built once) and the gamma DFTs
+    divided over the threads."""
+    import concurrent.futures
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    N = p.N
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    ns_all = np.unique(np.linspace(-(N - 1), N - 1, threads * per_thread).round().astype(int))
+    chunks = [ns_all[i::threads] for i in range(threads)]
+    stride = (2 * p.L - 1) * p.L
+    t_dl, t_g, _ = 0.0, 0.0, None
+    t_dl_one = pyoracle.time_sample(p, ns_all[:1], gamma_every=16)
+    t_dl, t_g = t_dl_one[1], t_dl_one[2]
+    t0 = time.time()
+    with concurrent.futures.ThreadPoolExecutor(threads) as ex:
+        res = list(ex.map(lambda c: pyoracle.time_sample(p, c, gamma_every=stride), chunks))
+    wall = time.time() - t0
+    # each thread also rebuilt the Delta table: take it out of the wall time
+    slices_wall = max(0.0, wall - max(r[1] for r in res))
+    per_slice = slices_wall / max(1, max(len(c) for c in chunks))
+    total = t_dl + per_slice * (2 * N - 1) / threads + 2 * t_g / threads
+    return {
+        "value": 1.0 / total,
+        "unit": "fwd+inv transforms/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "%d n-slices over %d threads (%d each) at L=N=%d, %.1f s wall; Delta table once, gamma DFTs "
+                  "divided over the threads; estimated %.1f s per fwd+inv" % (len(ns_all), threads, per_thread,
+                                                                              p.L, wall, total),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -405,6 +441,7 @@ def main():
         out["serving"] = serving_bench(A)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, args.cpu_slices)
+        out["cpu_baseline_mt"] = cpu_baseline_mt(p)
     print(json.dumps(out))
     if args.copy_calibration:
         a = torch.ones(1 << 27, dtype=torch.float64, device=dev)
