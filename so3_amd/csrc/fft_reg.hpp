@@ -13,15 +13,23 @@
 // both unnormalised with kernel exp(s 2 pi i j k / 1024).  Bluestein's
 // convolution F_{-1} -> kernel product -> G_{+1} and a chain such as
 // F_{+1} -> weights -> G_{-1} therefore run on registers with one LDS
-// transpose per transform.  Transpose rows are padded to 68 entries so
-// that neither the writes nor the reads conflict in LDS banks.
+// transpose per transform.  A transpose moves the real parts and then the
+// imaginary parts through a buffer of 16 rows x 68 doubles (half the LDS of
+// a complex one); the padding keeps both the writes and the reads free of
+// bank conflicts.  The twiddle tables are read from an LDS copy staged once
+// per workgroup (reg_stage_tw): as global loads they were most of the
+// vector-memory traffic of these kernels.
 #pragma once
 #include "fft_device.hpp"
 
 namespace so3g {
 
-constexpr int kRegLd = 68;            // LDS row stride (double2) of the transpose buffer
-constexpr int kRegLds = 16 * kRegLd;  // double2 per wavefront
+constexpr int kRegLd = 68;            // row stride (doubles) of the transpose buffer
+constexpr int kRegT = 16 * kRegLd;    // doubles per wavefront (8704 bytes)
+constexpr int kRegTw = 1024 + 64;     // double2 of the twiddle tables tw1 | tw2
+// double2 of the Bluestein tables: tw1 | tw2 | kernel spectrum (1024) | chirp
+// (n <= 512 entries: a length-1024 Bluestein plan has n <= 512)
+constexpr int kRegBlueTab = kRegTw + 1024 + 512;
 
 __device__ __forceinline__ int bitrev2(int a) { return ((a & 1) << 1) | (a >> 1); }
 
@@ -72,22 +80,78 @@ __device__ __forceinline__ double2 tw_load(const double2 *t, int i) {
   return w;
 }
 
+// copies the twiddle tables of a length-1024 descriptor (tw1 | tw2) to LDS;
+// every thread of the block takes part, the caller synchronises
+__device__ __forceinline__ void reg_stage_tw(double2 *dst, const FftDesc &d) {
+  for (int i = threadIdx.x; i < kRegTw; i += blockDim.x) dst[i] = d.tws[i];
+}
+
+// copies the tables of a Bluestein descriptor with inner length 1024 and
+// sign DIR to LDS (layout of kRegBlueTab); the caller synchronises
+template <int DIR>
+__device__ __forceinline__ void reg_stage_blue(double2 *dst, const FftDesc &d) {
+  const double2 *bh = DIR < 0 ? d.bhat_f : d.bhat_b;
+  for (int i = threadIdx.x; i < kRegTw; i += blockDim.x) dst[i] = d.tws[i];
+  for (int i = threadIdx.x; i < 1024; i += blockDim.x) dst[kRegTw + i] = bh[i];
+  for (int i = threadIdx.x; i < d.n; i += blockDim.x) dst[kRegTw + 1024 + i] = d.chirp[i];
+}
+
+// v[k] of lane l goes to row k, column l; lane 4 k1 + a then takes row k1,
+// columns 4 j + a as v[j]
+__device__ __forceinline__ void xpose_to_quads(double2 *v, double *T, int lane) {
+  const int rd = (lane >> 2) * kRegLd + (lane & 3);
+  double t[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) T[k * kRegLd + lane] = v[k].x;
+  __builtin_amdgcn_wave_barrier();
+  phase_fence();
+#pragma unroll
+  for (int j = 0; j < 16; ++j) t[j] = T[rd + 4 * j];
+  __builtin_amdgcn_wave_barrier();
+  phase_fence();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) T[k * kRegLd + lane] = v[k].y;
+  __builtin_amdgcn_wave_barrier();
+  phase_fence();
+#pragma unroll
+  for (int j = 0; j < 16; ++j) v[j] = make_double2(t[j], T[rd + 4 * j]);
+  __builtin_amdgcn_wave_barrier();
+  phase_fence();
+}
+
+// the inverse movement of xpose_to_quads
+__device__ __forceinline__ void xpose_from_quads(double2 *v, double *T, int lane) {
+  const int wr = (lane >> 2) * kRegLd + (lane & 3);
+  double t[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) T[wr + 4 * j] = v[j].x;
+  __builtin_amdgcn_wave_barrier();
+  phase_fence();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) t[k] = T[k * kRegLd + lane];
+  __builtin_amdgcn_wave_barrier();
+  phase_fence();
+#pragma unroll
+  for (int j = 0; j < 16; ++j) T[wr + 4 * j] = v[j].y;
+  __builtin_amdgcn_wave_barrier();
+  phase_fence();
+#pragma unroll
+  for (int k = 0; k < 16; ++k) v[k] = make_double2(t[k], T[k * kRegLd + lane]);
+  __builtin_amdgcn_wave_barrier();
+  phase_fence();
+}
+
 // F_DIR: layout N -> layout O.  T: this wavefront's transpose buffer;
 // tw1[k1 * 64 + l] = exp(-2 pi i l k1 / 1024), tw2[c * 4 + a] = exp(-2 pi i a c / 64).
 template <int DIR>
-__device__ __forceinline__ void fft_F(double2 *v, double2 *T, const double2 *tw1, const double2 *tw2,
+__device__ __forceinline__ void fft_F(double2 *v, double *T, const double2 *tw1, const double2 *tw2,
                                       int lane) {
   dft16n<DIR>(v);
 #pragma unroll
-  for (int k1 = 0; k1 < 16; ++k1) T[k1 * kRegLd + lane] = cmul(v[k1], tw_load<DIR>(tw1, k1 * 64 + lane));
-  __builtin_amdgcn_wave_barrier();
-  phase_fence();
-  const int k1 = lane >> 2, a = lane & 3;
-#pragma unroll
-  for (int j2 = 0; j2 < 16; ++j2) v[j2] = T[k1 * kRegLd + 4 * j2 + a];
-  __builtin_amdgcn_wave_barrier();
-  phase_fence();
+  for (int k1 = 0; k1 < 16; ++k1) v[k1] = cmul(v[k1], tw_load<DIR>(tw1, k1 * 64 + lane));
+  xpose_to_quads(v, T, lane);
   dft16n<DIR>(v);
+  const int a = lane & 3;
   const double sA = (a & 2) ? -1.0 : 1.0, sB = (a & 1) ? -1.0 : 1.0;
 #pragma unroll
   for (int c = 0; c < 16; ++c) {
@@ -101,9 +165,9 @@ __device__ __forceinline__ void fft_F(double2 *v, double2 *T, const double2 *tw1
 
 // G_DIR: layout O -> layout N (tables as for fft_F)
 template <int DIR>
-__device__ __forceinline__ void fft_G(double2 *v, double2 *T, const double2 *tw1, const double2 *tw2,
+__device__ __forceinline__ void fft_G(double2 *v, double *T, const double2 *tw1, const double2 *tw2,
                                       int lane) {
-  const int k1 = lane >> 2, a = lane & 3;
+  const int a = lane & 3;
   const double sA = (a & 2) ? -1.0 : 1.0, sB = (a & 1) ? -1.0 : 1.0;
 #pragma unroll
   for (int c = 0; c < 16; ++c) {
@@ -113,40 +177,36 @@ __device__ __forceinline__ void fft_G(double2 *v, double2 *T, const double2 *tw1
     v[c] = cmul(z, tw_load<DIR>(tw2, c * 4 + a));
   }
   dft16n<DIR>(v);
+  xpose_from_quads(v, T, lane);
 #pragma unroll
-  for (int j2 = 0; j2 < 16; ++j2) T[k1 * kRegLd + 4 * j2 + a] = v[j2];
-  __builtin_amdgcn_wave_barrier();
-  phase_fence();
-#pragma unroll
-  for (int r = 0; r < 16; ++r) v[r] = cmul(T[r * kRegLd + lane], tw_load<DIR>(tw1, r * 64 + lane));
-  __builtin_amdgcn_wave_barrier();
+  for (int r = 0; r < 16; ++r) v[r] = cmul(v[r], tw_load<DIR>(tw1, r * 64 + lane));
   phase_fence();
   dft16n<DIR>(v);
 }
 
-// DFT of length d.n <= 1024 (d.len == 1024, Bluestein) with sign DIR,
-// unnormalised, layout N in and out; entries at or beyond d.n must be zero
-// on entry and are undefined on exit.
+// DFT of length n <= 512 (Bluestein with inner length 1024) with sign DIR,
+// unnormalised, layout N in and out; entries at or beyond n must be zero on
+// entry and are undefined on exit.  tab: the LDS tables of reg_stage_blue<DIR>.
 template <int DIR>
-__device__ __forceinline__ void blue_reg(double2 *v, const FftDesc &d, double2 *T, int lane) {
+__device__ __forceinline__ void blue_reg(double2 *v, int n, const double2 *tab, double *T, int lane) {
+  const double2 *tw = tab, *bh = tab + kRegTw, *chirp = tab + kRegTw + 1024;
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int k = lane + 64 * q;
-    if (k < d.n) v[q] = cmul(v[q], tw_load<DIR>(d.chirp, k));
+    if (k < n) v[q] = cmul(v[q], tw_load<DIR>(chirp, k));
   }
   phase_fence();
-  fft_F<-1>(v, T, d.tws, d.tws + 1024, lane);
-  const double2 *bh = DIR < 0 ? d.bhat_f : d.bhat_b;
+  fft_F<-1>(v, T, tw, tw + 1024, lane);
 #pragma unroll
   for (int c = 0; c < 16; ++c) v[c] = cmul(v[c], bh[o_index(lane, c)]);
   phase_fence();
-  fft_G<+1>(v, T, d.tws, d.tws + 1024, lane);
+  fft_G<+1>(v, T, tw, tw + 1024, lane);
   phase_fence();
   constexpr double inv = 1.0 / 1024.0;
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int k = lane + 64 * q;
-    if (k < d.n) v[q] = cmul(v[q], cscale(tw_load<DIR>(d.chirp, k), inv));
+    if (k < n) v[q] = cmul(v[q], cscale(tw_load<DIR>(chirp, k), inv));
   }
 }
 

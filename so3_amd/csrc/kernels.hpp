@@ -36,7 +36,23 @@ struct NBase {
 // Workspace layouts (double2):
 //   X1 rows  : [n_slot][m_slot][m'], m' < L          (inverse contraction out)
 //   Gp       : [|n|][|m|][2*NS sign combos][m']        (forward contraction in)
-//   X2       : [n_slot][b][a or m_slot], rows of ldA  (beta / alpha stages)
+//   X2       : rows of ldA entries (a or m slot), one row per (n slot, b) at
+//              x2_row below                           (beta / alpha / gamma)
+
+// X2 rows come in blocks of kX2Blk beta samples: block h = b / kX2Blk holds,
+// n slot after n slot, its w = min(kX2Blk, nB - h kX2Blk) rows.  A gamma
+// column (b, a) then steps w rows per n slot (64 n slots of one load span a
+// few 2 MB pages rather than 64 of them), a beta sequence (n slot, m) steps
+// one row per b inside a block, and alpha sees nG nB contiguous rows.
+constexpr int kX2Blk = 8;
+__host__ __device__ inline int x2_w(const Geo &g, int b) {
+  const int r = g.nB - (b / kX2Blk) * kX2Blk;
+  return r < kX2Blk ? r : kX2Blk;
+}
+__host__ __device__ inline int64_t x2_row(const Geo &g, int nslot, int b) {
+  const int h = b / kX2Blk;
+  return (int64_t)h * kX2Blk * g.nG + (int64_t)nslot * x2_w(g, b) + (b - h * kX2Blk);
+}
 void launch_contract_inv(const Geo &g, const double2 *flmn, NBase nb, const DlTables &dl, double2 *X1,
                          hipStream_t st);
 void launch_contract_fwd(const Geo &g, const double2 *Gp, NBase nb, const DlTables &dl,

@@ -66,7 +66,7 @@ __global__ __launch_bounds__(kThreads) void k_inv_beta(Geo g, FftDesc fd, int K,
   fft_run<+1>(A, K, ld, fd, tw);
   for (int idx = threadIdx.x; idx < K * g.nB; idx += blockDim.x) {
     const int b = idx / K, s = idx - b * K;
-    if (ms0 + s < g.nA) X2[((int64_t)nslot * g.nB + b) * g.ldA + ms0 + s] = A[s * ld + b];
+    if (ms0 + s < g.nA) X2[x2_row(g, nslot, b) * g.ldA + ms0 + s] = A[s * ld + b];
   }
 }
 
@@ -90,216 +90,233 @@ __global__ __launch_bounds__(kThreads) void k_rows(FftDesc fd, int K, int ld, do
   }
 }
 
-// ---------------------------------------------------------------- rows, register path
+// ---------------------------------------------------------------- register path
 // Transforms whose inner length is 1024 (n = 1024, or a Bluestein n in
-// [513, 1023]): one row per wavefront in the register layouts of
-// fft_reg.hpp.
+// [257, 512]): one sequence per wavefront in the register layouts of
+// fft_reg.hpp.  The kernels are persistent: a block stages the twiddle
+// tables in LDS once and then takes groups of sequences t = xcd_tile(block)
+// + i * gridDim.x until none is left (the grid is what fits on the device at
+// once, reg_grid).
 constexpr int kRegWaves = kThreads / 64;
-// wavefronts per workgroup of the gamma (8: 128-byte HBM runs) and beta (4:
-// two workgroups per CU) kernels, measured best at L = 256
-constexpr int kGammaW = 8;
+// wavefronts per workgroup of the beta kernels, measured best at L = 256
 constexpr int kBetaW = 4;
-constexpr size_t kRegShared = sizeof(double2) * kRegWaves * kRegLds;
+
+// Workgroups go round-robin over the 8 XCDs, each with its own L2: block p
+// runs on XCD p % 8.  Renumbering block p to (p % 8) q + p / 8, q = n / 8,
+// lets the blocks that share an XCD take consecutive groups, so two groups
+// that share 128-byte lines (W = 4 complex doubles is half a line) meet in
+// one L2 instead of each XCD fetching the whole line.  The last n % 8 blocks
+// keep their number.
+__device__ __forceinline__ unsigned xcd_tile(unsigned p, unsigned n) {
+  const unsigned q = n >> 3;
+  return p < 8 * q ? (p & 7) * q + (p >> 3) : p;
+}
+
+// The lane index, recomputed in every pass of a persistent loop: an opaque
+// value keeps the compiler from hoisting the many lane-dependent addresses
+// of a sequence out of the loop, where they would stay live in registers.
+__device__ __forceinline__ int loop_lane() {
+  int l;
+  asm volatile("v_and_b32 %0, 63, %1" : "=v"(l) : "v"((int)threadIdx.x));
+  return l;
+}
+
+// LDS of a register-path block: `tabs` double2 of tables, then W transpose
+// buffers (which the beta kernels also use as their tile of `rows` x (W + 1)
+// entries)
+template <int W>
+constexpr int tile_ld() { return W + 1; }
+template <int W>
+size_t reg_shared(int tabs, int rows) {
+  return sizeof(double2) * tabs +
+         std::max<size_t>(sizeof(double2) * (size_t)rows * tile_ld<W>(), sizeof(double) * (size_t)W * kRegT);
+}
 
 template <int DIR, int BLUE>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_rows_reg(
     FftDesc fd, double2 *__restrict__ X, int64_t nrows, int n, int64_t rs, double scale) {
   extern __shared__ double2 lds[];
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t r = (int64_t)blockIdx.x * kRegWaves + wv;
-  if (r >= nrows) return;  // the whole wavefront
-  double2 *T = lds + wv * kRegLds;
-  double2 *row = X + r * rs;
-  double2 v[16];
-#pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int k = lane + 64 * q;
-    v[q] = k < n ? row[k] : make_double2(0.0, 0.0);
-  }
-  if constexpr (BLUE) {
-    blue_reg<DIR>(v, fd, T, lane);
+  const int wv = threadIdx.x >> 6;
+  constexpr int kTabs = BLUE ? kRegBlueTab : kRegTw;
+  double2 *tw = lds;
+  double *T = (double *)(lds + kTabs) + wv * kRegT;
+  if constexpr (BLUE) reg_stage_blue<DIR>(tw, fd);
+  else reg_stage_tw(tw, fd);
+  __syncthreads();
+  const int64_t groups = (nrows + kRegWaves - 1) / kRegWaves;
+  for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
+    const int lane = loop_lane();
+    const int64_t r = t * kRegWaves + wv;
+    if (r >= nrows) break;  // the whole wavefront, last group
+    double2 *row = X + r * rs;
+    double2 v[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int k = lane + 64 * q;
-      if (k < n) row[k] = cscale(v[q], scale);
+      v[q] = k < n ? row[k] : make_double2(0.0, 0.0);
     }
-  } else {
-    fft_F<DIR>(v, T, fd.tws, fd.tws + 1024, lane);
+    if constexpr (BLUE) {
+      blue_reg<DIR>(v, n, tw, T, lane);
 #pragma unroll
-    for (int c = 0; c < 16; ++c) row[o_index(lane, c)] = cscale(v[c], scale);
+      for (int q = 0; q < 16; ++q) {
+        const int k = lane + 64 * q;
+        if (k < n) row[k] = cscale(v[q], scale);
+      }
+    } else {
+      fft_F<DIR>(v, T, tw, tw + 1024, lane);
+#pragma unroll
+      for (int c = 0; c < 16; ++c) row[o_index(lane, c)] = cscale(v[c], scale);
+    }
   }
 }
 
-// ---------------------------------------------------------------- columns, register path
-// Gamma stages and the inverse beta stage with inner length 1024: a
-// workgroup of 8 wavefronts owns 8 sequences whose elements are adjacent in
-// HBM across the sequences (8 consecutive columns p of the gamma stages, 8
-// consecutive m of the beta output), moved through an LDS tile with rows of
-// 9 entries so that HBM sees 128-byte runs and LDS no bank conflicts.  The
-// tile shares LDS with the transpose buffers (barriers separate the uses).
-// W: wavefronts (sequences) per workgroup; the tile rows hold W + 1 entries
-template <int W>
-constexpr int tile_ld() { return W + 1; }
-// LDS bytes of a workgroup whose tile has `rows` rows
-template <int W>
-size_t col_shared(int rows) {
-  return sizeof(double2) * std::max<size_t>((size_t)rows * tile_ld<W>(), (size_t)W * kRegLds);
-}
-
-template <int W, int BLUE>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_inv_gamma_reg(
+// Gamma stages, Bluestein length 2N-1 with inner length 1024: one column
+// (b, a) per wavefront, moved straight between HBM and the register layout
+// (lane l, slot q holds n slot l + 64 q), kRegWaves consecutive columns per
+// group.  Each access instruction touches 64 lines; the columns of the
+// neighbouring groups on the same XCD use the rest of each line from L2.  On
+// the X2 side the 64 lines lie within a few pages (x2_row); on the f side
+// they are nA nB entries apart.  No barrier separates the load, DFT and store
+// phases, so the blocks of a CU overlap one's memory traffic with another's
+// DFT.
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_inv_gamma_col(
     Geo g, FftDesc fd, const double2 *__restrict__ X2, double2 *__restrict__ fc, double *__restrict__ fr,
     double scale) {
   extern __shared__ double2 lds[];
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  // columns p2 of X2 (rows of ldA: 128-byte aligned tiles) map to columns
-  // b nA + a of f
-  const int64_t S = (int64_t)g.nB * g.nA, S2 = (int64_t)g.nB * g.ldA;
-  const int64_t p0 = (int64_t)blockIdx.x * W;
+  const int wv = threadIdx.x >> 6;
+  double2 *tab = lds;
+  double *T = (double *)(lds + kRegBlueTab) + wv * kRegT;
+  reg_stage_blue<+1>(tab, fd);
+  __syncthreads();
+  const int64_t S = (int64_t)g.nB * g.nA;
+  const int64_t groups = (S + kRegWaves - 1) / kRegWaves;
   const int nG = 2 * g.N - 1;
-  for (int idx = threadIdx.x; idx < nG * W; idx += (64 * W)) {
-    const int k = idx / W, c = idx - k * W;
-    const int64_t p = p0 + c;
-    double2 v = make_double2(0.0, 0.0);
-    if (p < S2 && p % g.ldA < g.nA) {
-      if (!g.real) v = X2[k * S2 + p];
-      else if (k == 0) v = make_double2(X2[p].x, 0.0);
-      else if (k < g.N) v = X2[k * S2 + p];
-      else v = cconj(X2[(nG - k) * S2 + p]);
-    }
-    lds[k * tile_ld<W>() + c] = v;
-  }
-  __syncthreads();
-  double2 v[16];
-#pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int k = lane + 64 * q;
-    v[q] = k < nG ? lds[k * tile_ld<W>() + wv] : make_double2(0.0, 0.0);
-  }
-  __syncthreads();
-  double2 *T = lds + wv * kRegLds;
-  if constexpr (BLUE) {
-    blue_reg<+1>(v, fd, T, lane);
-    __syncthreads();
+  for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
+    const int lane = loop_lane();
+    const int64_t p = t * kRegWaves + wv;
+    if (p >= S) break;  // the whole wavefront, last group
+    const int b = (int)(p / g.nA), a = (int)(p % g.nA);
+    // column (b, a) of X2: n slot k at col[k * cs]
+    const double2 *col = X2 + x2_row(g, 0, b) * g.ldA + a;
+    const int64_t cs = (int64_t)x2_w(g, b) * g.ldA;
+    double2 v[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int k = lane + 64 * q;
-      if (k < nG) lds[k * tile_ld<W>() + wv] = v[q];
+      double2 x = make_double2(0.0, 0.0);
+      if (k < nG) {
+        if (!g.real) x = col[k * cs];
+        else if (k == 0) x = make_double2(col[0].x, 0.0);
+        else if (k < g.N) x = col[k * cs];
+        else x = cconj(col[(nG - k) * cs]);
+      }
+      v[q] = x;
     }
-  } else {
-    fft_F<+1>(v, T, fd.tws, fd.tws + 1024, lane);
-    __syncthreads();
+    blue_reg<+1>(v, nG, tab, T, lane);
 #pragma unroll
-    for (int c = 0; c < 16; ++c) lds[o_index(lane, c) * tile_ld<W>() + wv] = v[c];
-  }
-  __syncthreads();
-  for (int idx = threadIdx.x; idx < nG * W; idx += (64 * W)) {
-    const int k = idx / W, c = idx - k * W;
-    const int64_t p2 = p0 + c;
-    const int a = (int)(p2 % g.ldA);
-    if (p2 >= S2 || a >= g.nA) continue;
-    const int64_t p = (p2 / g.ldA) * g.nA + a;
-    const double2 v2 = lds[k * tile_ld<W>() + c];
-    if (g.real) fr[k * S + p] = v2.x * scale;
-    else fc[k * S + p] = cscale(v2, scale);
+    for (int q = 0; q < 16; ++q) {
+      const int k = lane + 64 * q;
+      if (k < nG) {
+        if (g.real) fr[k * S + p] = v[q].x * scale;
+        else fc[k * S + p] = cscale(v[q], scale);
+      }
+    }
   }
 }
 
-template <int W, int BLUE>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_fwd_gamma_reg(
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_fwd_gamma_col(
     Geo g, FftDesc fd, const double2 *__restrict__ fc, const double *__restrict__ fr,
     double2 *__restrict__ X2, double sc) {
   extern __shared__ double2 lds[];
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  // columns p of f (aligned tiles of the flat f layout) map to columns
-  // (p / nA) ldA + p % nA of X2
-  const int64_t S = (int64_t)g.nB * g.nA, S2 = (int64_t)g.nB * g.ldA;
-  const int64_t p0 = (int64_t)blockIdx.x * W;
+  const int wv = threadIdx.x >> 6;
+  double2 *tab = lds;
+  double *T = (double *)(lds + kRegBlueTab) + wv * kRegT;
+  reg_stage_blue<-1>(tab, fd);
+  __syncthreads();
+  const int64_t S = (int64_t)g.nB * g.nA;
+  const int64_t groups = (S + kRegWaves - 1) / kRegWaves;
   const int nG = 2 * g.N - 1;
-  for (int idx = threadIdx.x; idx < nG * W; idx += (64 * W)) {
-    const int k = idx / W, c = idx - k * W;
-    const int64_t p = p0 + c;
-    double2 v = make_double2(0.0, 0.0);
-    if (p < S) v = g.real ? make_double2(fr[k * S + p], 0.0) : fc[k * S + p];
-    lds[k * tile_ld<W>() + c] = v;
-  }
-  __syncthreads();
-  double2 v[16];
-#pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int k = lane + 64 * q;
-    v[q] = k < nG ? lds[k * tile_ld<W>() + wv] : make_double2(0.0, 0.0);
-  }
-  __syncthreads();
-  double2 *T = lds + wv * kRegLds;
-  if constexpr (BLUE) {
-    blue_reg<-1>(v, fd, T, lane);
-    __syncthreads();
+  const int nout = g.real ? g.N : nG;
+  for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
+    const int lane = loop_lane();
+    const int64_t p = t * kRegWaves + wv;
+    if (p >= S) break;  // the whole wavefront, last group
+    const int b = (int)(p / g.nA), a = (int)(p % g.nA);
+    double2 v[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int k = lane + 64 * q;
-      if (k < nG) lds[k * tile_ld<W>() + wv] = v[q];
+      double2 x = make_double2(0.0, 0.0);
+      if (k < nG) x = g.real ? make_double2(fr[k * S + p], 0.0) : fc[k * S + p];
+      v[q] = x;
     }
-  } else {
-    fft_F<-1>(v, T, fd.tws, fd.tws + 1024, lane);
-    __syncthreads();
+    blue_reg<-1>(v, nG, tab, T, lane);
+    double2 *col = X2 + x2_row(g, 0, b) * g.ldA + a;
+    const int64_t cs = (int64_t)x2_w(g, b) * g.ldA;
 #pragma unroll
-    for (int c = 0; c < 16; ++c) lds[o_index(lane, c) * tile_ld<W>() + wv] = v[c];
-  }
-  __syncthreads();
-  const int nout = g.real ? g.N : nG;
-  for (int idx = threadIdx.x; idx < nout * W; idx += (64 * W)) {
-    const int k = idx / W, c = idx - k * W;
-    const int64_t p = p0 + c;
-    if (p < S) X2[k * S2 + (p / g.nA) * g.ldA + p % g.nA] = cscale(lds[k * tile_ld<W>() + c], sc);
+    for (int q = 0; q < 16; ++q) {
+      const int k = lane + 64 * q;
+      if (k < nout) col[k * cs] = cscale(v[q], sc);
+    }
   }
 }
 
-// inverse beta: sequence (n slot, m slot) per wavefront, 8 consecutive m
-// slots per workgroup; Pb is a Bluestein length with inner length 1024
+// inverse beta: sequence (n slot, m slot) per wavefront, W consecutive m
+// slots of one n slot per group; the b < nB outputs leave through the LDS
+// tile so that HBM sees runs of W entries.  Pb is a Bluestein length with
+// inner length 1024.
 template <int W>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_inv_beta_reg(
     Geo g, FftDesc fd, const double2 *__restrict__ X1, double2 *__restrict__ X2,
     const int64_t *__restrict__ nbase, const double2 *__restrict__ phase) {
   extern __shared__ double2 lds[];
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  double2 *tab = lds, *tile = lds + kRegBlueTab;
+  double *T = (double *)(lds + kRegBlueTab) + wv * kRegT;
+  reg_stage_blue<+1>(tab, fd);
+  __syncthreads();
   const int chunks = (g.nA + W - 1) / W;
-  const int nslot = blockIdx.x / chunks;
-  const int ms0 = (blockIdx.x % chunks) * W;
-  const int n = n_of_slot(g, nslot);
-  const bool active = nbase[n + g.N - 1] != kNoBlock;
+  const int64_t groups = (int64_t)g.nG * chunks;
   const int Pb = g.Pb, L = g.L;
-  const int mslot = ms0 + wv;
-  const bool mok = active && mslot < g.nA;
-  const int m = m_of_slot(g, mslot);
-  const double2 *row = X1 + ((int64_t)nslot * g.nA + mslot) * L;
-  const bool flip = ((m + n) & 1) != 0;
-  double2 v[16];
+  for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
+    const int lane = loop_lane();
+    const int nslot = (int)(t / chunks);
+    const int ms0 = (int)(t % chunks) * W;
+    const int n = n_of_slot(g, nslot);
+    const bool active = nbase[n + g.N - 1] != kNoBlock;
+    const int mslot = ms0 + wv;
+    const bool mok = active && mslot < g.nA;
+    const int m = m_of_slot(g, mslot);
+    const double2 *row = X1 + ((int64_t)nslot * g.nA + mslot) * L;
+    const bool flip = ((m + n) & 1) != 0;
+    double2 v[16];
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int k = lane + 64 * q;
-    double2 x = make_double2(0.0, 0.0);
-    if (mok && k < Pb && m >= -(L - 1)) {
-      const int mp = k <= L - 1 ? k : k - Pb;
-      if (mp >= -(L - 1)) {
-        x = row[mp < 0 ? -mp : mp];
-        if (mp < 0 && flip) x = make_double2(-x.x, -x.y);
-        x = cmul(x, phase[k]);
+    for (int q = 0; q < 16; ++q) {
+      const int k = lane + 64 * q;
+      double2 x = make_double2(0.0, 0.0);
+      if (mok && k < Pb && m >= -(L - 1)) {
+        const int mp = k <= L - 1 ? k : k - Pb;
+        if (mp >= -(L - 1)) {
+          x = row[mp < 0 ? -mp : mp];
+          if (mp < 0 && flip) x = make_double2(-x.x, -x.y);
+          x = cmul(x, phase[k]);
+        }
       }
+      v[q] = x;
     }
-    v[q] = x;
-  }
-  blue_reg<+1>(v, fd, lds + wv * kRegLds, lane);
-  __syncthreads();
+    blue_reg<+1>(v, Pb, tab, T, lane);
+    __syncthreads();  // every wavefront is done with its transpose buffer
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int b = lane + 64 * q;
-    if (b < g.nB) lds[b * tile_ld<W>() + wv] = v[q];
-  }
-  __syncthreads();
-  for (int idx = threadIdx.x; idx < g.nB * W; idx += (64 * W)) {
-    const int b = idx / W, c = idx - b * W;
-    if (ms0 + c < g.nA) X2[((int64_t)nslot * g.nB + b) * g.ldA + ms0 + c] = lds[b * tile_ld<W>() + c];
+    for (int q = 0; q < 16; ++q) {
+      const int b = lane + 64 * q;
+      if (b < g.nB) tile[b * tile_ld<W>() + wv] = v[q];
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < g.nB * W; idx += (64 * W)) {
+      const int b = idx / W, c = idx - b * W;
+      if (ms0 + c < g.nA) X2[x2_row(g, nslot, b) * g.ldA + ms0 + c] = tile[b * tile_ld<W>() + c];
+    }
+    __syncthreads();  // the tile is read out before the next group's transposes
   }
 }
 
@@ -311,23 +328,25 @@ __global__ __launch_bounds__(kThreads) void k_inv_gamma(Geo g, FftDesc fd, int K
   extern __shared__ double2 lds[];
   double2 *A = lds, *tw = lds + K * ld;
   fft_load_twiddles(tw, fd);
-  // columns p2 of X2 (rows of ldA) map to columns b nA + a of f
-  const int64_t S = (int64_t)g.nB * g.nA, S2 = (int64_t)g.nB * g.ldA;
+  // columns p = b nA + a of f; column (b, a) of X2 holds n slot k at
+  // x2_row(k, b) ldA + a
+  const int64_t S = (int64_t)g.nB * g.nA;
   const int64_t p0 = (int64_t)blockIdx.x * K;
   const int nG = 2 * g.N - 1;
   for (int idx = threadIdx.x; idx < K * nG; idx += blockDim.x) {
     const int k = idx / K, s = idx - k * K;
     const int64_t p = p0 + s;
     double2 v = make_double2(0.0, 0.0);
-    if (p < S2 && p % g.ldA < g.nA) {
+    if (p < S) {
+      const int b = (int)(p / g.nA), a = (int)(p % g.nA);
       if (!g.real) {
-        v = X2[k * S2 + p];
+        v = X2[x2_row(g, k, b) * g.ldA + a];
       } else if (k == 0) {
-        v = make_double2(X2[p].x, 0.0);
+        v = make_double2(X2[x2_row(g, 0, b) * g.ldA + a].x, 0.0);
       } else if (k < g.N) {
-        v = X2[k * S2 + p];
+        v = X2[x2_row(g, k, b) * g.ldA + a];
       } else {
-        v = cconj(X2[(nG - k) * S2 + p]);
+        v = cconj(X2[x2_row(g, nG - k, b) * g.ldA + a]);
       }
     }
     A[s * ld + k] = v;
@@ -336,10 +355,8 @@ __global__ __launch_bounds__(kThreads) void k_inv_gamma(Geo g, FftDesc fd, int K
   fft_run<+1>(A, K, ld, fd, tw);
   for (int idx = threadIdx.x; idx < K * nG; idx += blockDim.x) {
     const int k = idx / K, s = idx - k * K;
-    const int64_t p2 = p0 + s;
-    const int a = (int)(p2 % g.ldA);
-    if (p2 >= S2 || a >= g.nA) continue;
-    const int64_t p = (p2 / g.ldA) * g.nA + a;
+    const int64_t p = p0 + s;
+    if (p >= S) continue;
     if (g.real)
       fr[k * S + p] = A[s * ld + k].x * scale;
     else
@@ -355,8 +372,8 @@ __global__ __launch_bounds__(kThreads) void k_fwd_gamma(Geo g, FftDesc fd, int K
   extern __shared__ double2 lds[];
   double2 *A = lds, *tw = lds + K * ld;
   fft_load_twiddles(tw, fd);
-  // columns p of f map to columns (p / nA) ldA + p % nA of X2
-  const int64_t S = (int64_t)g.nB * g.nA, S2 = (int64_t)g.nB * g.ldA;
+  // columns p = b nA + a of f map to columns (b, a) of X2
+  const int64_t S = (int64_t)g.nB * g.nA;
   const int64_t p0 = (int64_t)blockIdx.x * K;
   const int nG = 2 * g.N - 1;
   for (int idx = threadIdx.x; idx < K * nG; idx += blockDim.x) {
@@ -372,7 +389,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_gamma(Geo g, FftDesc fd, int K
   for (int idx = threadIdx.x; idx < K * nout; idx += blockDim.x) {
     const int k = idx / K, s = idx - k * K;
     const int64_t p = p0 + s;
-    if (p < S) X2[k * S2 + (p / g.nA) * g.ldA + p % g.nA] = cscale(A[s * ld + k], sc);
+    if (p < S) X2[x2_row(g, k, (int)(p / g.nA)) * g.ldA + p % g.nA] = cscale(A[s * ld + k], sc);
   }
 }
 
@@ -408,7 +425,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_beta(Geo g, FftDesc fb, FftDes
       } else {
         if (b > L) { b = 2 * L - b; neg = ((m + n) & 1); zero = adjoint; }
       }
-      v = zero ? make_double2(0.0, 0.0) : X2[((int64_t)nslot * g.nB + b) * g.ldA + mslot];
+      v = zero ? make_double2(0.0, 0.0) : X2[x2_row(g, nslot, b) * g.ldA + mslot];
       if (neg) v = make_double2(-v.x, -v.y);
     }
     A[s * ld + k] = v;
@@ -484,8 +501,9 @@ __global__ __launch_bounds__(kThreads) void k_fwd_beta(Geo g, FftDesc fb, FftDes
 }
 
 // forward beta, register path (Pb Bluestein with inner length 1024,
-// Pw = 1024; MW and MW_SS): sequence (n slot, m slot) per wavefront, W
-// consecutive m slots per workgroup
+// Pw = 1024, so L <= 256; MW and MW_SS): sequence (n slot, m slot) per
+// wavefront, W consecutive m slots of one n slot per group, the beta samples
+// coming in through the LDS tile.
 template <int W>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_fwd_beta_reg(
     Geo g, FftDesc fb, FftDesc fw, const double2 *__restrict__ X2, double2 *__restrict__ Gp,
@@ -493,87 +511,133 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) 
     int adjoint) {
   extern __shared__ double2 lds[];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // one twiddle table serves the Bluestein and the convolution transforms
+  // (both of length 1024)
+  double2 *tw = lds, *tile = lds + kRegBlueTab;
+  double *T = (double *)(lds + kRegBlueTab) + wv * kRegT;
+  reg_stage_blue<-1>(tw, fb);
+  __syncthreads();
   const int chunks = (g.nA + W - 1) / W;
-  const int nslot = blockIdx.x / chunks;
-  const int ms0 = (blockIdx.x % chunks) * W;
-  const int n = n_of_slot(g, nslot);
-  if (nbase[n + g.N - 1] == kNoBlock) return;  // block not transformed (whole workgroup)
+  const int64_t groups = (int64_t)g.nG * chunks;
   const int Pb = g.Pb, L = g.L;
-  // beta samples of the 8 sequences: tile[b][s]
-  for (int idx = threadIdx.x; idx < g.nB * W; idx += (64 * W)) {
-    const int b = idx / W, c = idx - b * W;
-    lds[b * tile_ld<W>() + c] =
-        ms0 + c < g.nA ? X2[((int64_t)nslot * g.nB + b) * g.ldA + ms0 + c] : make_double2(0.0, 0.0);
-  }
-  __syncthreads();
-  const int mslot = ms0 + wv;
-  const int m = m_of_slot(g, mslot);
-  const bool flip = ((m + n) & 1) != 0;
-  double2 v[16];
-  // extension F(2L-2-b) = (-1)^{m+n} F(b) (zero for the adjoint)
-#pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int k = lane + 64 * q;
-    double2 x = make_double2(0.0, 0.0);
-    if (k < Pb) {
-      // MW: F(2L-2-b) for b >= L; MW_SS: F(2L-b) for b > L
-      const int kr = g.ss ? 2 * L - k : 2 * L - 2 - k;
-      if (g.ss ? k <= L : k < L) {
-        x = lds[k * tile_ld<W>() + wv];
-      } else if (!adjoint) {
-        x = lds[kr * tile_ld<W>() + wv];
-        if (flip) x = make_double2(-x.x, -x.y);
-      }
-    }
-    v[q] = x;
-  }
-  __syncthreads();
-  double2 *T = lds + wv * kRegLds;
-  blue_reg<-1>(v, fb, T, lane);
-  // F~(m'') times the phase (which carries 1/Pb) at m'' mod Pw, |m''| < L
-#pragma unroll
-  for (int q = 0; q < 16; ++q) T[lane + 64 * q] = v[q];
-  __builtin_amdgcn_wave_barrier();
-#pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int r = lane + 64 * q;
-    const int mpp = r <= L - 1 ? r : r - 1024;
-    double2 x = make_double2(0.0, 0.0);
-    if (mpp >= -(L - 1)) x = cmul(T[mpp < 0 ? mpp + Pb : mpp], phase[mpp + L - 1]);
-    v[q] = x;
-  }
-  __builtin_amdgcn_wave_barrier();
-  double sc = 1.0;
-  if (!adjoint) {
-    // quadrature-weight convolution as a product in the conjugate domain
-    fft_F<+1>(v, T, fw.tws, fw.tws + 1024, lane);
-#pragma unroll
-    for (int c = 0; c < 16; ++c) v[c] = cmul(v[c], wr[o_index(lane, c)]);
-    fft_G<-1>(v, T, fw.tws, fw.tws + 1024, lane);
-    sc = kTwoPi / 1024.0;
-  }
-  // fold to m' >= 0: G'(m') = G(m') + (-1)^{m+n} G(-m')
-#pragma unroll
-  for (int q = 0; q < 16; ++q) T[lane + 64 * q] = v[q];
-  __builtin_amdgcn_wave_barrier();
-  if (mslot >= g.nA || m < -(L - 1)) return;  // MW_SS Nyquist column / tail (whole wavefront)
   const int C = 2 * g.NS;
-  const int an = n < 0 ? -n : n, am = m < 0 ? -m : m;
-  const int sm = m < 0, sn = n < 0;
-  double2 *dst = Gp + ((int64_t)an * L + am) * C * L;
-  for (int mp = lane; mp < L; mp += 64) {
-    double2 G0 = cscale(T[mp], sc);
-    if (mp > 0) {
-      const double2 Gn = cscale(T[1024 - mp], sc);
-      G0 = flip ? csub(G0, Gn) : cadd(G0, Gn);
+  for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
+    const int lane = loop_lane();
+    const int nslot = (int)(t / chunks);
+    const int ms0 = (int)(t % chunks) * W;
+    const int n = n_of_slot(g, nslot);
+    if (nbase[n + g.N - 1] == kNoBlock) continue;  // block not transformed (whole workgroup)
+    // beta samples of the W sequences: tile[b][s]
+    for (int idx = threadIdx.x; idx < g.nB * W; idx += (64 * W)) {
+      const int b = idx / W, c = idx - b * W;
+      tile[b * tile_ld<W>() + c] =
+          ms0 + c < g.nA ? X2[x2_row(g, nslot, b) * g.ldA + ms0 + c] : make_double2(0.0, 0.0);
     }
-    for (int cm = 0; cm < 2; ++cm) {
-      if (cm != sm && am != 0) continue;
-      for (int cn = 0; cn < g.NS; ++cn) {
-        if (cn != sn && an != 0) continue;
-        dst[(int64_t)(cm + 2 * cn) * L + mp] = ((cm != cn) && (mp & 1)) ? make_double2(-G0.x, -G0.y) : G0;
+    __syncthreads();
+    const int mslot = ms0 + wv;
+    const int m = m_of_slot(g, mslot);
+    const bool flip = ((m + n) & 1) != 0;
+    double2 v[16];
+    // extension F(2L-2-b) = (-1)^{m+n} F(b) (zero for the adjoint)
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int k = lane + 64 * q;
+      double2 x = make_double2(0.0, 0.0);
+      if (k < Pb) {
+        // MW: F(2L-2-b) for b >= L; MW_SS: F(2L-b) for b > L
+        const int kr = g.ss ? 2 * L - k : 2 * L - 2 - k;
+        if (g.ss ? k <= L : k < L) {
+          x = tile[k * tile_ld<W>() + wv];
+        } else if (!adjoint) {
+          x = tile[kr * tile_ld<W>() + wv];
+          if (flip) x = make_double2(-x.x, -x.y);
+        }
+      }
+      v[q] = x;
+    }
+    __syncthreads();  // the tile is read before the transposes reuse it
+    blue_reg<-1>(v, Pb, tw, T, lane);
+    // F~(m'') times the phase (which carries 1/Pb) at m'' mod Pw, |m''| < L:
+    // a gather through the transpose buffer, real parts then imaginary parts
+    {
+      double t[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) T[lane + 64 * q] = v[q].x;
+      __builtin_amdgcn_wave_barrier();
+      phase_fence();
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int r = lane + 64 * q;
+        const int mpp = r <= L - 1 ? r : r - 1024;
+        t[q] = mpp >= -(L - 1) ? T[mpp < 0 ? mpp + Pb : mpp] : 0.0;
+      }
+      __builtin_amdgcn_wave_barrier();
+      phase_fence();
+#pragma unroll
+      for (int q = 0; q < 16; ++q) T[lane + 64 * q] = v[q].y;
+      __builtin_amdgcn_wave_barrier();
+      phase_fence();
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int r = lane + 64 * q;
+        const int mpp = r <= L - 1 ? r : r - 1024;
+        double2 x = make_double2(0.0, 0.0);
+        if (mpp >= -(L - 1)) x = cmul(make_double2(t[q], T[mpp < 0 ? mpp + Pb : mpp]), phase[mpp + L - 1]);
+        v[q] = x;
+      }
+      __builtin_amdgcn_wave_barrier();
+      phase_fence();
+    }
+    double sc = 1.0;
+    if (!adjoint) {
+      // quadrature-weight convolution as a product in the conjugate domain
+      fft_F<+1>(v, T, tw, tw + 1024, lane);
+#pragma unroll
+      for (int c = 0; c < 16; ++c) v[c] = cmul(v[c], wr[o_index(lane, c)]);
+      fft_G<-1>(v, T, tw, tw + 1024, lane);
+      sc = kTwoPi / 1024.0;
+    }
+    // fold to m' >= 0: G'(m') = G(m') + (-1)^{m+n} G(-m'), m' = lane + 64 j
+    constexpr int kFold = 4;  // L <= 256
+    double g0r[kFold], gnr[kFold];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) T[lane + 64 * q] = v[q].x;
+    __builtin_amdgcn_wave_barrier();
+    phase_fence();
+#pragma unroll
+    for (int j = 0; j < kFold; ++j) {
+      const int mp = lane + 64 * j;
+      g0r[j] = T[mp];
+      gnr[j] = T[(1024 - mp) & 1023];
+    }
+    __builtin_amdgcn_wave_barrier();
+    phase_fence();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) T[lane + 64 * q] = v[q].y;
+    __builtin_amdgcn_wave_barrier();
+    phase_fence();
+    const bool store = mslot < g.nA && m >= -(L - 1);  // not the MW_SS Nyquist column / tail
+    const int an = n < 0 ? -n : n, am = m < 0 ? -m : m;
+    const int sm = m < 0, sn = n < 0;
+    double2 *dst = Gp + ((int64_t)an * L + am) * C * L;
+#pragma unroll
+    for (int j = 0; j < kFold; ++j) {
+      const int mp = lane + 64 * j;
+      double2 G0 = make_double2(g0r[j] * sc, T[mp] * sc);
+      if (mp > 0) {
+        const double2 Gn = make_double2(gnr[j] * sc, T[1024 - mp] * sc);
+        G0 = flip ? csub(G0, Gn) : cadd(G0, Gn);
+      }
+      if (!store || mp >= L) continue;
+      for (int cm = 0; cm < 2; ++cm) {
+        if (cm != sm && am != 0) continue;
+        for (int cn = 0; cn < g.NS; ++cn) {
+          if (cn != sn && an != 0) continue;
+          dst[(int64_t)(cm + 2 * cn) * L + mp] = ((cm != cn) && (mp & 1)) ? make_double2(-G0.x, -G0.y) : G0;
+        }
       }
     }
+    __syncthreads();  // transposes done before the next group's tile
   }
 }
 
@@ -656,7 +720,7 @@ __global__ __launch_bounds__(kThreads) void k_adjf_beta(Geo g, FftDesc fb, FftDe
       const double2 e = A[s * ld + (2 * L - 2 - b)];
       v = ((m + n) & 1) ? csub(v, e) : cadd(v, e);
     }
-    X2[((int64_t)nslot * g.nB + b) * g.ldA + mslot] = v;
+    X2[x2_row(g, nslot, b) * g.ldA + mslot] = v;
   }
 }
 
@@ -731,15 +795,30 @@ static void set_lds(Kern k, size_t bytes) {
   if (bytes > 65536) SO3_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
+// grid of a persistent register-path kernel: the blocks that fit on the
+// device at once, a multiple of 8 (xcd_tile), at most `groups`
+template <typename Kern>
+static unsigned reg_grid(Kern k, int threads, size_t lds, int64_t groups) {
+  set_lds(k, lds);
+  int dev = 0, cus = 0, per = 0;
+  SO3_HIP(hipGetDevice(&dev));
+  SO3_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  SO3_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)k, threads, lds));
+  int64_t n = (int64_t)std::max(per, 1) * std::max(cus, 1);
+  if (n > groups) n = groups;
+  if (n >= 8) n -= n % 8;
+  return (unsigned)std::max<int64_t>(n, 1);
+}
+
 void launch_inv_beta(const Geo &g, const double2 *X1, double2 *X2, NBase nb, const double2 *phase,
                      hipStream_t st) {
   const FftDesc fd = fft_plan(g.Pb);
   if (fd.len == 1024 && fd.blue && g.nB <= 1024) {
     constexpr int W = kBetaW;
-    const size_t sh = col_shared<W>(g.nB);
-    set_lds(k_inv_beta_reg<W>, sh);
+    const size_t sh = reg_shared<W>(kRegBlueTab, g.nB);
     const int chunks = (g.nA + W - 1) / W;
-    k_inv_beta_reg<W><<<g.nG * chunks, 64 * W, sh, st>>>(g, fd, X1, X2, nb.base, phase);
+    const unsigned grid = reg_grid(k_inv_beta_reg<W>, 64 * W, sh, (int64_t)g.nG * chunks);
+    k_inv_beta_reg<W><<<grid, 64 * W, sh, st>>>(g, fd, X1, X2, nb.base, phase);
     SO3_LAUNCH_CHECK();
     return;
   }
@@ -757,10 +836,10 @@ void launch_rows(double2 *X, int64_t nrows, int n, int ldr, int dir, double scal
   const FftDesc fd = fft_plan(n);
   if (nrows <= 0) return;
   if (fd.len == 1024) {
-    const unsigned grid = (unsigned)((nrows + kRegWaves - 1) / kRegWaves);
+    const size_t sh = reg_shared<kRegWaves>(fd.blue ? kRegBlueTab : kRegTw, 0);
     auto go = [&](auto kern) {
-      set_lds(kern, kRegShared);
-      kern<<<grid, kThreads, kRegShared, st>>>(fd, X, nrows, n, (int64_t)ldr, scale);
+      const unsigned grid = reg_grid(kern, kThreads, sh, (nrows + kRegWaves - 1) / kRegWaves);
+      kern<<<grid, kThreads, sh, st>>>(fd, X, nrows, n, (int64_t)ldr, scale);
     };
     if (fd.blue)
       dir > 0 ? go(k_rows_reg<+1, 1>) : go(k_rows_reg<-1, 1>);
@@ -790,17 +869,12 @@ void launch_alpha(const Geo &g, double2 *X2, int dir, double scale, hipStream_t 
 
 void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, double scale, hipStream_t st) {
   const FftDesc fd = fft_plan(2 * g.N - 1);
-  if (fd.len == 1024) {
-    constexpr int W = kGammaW;
-    const int64_t S2 = (int64_t)g.nB * g.ldA;
-    const unsigned grid = (unsigned)((S2 + W - 1) / W);
-    const size_t sh = col_shared<W>(fd.blue ? fd.n : 1024);
-    auto go = [&](auto kern) {
-      set_lds(kern, sh);
-      kern<<<grid, 64 * W, sh, st>>>(g, fd, X2, g.real ? nullptr : (double2 *)f,
-                                                   g.real ? (double *)f : nullptr, scale);
-    };
-    fd.blue ? go(k_inv_gamma_reg<W, 1>) : go(k_inv_gamma_reg<W, 0>);
+  if (fd.len == 1024 && fd.blue) {
+    const int64_t S = (int64_t)g.nB * g.nA;
+    const size_t sh = reg_shared<kRegWaves>(kRegBlueTab, 0);
+    const unsigned grid = reg_grid(k_inv_gamma_col, kThreads, sh, (S + kRegWaves - 1) / kRegWaves);
+    k_inv_gamma_col<<<grid, kThreads, sh, st>>>(
+        g, fd, X2, g.real ? nullptr : (double2 *)f, g.real ? (double *)f : nullptr, scale);
     SO3_LAUNCH_CHECK();
     return;
   }
@@ -809,25 +883,20 @@ void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, double scale, hi
   const int K = choose_k(ld, fft_capacity(fd), fd.len, tb, 16);
   const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
   set_lds(k_inv_gamma, sh);
-  const int64_t S2 = (int64_t)g.nB * g.ldA;
-  k_inv_gamma<<<(unsigned)((S2 + K - 1) / K), kThreads, sh, st>>>(
+  const int64_t S = (int64_t)g.nB * g.nA;
+  k_inv_gamma<<<(unsigned)((S + K - 1) / K), kThreads, sh, st>>>(
       g, fd, K, ld, X2, g.real ? nullptr : (double2 *)f, g.real ? (double *)f : nullptr, scale);
   SO3_LAUNCH_CHECK();
 }
 
 void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, double scale, hipStream_t st) {
   const FftDesc fd = fft_plan(2 * g.N - 1);
-  if (fd.len == 1024) {
-    constexpr int W = kGammaW;
+  if (fd.len == 1024 && fd.blue) {
     const int64_t S = (int64_t)g.nB * g.nA;
-    const unsigned grid = (unsigned)((S + W - 1) / W);
-    const size_t sh = col_shared<W>(fd.blue ? fd.n : 1024);
-    auto go = [&](auto kern) {
-      set_lds(kern, sh);
-      kern<<<grid, 64 * W, sh, st>>>(g, fd, g.real ? nullptr : (const double2 *)f,
-                                                   g.real ? (const double *)f : nullptr, X2, scale);
-    };
-    fd.blue ? go(k_fwd_gamma_reg<W, 1>) : go(k_fwd_gamma_reg<W, 0>);
+    const size_t sh = reg_shared<kRegWaves>(kRegBlueTab, 0);
+    const unsigned grid = reg_grid(k_fwd_gamma_col, kThreads, sh, (S + kRegWaves - 1) / kRegWaves);
+    k_fwd_gamma_col<<<grid, kThreads, sh, st>>>(
+        g, fd, g.real ? nullptr : (const double2 *)f, g.real ? (const double *)f : nullptr, X2, scale);
     SO3_LAUNCH_CHECK();
     return;
   }
@@ -844,13 +913,13 @@ void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, double scale, hi
 
 void launch_fwd_beta(const Geo &g, const double2 *X2, double2 *Gp, NBase nb, const double2 *phase,
                      const double2 *wr, int Pw, int adjoint, hipStream_t st) {
-  if (Pw == 1024 && fft_plan(g.Pb).len == 1024 && fft_plan(g.Pb).blue && g.nB <= 1024) {
+  if (Pw == 1024 && g.L <= 256 && fft_plan(g.Pb).len == 1024 && fft_plan(g.Pb).blue) {
     const FftDesc fb = fft_plan(g.Pb), fw = fft_plan(Pw);
     constexpr int W = kBetaW;
-    const size_t sh = col_shared<W>(g.nB);
-    set_lds(k_fwd_beta_reg<W>, sh);
+    const size_t sh = reg_shared<W>(kRegBlueTab, g.nB);
     const int chunks = (g.nA + W - 1) / W;
-    k_fwd_beta_reg<W><<<g.nG * chunks, 64 * W, sh, st>>>(g, fb, fw, X2, Gp, nb.base, phase, wr, adjoint);
+    const unsigned grid = reg_grid(k_fwd_beta_reg<W>, 64 * W, sh, (int64_t)g.nG * chunks);
+    k_fwd_beta_reg<W><<<grid, 64 * W, sh, st>>>(g, fb, fw, X2, Gp, nb.base, phase, wr, adjoint);
     SO3_LAUNCH_CHECK();
     return;
   }
