@@ -69,9 +69,9 @@ def stage_model(p, A, T):
 
 
 # kernel that carries each stage (names as rocprofv3 reports them)
-STAGE_KERNEL = {"contract_inv": "k_contract_inv", "beta_inv": "k_inv_beta", "alpha_inv": "k_rows",
-                "gamma_inv": "k_inv_gamma", "gamma_fwd": "k_fwd_gamma", "alpha_fwd": "k_rows",
-                "beta_fwd": "k_fwd_beta", "contract_fwd": "k_contract_fwd"}
+STAGE_KERNEL = {"contract_inv": "k_contract_inv", "beta_inv": "k_inv_beta_pair", "alpha_inv": "k_rows_reg<1, 1>",
+                "gamma_inv": "k_gamma_tile<2>", "gamma_fwd": "k_gamma_tile<3>", "alpha_fwd": "k_rows_reg<-1, 1>",
+                "beta_fwd": "k_fwd_beta_pair", "contract_fwd": "k_contract_fwd"}
 
 
 def measured_traffic(kernel):
@@ -82,7 +82,7 @@ def measured_traffic(kernel):
         return None
     for line in open(path):
         parts = line.split()
-        if len(parts) == 3 and not line.startswith("#") and parts[0].startswith(kernel):
+        if len(parts) == 3 and not line.startswith("#") and parts[0].startswith(kernel.replace(" ", "")):
             return float(parts[1]) + float(parts[2])
     return None
 

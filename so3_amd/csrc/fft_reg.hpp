@@ -27,8 +27,9 @@ namespace so3g {
 constexpr int kRegLd = 68;            // row stride (doubles) of the transpose buffer
 constexpr int kRegT = 16 * kRegLd;    // doubles per wavefront (8704 bytes)
 constexpr int kRegTw = 1024 + 64;     // double2 of the twiddle tables tw1 | tw2
-// double2 of the Bluestein tables: tw1 | tw2 | kernel spectrum (1024) | chirp
-// (n <= 512 entries: a length-1024 Bluestein plan has n <= 512)
+// double2 of the Bluestein tables: tw1 | tw2 | kernel spectrum (1024, in
+// register order: entry c * 64 + lane is the value at o_index(lane, c)) |
+// chirp (n <= 512 entries: a length-1024 Bluestein plan has n <= 512)
 constexpr int kRegBlueTab = kRegTw + 1024 + 512;
 
 __device__ __forceinline__ int bitrev2(int a) { return ((a & 1) << 1) | (a >> 1); }
@@ -92,7 +93,7 @@ template <int DIR>
 __device__ __forceinline__ void reg_stage_blue(double2 *dst, const FftDesc &d) {
   const double2 *bh = DIR < 0 ? d.bhat_f : d.bhat_b;
   for (int i = threadIdx.x; i < kRegTw; i += blockDim.x) dst[i] = d.tws[i];
-  for (int i = threadIdx.x; i < 1024; i += blockDim.x) dst[kRegTw + i] = bh[i];
+  for (int i = threadIdx.x; i < 1024; i += blockDim.x) dst[kRegTw + i] = bh[o_index(i & 63, i >> 6)];
   for (int i = threadIdx.x; i < d.n; i += blockDim.x) dst[kRegTw + 1024 + i] = d.chirp[i];
 }
 
@@ -198,7 +199,7 @@ __device__ __forceinline__ void blue_reg(double2 *v, int n, const double2 *tab, 
   phase_fence();
   fft_F<-1>(v, T, tw, tw + 1024, lane);
 #pragma unroll
-  for (int c = 0; c < 16; ++c) v[c] = cmul(v[c], bh[o_index(lane, c)]);
+  for (int c = 0; c < 16; ++c) v[c] = cmul(v[c], bh[c * 64 + lane]);
   phase_fence();
   fft_G<+1>(v, T, tw, tw + 1024, lane);
   phase_fence();

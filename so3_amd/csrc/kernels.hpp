@@ -60,15 +60,27 @@ void launch_contract_fwd(const Geo &g, const double2 *Gp, NBase nb, const DlTabl
 
 void launch_inv_beta(const Geo &g, const double2 *X1, double2 *X2, NBase nb, const double2 *phase,
                      hipStream_t st);
+// Complex transforms whose alpha and gamma DFTs both run on the register
+// path take the gamma DFT in place on X2 (whose blocked rows keep a column's
+// accesses within a few pages) and the alpha DFT between X2 rows and f rows:
+//   inverse: launch_gamma_x2(+1) then launch_alpha_x2f(+1)
+//   forward: launch_alpha_fx2(-1) then launch_gamma_x2(-1)
+bool reg_order(const Geo &g);
+void launch_gamma_x2(const Geo &g, double2 *X2, int dir, double scale, hipStream_t st);
+void launch_alpha_x2f(const Geo &g, const double2 *X2, double2 *f, int dir, double scale, hipStream_t st);
+void launch_alpha_fx2(const Geo &g, const double2 *f, double2 *X2, int dir, double scale, hipStream_t st);
 void launch_alpha(const Geo &g, double2 *X2, int dir, double scale, hipStream_t st);
 // nrows DFTs of length n (sign dir) on rows ld apart, results times scale
 void launch_rows(double2 *X, int64_t nrows, int n, int ld, int dir, double scale, hipStream_t st);
 void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, double scale, hipStream_t st);
 void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, double scale, hipStream_t st);
 // adjoint != 0: zero-fill instead of the beta extension and no weights (the
-// beta stage of the adjoint of the inverse)
+// beta stage of the adjoint of the inverse).  wr_even (the weight spectrum
+// without the two odd taps) and pole (the folded responses of the pole
+// samples, 2L entries) serve the paired register path; null elsewhere.
 void launch_fwd_beta(const Geo &g, const double2 *X2, double2 *Gp, NBase nb, const double2 *phase,
-                     const double2 *wr, int Pw, int adjoint, hipStream_t st);
+                     const double2 *wr, const double2 *wr_even, const double2 *pole, int Pw, int adjoint,
+                     hipStream_t st);
 // beta stage of the adjoint of the forward transform (so3_adjoint.c:730-905):
 // m' < 0 by symmetry, weight convolution, phase exp(+i m' pi/(2L-1)),
 // backward DFT over m' / (2L-1), adjoint of the extension

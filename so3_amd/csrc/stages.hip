@@ -122,19 +122,26 @@ __device__ __forceinline__ int loop_lane() {
 }
 
 // LDS of a register-path block: `tabs` double2 of tables, then W transpose
-// buffers (which the beta kernels also use as their tile of `rows` x (W + 1)
-// entries)
+// buffers, which the tiled kernels also use as their tile of `tile` double2
 template <int W>
 constexpr int tile_ld() { return W + 1; }
 template <int W>
-size_t reg_shared(int tabs, int rows) {
+size_t reg_shared(int tabs, int64_t tile) {
   return sizeof(double2) * tabs +
-         std::max<size_t>(sizeof(double2) * (size_t)rows * tile_ld<W>(), sizeof(double) * (size_t)W * kRegT);
+         std::max<size_t>(sizeof(double2) * (size_t)tile, sizeof(double) * (size_t)W * kRegT);
 }
+
+// Row maps of the rows kernel: row r of the transform reads src and writes
+// dst at
+//   kRowsFlat : r rs in both (in place when src == dst)
+//   kRowsX2F  : X2 row x2_row(k, b) -> f row r = k nB + b (rows of nA)
+//   kRowsFX2  : f row r -> X2 row x2_row(k, b)
+enum RowsMap { kRowsFlat = 0, kRowsX2F = 1, kRowsFX2 = 2 };
 
 template <int DIR, int BLUE>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_rows_reg(
-    FftDesc fd, double2 *__restrict__ X, int64_t nrows, int n, int64_t rs, double scale) {
+    FftDesc fd, const double2 *src, double2 *dst, int64_t nrows, int n, int64_t rs, double scale, Geo g,
+    int map) {
   extern __shared__ double2 lds[];
   const int wv = threadIdx.x >> 6;
   constexpr int kTabs = BLUE ? kRegBlueTab : kRegTw;
@@ -148,12 +155,20 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))
     const int lane = loop_lane();
     const int64_t r = t * kRegWaves + wv;
     if (r >= nrows) break;  // the whole wavefront, last group
-    double2 *row = X + r * rs;
+    int64_t ri = r * rs, ro = r * rs;
+    if (map != kRowsFlat) {
+      const int k = (int)(r / g.nB), b = (int)(r % g.nB);
+      const int64_t xr = x2_row(g, k, b) * g.ldA, fr = r * g.nA;
+      ri = map == kRowsX2F ? xr : fr;
+      ro = map == kRowsX2F ? fr : xr;
+    }
+    const double2 *in = src + ri;
+    double2 *row = dst + ro;
     double2 v[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int k = lane + 64 * q;
-      v[q] = k < n ? row[k] : make_double2(0.0, 0.0);
+      v[q] = k < n ? in[k] : make_double2(0.0, 0.0);
     }
     if constexpr (BLUE) {
       blue_reg<DIR>(v, n, tw, T, lane);
@@ -170,94 +185,92 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))
   }
 }
 
-// Gamma stages, Bluestein length 2N-1 with inner length 1024: one column
-// (b, a) per wavefront, moved straight between HBM and the register layout
-// (lane l, slot q holds n slot l + 64 q), kRegWaves consecutive columns per
-// group.  Each access instruction touches 64 lines; the columns of the
-// neighbouring groups on the same XCD use the rest of each line from L2.  On
-// the X2 side the 64 lines lie within a few pages (x2_row); on the f side
-// they are nA nB entries apart.  No barrier separates the load, DFT and store
-// phases, so the blocks of a CU overlap one's memory traffic with another's
-// DFT.
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_inv_gamma_col(
-    Geo g, FftDesc fd, const double2 *__restrict__ X2, double2 *__restrict__ fc, double *__restrict__ fr,
-    double scale) {
-  extern __shared__ double2 lds[];
-  const int wv = threadIdx.x >> 6;
-  double2 *tab = lds;
-  double *T = (double *)(lds + kRegBlueTab) + wv * kRegT;
-  reg_stage_blue<+1>(tab, fd);
-  __syncthreads();
-  const int64_t S = (int64_t)g.nB * g.nA;
-  const int64_t groups = (S + kRegWaves - 1) / kRegWaves;
-  const int nG = 2 * g.N - 1;
-  for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
-    const int lane = loop_lane();
-    const int64_t p = t * kRegWaves + wv;
-    if (p >= S) break;  // the whole wavefront, last group
-    const int b = (int)(p / g.nA), a = (int)(p % g.nA);
-    // column (b, a) of X2: n slot k at col[k * cs]
-    const double2 *col = X2 + x2_row(g, 0, b) * g.ldA + a;
-    const int64_t cs = (int64_t)x2_w(g, b) * g.ldA;
-    double2 v[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int k = lane + 64 * q;
-      double2 x = make_double2(0.0, 0.0);
-      if (k < nG) {
-        if (!g.real) x = col[k * cs];
-        else if (k == 0) x = make_double2(col[0].x, 0.0);
-        else if (k < g.N) x = col[k * cs];
-        else x = cconj(col[(nG - k) * cs]);
-      }
-      v[q] = x;
-    }
-    blue_reg<+1>(v, nG, tab, T, lane);
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int k = lane + 64 * q;
-      if (k < nG) {
-        if (g.real) fr[k * S + p] = v[q].x * scale;
-        else fc[k * S + p] = cscale(v[q], scale);
-      }
-    }
-  }
-}
+// Gamma stages, Bluestein length 2N-1 with inner length 1024: a group is
+// kRegWaves consecutive columns p = b nA + a (one per wavefront), moved
+// between HBM and the registers through an LDS tile of nG x kRegWaves
+// entries, so that one access instruction covers 16 n slots x 4 columns: 16
+// runs of 64 bytes (the neighbouring group on the same XCD, xcd_tile, uses
+// the other half of each 128-byte line) instead of 64 scattered entries.
+// The tile is unpadded; entry (k, c) sits at gtile(k, c), whose swizzle
+// keeps both the row-wise and the column-wise accesses free of bank
+// conflicts.  It shares LDS with the transpose buffers (barriers separate
+// the uses).
+__device__ __forceinline__ int gtile(int k, int c) { return 4 * k + (c ^ ((k >> 2) & 3)); }
 
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_fwd_gamma_col(
-    Geo g, FftDesc fd, const double2 *__restrict__ fc, const double *__restrict__ fr,
-    double2 *__restrict__ X2, double sc) {
+// Gamma tile kernel modes: where the columns come from and go to.
+//   kGxF  : inverse, X2 -> f        kFxG  : forward, f -> X2
+//   kGinv : inverse, in place on X2 kGfwd : forward, in place on X2 (complex)
+enum GammaMode { kGxF = 0, kFxG = 1, kGinv = 2, kGfwd = 3 };
+
+template <int MODE>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gamma_tile(
+    Geo g, FftDesc fd, const void *src, void *dst, double scale) {
+  // src and dst are the same array in the in-place modes
+  static_assert(kRegWaves == 4, "gamma tile of 4 columns");
+  constexpr int DIR = (MODE == kGxF || MODE == kGinv) ? +1 : -1;
   extern __shared__ double2 lds[];
   const int wv = threadIdx.x >> 6;
-  double2 *tab = lds;
+  double2 *tab = lds, *tile = lds + kRegBlueTab;
   double *T = (double *)(lds + kRegBlueTab) + wv * kRegT;
-  reg_stage_blue<-1>(tab, fd);
+  reg_stage_blue<DIR>(tab, fd);
   __syncthreads();
   const int64_t S = (int64_t)g.nB * g.nA;
   const int64_t groups = (S + kRegWaves - 1) / kRegWaves;
   const int nG = 2 * g.N - 1;
-  const int nout = g.real ? g.N : nG;
+  const int nout = (MODE == kFxG && g.real) ? g.N : nG;
+  const double2 *X2s = (const double2 *)src;
+  double2 *X2d = (double2 *)dst;
   for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
     const int lane = loop_lane();
-    const int64_t p = t * kRegWaves + wv;
-    if (p >= S) break;  // the whole wavefront, last group
-    const int b = (int)(p / g.nA), a = (int)(p % g.nA);
+    const int64_t p0 = t * kRegWaves;
+    for (int idx = threadIdx.x; idx < nG * 4; idx += kThreads) {
+      const int k = idx >> 2, c = idx & 3;
+      const int64_t p = p0 + c;
+      double2 x = make_double2(0.0, 0.0);
+      if (p < S) {
+        const int b = (int)(p / g.nA), a = (int)(p % g.nA);
+        if constexpr (MODE == kFxG) {
+          x = g.real ? make_double2(((const double *)src)[k * S + p], 0.0) : ((const double2 *)src)[k * S + p];
+        } else if (MODE == kGxF && g.real) {
+          if (k == 0) x = make_double2(X2s[x2_row(g, 0, b) * g.ldA + a].x, 0.0);
+          else if (k < g.N) x = X2s[x2_row(g, k, b) * g.ldA + a];
+          else x = cconj(X2s[x2_row(g, nG - k, b) * g.ldA + a]);
+        } else {
+          x = X2s[x2_row(g, k, b) * g.ldA + a];
+        }
+      }
+      tile[gtile(k, c)] = x;
+    }
+    __syncthreads();
     double2 v[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int k = lane + 64 * q;
-      double2 x = make_double2(0.0, 0.0);
-      if (k < nG) x = g.real ? make_double2(fr[k * S + p], 0.0) : fc[k * S + p];
-      v[q] = x;
+      v[q] = k < nG ? tile[gtile(k, wv)] : make_double2(0.0, 0.0);
     }
-    blue_reg<-1>(v, nG, tab, T, lane);
-    double2 *col = X2 + x2_row(g, 0, b) * g.ldA + a;
-    const int64_t cs = (int64_t)x2_w(g, b) * g.ldA;
+    __syncthreads();  // the tile is read before the transposes reuse it
+    blue_reg<DIR>(v, nG, tab, T, lane);
+    __syncthreads();
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int k = lane + 64 * q;
-      if (k < nout) col[k * cs] = cscale(v[q], sc);
+      if (k < nout) tile[gtile(k, wv)] = v[q];
     }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < nout * 4; idx += kThreads) {
+      const int k = idx >> 2, c = idx & 3;
+      const int64_t p = p0 + c;
+      if (p >= S) continue;
+      const double2 x = tile[gtile(k, c)];
+      if constexpr (MODE == kGxF) {
+        if (g.real) ((double *)dst)[k * S + p] = x.x * scale;
+        else ((double2 *)dst)[k * S + p] = cscale(x, scale);
+      } else {
+        const int b = (int)(p / g.nA), a = (int)(p % g.nA);
+        X2d[x2_row(g, k, b) * g.ldA + a] = cscale(x, scale);
+      }
+    }
+    __syncthreads();  // the tile is read out before the next group's loads
   }
 }
 
@@ -641,6 +654,298 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) 
   }
 }
 
+// ---------------------------------------------------------------- paired beta, register path
+// A beta sequence (n, m) is symmetric under m' -> -m' with sign
+// s = (-1)^{m+n}: X1(-m') = s X1(m') on the way in, and the extended samples
+// obey ext(b') = s ext(b), b' = Pb - 1 - b (MW) or Pb - b (MW_SS), on the
+// way out.  Its DFT then has the same symmetry: Z(b') = s Z(b) (inverse),
+// F(-m'') = s F(m'') (forward).  Two sequences A, B of opposite signs share
+// one DFT of A + i B and separate afterwards:
+//   A = (Z(b) + s_A Z(b')) / 2,   B = (Z(b) - s_A Z(b')) / (2 i).
+// Consecutive m have opposite signs, so a wavefront takes the pair
+// (m, m + 1); a group is 2W consecutive m from m_lo = -(L-1) - ss (the
+// MW_SS Nyquist column, zero, first) and leaves or enters through an LDS
+// tile of nB x (2W + 1) entries (128-byte HBM runs for W = 4).
+__device__ __forceinline__ int m_slot_of(const Geo &g, int m) { return m >= 0 ? m : m + g.nA; }
+
+template <int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_inv_beta_pair(
+    Geo g, FftDesc fd, const double2 *__restrict__ X1, double2 *__restrict__ X2,
+    const int64_t *__restrict__ nbase, const double2 *__restrict__ phase) {
+  constexpr int C = 2 * W, LD = C + 1;
+  constexpr int kQB = 5;  // b = lane + 64 q < nB <= 257
+  extern __shared__ double2 lds[];
+  const int wv = threadIdx.x >> 6;
+  double2 *tab = lds, *tile = lds + kRegBlueTab;
+  double *T = (double *)(lds + kRegBlueTab) + wv * kRegT;
+  reg_stage_blue<+1>(tab, fd);
+  __syncthreads();
+  const int L = g.L, Pb = g.Pb;
+  const int mlo = -(L - 1) - g.ss;
+  const int chunks = (g.nA + C - 1) / C;
+  const int64_t groups = (int64_t)g.nG * chunks;
+  for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
+    const int lane = loop_lane();
+    const int nslot = (int)(t / chunks);
+    const int m0 = mlo + (int)(t % chunks) * C;
+    const int n = n_of_slot(g, nslot);
+    const bool active = nbase[n + g.N - 1] != kNoBlock;
+    const int mA = m0 + 2 * wv, mB = mA + 1;
+    const bool hA = active && mA >= -(L - 1) && mA <= L - 1;
+    const bool hB = active && mB >= -(L - 1) && mB <= L - 1;
+    const bool negA = ((mA + n) & 1) != 0;  // s_A = -1 (then s_B = +1)
+    const double sA = negA ? -1.0 : 1.0;
+    const double2 *rA = X1 + ((int64_t)nslot * g.nA + (hA ? m_slot_of(g, mA) : 0)) * L;
+    const double2 *rB = X1 + ((int64_t)nslot * g.nA + (hB ? m_slot_of(g, mB) : 0)) * L;
+    double2 v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int k = lane + 64 * q;
+      double2 x = make_double2(0.0, 0.0);
+      if (k < Pb) {
+        const int mp = k <= L - 1 ? k : k - Pb;
+        if (mp >= -(L - 1)) {
+          const int am = mp < 0 ? -mp : mp;
+          double2 a = hA ? rA[am] : make_double2(0.0, 0.0);
+          double2 b = hB ? rB[am] : make_double2(0.0, 0.0);
+          if (mp < 0) {
+            if (negA) a = make_double2(-a.x, -a.y);
+            else b = make_double2(-b.x, -b.y);
+          }
+          if (mp == 0) {  // the antisymmetric member vanishes at m' = 0
+            if (negA) a = make_double2(0.0, 0.0);
+            else b = make_double2(0.0, 0.0);
+          }
+          x = cmul(make_double2(a.x - b.y, a.y + b.x), phase[k]);
+        }
+      }
+      v[q] = x;
+    }
+    blue_reg<+1>(v, Pb, tab, T, lane);
+    // Z(b') for b < nB through the transpose buffer, real then imaginary parts
+    double zr[kQB];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) T[lane + 64 * q] = v[q].x;
+    __builtin_amdgcn_wave_barrier();
+    phase_fence();
+#pragma unroll
+    for (int j = 0; j < kQB; ++j) {
+      const int b = lane + 64 * j;
+      const int bp = g.ss ? (b == 0 ? 0 : Pb - b) : Pb - 1 - b;
+      zr[j] = b < g.nB ? T[bp] : 0.0;
+    }
+    __builtin_amdgcn_wave_barrier();
+    phase_fence();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) T[lane + 64 * q] = v[q].y;
+    __builtin_amdgcn_wave_barrier();
+    phase_fence();
+    double2 oA[kQB], oB[kQB];
+#pragma unroll
+    for (int j = 0; j < kQB; ++j) {
+      const int b = lane + 64 * j;
+      const int bp = g.ss ? (b == 0 ? 0 : Pb - b) : Pb - 1 - b;
+      const double zi = b < g.nB ? T[bp] : 0.0;
+      const double2 su = make_double2(v[j].x + sA * zr[j], v[j].y + sA * zi);
+      const double2 df = make_double2(v[j].x - sA * zr[j], v[j].y - sA * zi);
+      oA[j] = make_double2(0.5 * su.x, 0.5 * su.y);
+      oB[j] = make_double2(0.5 * df.y, -0.5 * df.x);
+    }
+    __syncthreads();  // every wavefront is done with its transpose buffer
+#pragma unroll
+    for (int j = 0; j < kQB; ++j) {
+      const int b = lane + 64 * j;
+      if (b < g.nB) {
+        tile[b * LD + 2 * wv] = oA[j];
+        tile[b * LD + 2 * wv + 1] = oB[j];
+      }
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < g.nB * C; idx += (64 * W)) {
+      const int b = idx / C, c = idx - b * C;
+      const int m = m0 + c;
+      if (m <= L - 1) X2[x2_row(g, nslot, b) * g.ldA + m_slot_of(g, m)] = tile[b * LD + c];
+    }
+    __syncthreads();  // the tile is read out before the next group's transposes
+  }
+}
+
+// forward beta, paired (not the adjoint): the Bluestein DFT and the
+// convolution with the even weight taps run once per pair.  The two odd
+// taps w(+-1) = -+ i pi/2 cancel in the fold G'(m') = G(m') + s G(-m') for
+// m' > 0 and give 2 pi i pi X~(1) at m' = 0 for the antisymmetric member.
+// The pole samples (b = L-1 for MW, b = 0 and L for MW_SS) of the
+// antisymmetric member break its antisymmetry; they are left out of the
+// pair and added back through their folded responses (pole table).
+template <int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_fwd_beta_pair(
+    Geo g, FftDesc fb, const double2 *__restrict__ X2, double2 *__restrict__ Gp,
+    const int64_t *__restrict__ nbase, const double2 *__restrict__ phase, const double2 *__restrict__ wre,
+    const double2 *__restrict__ pole) {
+  constexpr int C = 2 * W, LD = C + 1;
+  constexpr double kPi = 3.141592653589793238462643383279502884;
+  extern __shared__ double2 lds[];
+  const int wv = threadIdx.x >> 6;
+  double2 *tw = lds, *tile = lds + kRegBlueTab;
+  double *T = (double *)(lds + kRegBlueTab) + wv * kRegT;
+  reg_stage_blue<-1>(tw, fb);
+  __syncthreads();
+  const int L = g.L, Pb = g.Pb;
+  const int mlo = -(L - 1) - g.ss;
+  const int chunks = (g.nA + C - 1) / C;
+  const int64_t groups = (int64_t)g.nG * chunks;
+  const int C2 = 2 * g.NS;
+  for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
+    const int lane = loop_lane();
+    const int nslot = (int)(t / chunks);
+    const int m0 = mlo + (int)(t % chunks) * C;
+    const int n = n_of_slot(g, nslot);
+    if (nbase[n + g.N - 1] == kNoBlock) continue;  // block not transformed (whole workgroup)
+    for (int idx = threadIdx.x; idx < g.nB * C; idx += (64 * W)) {
+      const int b = idx / C, c = idx - b * C;
+      const int m = m0 + c;
+      tile[b * LD + c] = (m >= -(L - 1) && m <= L - 1) ? X2[x2_row(g, nslot, b) * g.ldA + m_slot_of(g, m)]
+                                                       : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    const int mA = m0 + 2 * wv, mB = mA + 1;
+    const bool negA = ((mA + n) & 1) != 0;
+    const double sA = negA ? -1.0 : 1.0;
+    // pole samples of the antisymmetric member
+    const int cx = negA ? 2 * wv : 2 * wv + 1;
+    const double2 p0 = tile[(g.ss ? 0 : L - 1) * LD + cx];
+    const double2 p1 = g.ss ? tile[L * LD + cx] : make_double2(0.0, 0.0);
+    double2 v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int k = lane + 64 * q;
+      double2 x = make_double2(0.0, 0.0);
+      if (k < Pb) {
+        // MW: F(2L-2-b) for b >= L; MW_SS: F(2L-b) for b > L
+        const bool direct = g.ss ? k <= L : k < L;
+        const int kr = direct ? k : (g.ss ? 2 * L - k : 2 * L - 2 - k);
+        double2 a = tile[kr * LD + 2 * wv], b = tile[kr * LD + 2 * wv + 1];
+        if (!direct) {
+          if (negA) a = make_double2(-a.x, -a.y);
+          else b = make_double2(-b.x, -b.y);
+        }
+        const bool pk = g.ss ? (k == 0 || k == L) : (k == L - 1);
+        if (pk) {
+          if (negA) a = make_double2(0.0, 0.0);
+          else b = make_double2(0.0, 0.0);
+        }
+        x = make_double2(a.x - b.y, a.y + b.x);
+      }
+      v[q] = x;
+    }
+    __syncthreads();  // the tile is read before the transposes reuse it
+    blue_reg<-1>(v, Pb, tw, T, lane);
+    // F~(m'') times the phase at m'' mod Pw, |m''| < L, and F~(+-1)
+    double2 z1, zm1;
+    {
+      double t[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) T[lane + 64 * q] = v[q].x;
+      __builtin_amdgcn_wave_barrier();
+      phase_fence();
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int r = lane + 64 * q;
+        const int mpp = r <= L - 1 ? r : r - 1024;
+        t[q] = mpp >= -(L - 1) ? T[mpp < 0 ? mpp + Pb : mpp] : 0.0;
+      }
+      const double r1 = T[1], rm1 = T[Pb - 1];
+      __builtin_amdgcn_wave_barrier();
+      phase_fence();
+#pragma unroll
+      for (int q = 0; q < 16; ++q) T[lane + 64 * q] = v[q].y;
+      __builtin_amdgcn_wave_barrier();
+      phase_fence();
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int r = lane + 64 * q;
+        const int mpp = r <= L - 1 ? r : r - 1024;
+        double2 x = make_double2(0.0, 0.0);
+        if (mpp >= -(L - 1)) x = cmul(make_double2(t[q], T[mpp < 0 ? mpp + Pb : mpp]), phase[mpp + L - 1]);
+        v[q] = x;
+      }
+      z1 = cmul(make_double2(r1, T[1]), phase[L]);
+      zm1 = cmul(make_double2(rm1, T[Pb - 1]), phase[L - 2]);
+      __builtin_amdgcn_wave_barrier();
+      phase_fence();
+    }
+    // convolution with the even weight taps
+    fft_F<+1>(v, T, tw, tw + 1024, lane);
+#pragma unroll
+    for (int c = 0; c < 16; ++c) v[c] = cmul(v[c], wre[o_index(lane, c)]);
+    fft_G<-1>(v, T, tw, tw + 1024, lane);
+    constexpr double sc = 2.0 * kPi / 1024.0;
+    // separate and fold: m' = lane + 64 j
+    constexpr int kFold = 4;  // L <= 256
+    double zpr[kFold], zmr[kFold];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) T[lane + 64 * q] = v[q].x;
+    __builtin_amdgcn_wave_barrier();
+    phase_fence();
+#pragma unroll
+    for (int j = 0; j < kFold; ++j) {
+      const int mp = lane + 64 * j;
+      zpr[j] = T[mp];
+      zmr[j] = T[(1024 - mp) & 1023];
+    }
+    __builtin_amdgcn_wave_barrier();
+    phase_fence();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) T[lane + 64 * q] = v[q].y;
+    __builtin_amdgcn_wave_barrier();
+    phase_fence();
+    const bool sAok = mA >= -(L - 1) && mA <= L - 1, sBok = mB >= -(L - 1) && mB <= L - 1;
+    const int an = n < 0 ? -n : n, sn = n < 0;
+#pragma unroll
+    for (int j = 0; j < kFold; ++j) {
+      const int mp = lane + 64 * j;
+      const double2 zp = make_double2(zpr[j] * sc, T[mp] * sc);
+      const double2 zm = make_double2(zmr[j] * sc, T[(1024 - mp) & 1023] * sc);
+      double2 GA, GB;
+      if (mp > 0) {
+        GA = make_double2(zp.x + sA * zm.x, zp.y + sA * zm.y);
+        GB = make_double2(zp.y - sA * zm.y, -(zp.x - sA * zm.x));
+      } else {
+        const double pp = kPi * kPi;
+        const double2 dz = make_double2(z1.x - zm1.x, z1.y - zm1.y);
+        if (!negA) {
+          GA = zp;
+          GB = make_double2(pp * dz.x, pp * dz.y);
+        } else {
+          GA = make_double2(-pp * dz.y, pp * dz.x);
+          GB = make_double2(zp.y, -zp.x);
+        }
+      }
+      if (mp < L) {
+        double2 corr = cmul(p0, pole[mp]);
+        if (g.ss) corr = cadd(corr, cmul(p1, pole[L + mp]));
+        if (negA) GA = cadd(GA, corr);
+        else GB = cadd(GB, corr);
+      }
+      for (int e = 0; e < 2; ++e) {
+        const int m = e ? mB : mA;
+        if (mp >= L || !(e ? sBok : sAok)) continue;
+        const double2 G0 = e ? GB : GA;
+        const int am = m < 0 ? -m : m, sm = m < 0;
+        double2 *dst = Gp + ((int64_t)an * L + am) * C2 * L;
+        for (int cm = 0; cm < 2; ++cm) {
+          if (cm != sm && am != 0) continue;
+          for (int cn = 0; cn < g.NS; ++cn) {
+            if (cn != sn && an != 0) continue;
+            dst[(int64_t)(cm + 2 * cn) * L + mp] = ((cm != cn) && (mp & 1)) ? make_double2(-G0.x, -G0.y) : G0;
+          }
+        }
+      }
+    }
+    __syncthreads();  // transposes done before the next group's tile
+  }
+}
+
 // ---------------------------------------------------------------- adjoint of the forward: beta
 __global__ __launch_bounds__(kThreads) void k_adjf_beta(Geo g, FftDesc fb, FftDesc fw, int K, int ld,
                                                         const double2 *__restrict__ X1,
@@ -810,12 +1115,31 @@ static unsigned reg_grid(Kern k, int threads, size_t lds, int64_t groups) {
   return (unsigned)std::max<int64_t>(n, 1);
 }
 
+template <int MODE>
+static void launch_gamma_tile(const Geo &g, const FftDesc &fd, const void *src, void *dst, double scale,
+                              hipStream_t st) {
+  const int64_t S = (int64_t)g.nB * g.nA;
+  const size_t sh = reg_shared<kRegWaves>(kRegBlueTab, (int64_t)fd.n * 4);
+  const unsigned grid = reg_grid(k_gamma_tile<MODE>, kThreads, sh, (S + kRegWaves - 1) / kRegWaves);
+  k_gamma_tile<MODE><<<grid, kThreads, sh, st>>>(g, fd, src, dst, scale);
+  SO3_LAUNCH_CHECK();
+}
+
 void launch_inv_beta(const Geo &g, const double2 *X1, double2 *X2, NBase nb, const double2 *phase,
                      hipStream_t st) {
   const FftDesc fd = fft_plan(g.Pb);
+  if (fd.len == 1024 && fd.blue && g.nB <= 257) {
+    constexpr int W = kBetaW;
+    const size_t sh = reg_shared<W>(kRegBlueTab, (int64_t)g.nB * (2 * W + 1));
+    const int chunks = (g.nA + 2 * W - 1) / (2 * W);
+    const unsigned grid = reg_grid(k_inv_beta_pair<W>, 64 * W, sh, (int64_t)g.nG * chunks);
+    k_inv_beta_pair<W><<<grid, 64 * W, sh, st>>>(g, fd, X1, X2, nb.base, phase);
+    SO3_LAUNCH_CHECK();
+    return;
+  }
   if (fd.len == 1024 && fd.blue && g.nB <= 1024) {
     constexpr int W = kBetaW;
-    const size_t sh = reg_shared<W>(kRegBlueTab, g.nB);
+    const size_t sh = reg_shared<W>(kRegBlueTab, (int64_t)g.nB * tile_ld<W>());
     const int chunks = (g.nA + W - 1) / W;
     const unsigned grid = reg_grid(k_inv_beta_reg<W>, 64 * W, sh, (int64_t)g.nG * chunks);
     k_inv_beta_reg<W><<<grid, 64 * W, sh, st>>>(g, fd, X1, X2, nb.base, phase);
@@ -832,20 +1156,49 @@ void launch_inv_beta(const Geo &g, const double2 *X1, double2 *X2, NBase nb, con
   SO3_LAUNCH_CHECK();
 }
 
+// rows of length n on the register path (fft_plan(n).len == 1024)
+static void launch_rows_reg(const FftDesc &fd, const double2 *src, double2 *dst, int64_t nrows, int n,
+                            int64_t rs, int dir, double scale, const Geo &g, int map, hipStream_t st) {
+  const size_t sh = reg_shared<kRegWaves>(fd.blue ? kRegBlueTab : kRegTw, 0);
+  auto go = [&](auto kern) {
+    const unsigned grid = reg_grid(kern, kThreads, sh, (nrows + kRegWaves - 1) / kRegWaves);
+    kern<<<grid, kThreads, sh, st>>>(fd, src, dst, nrows, n, rs, scale, g, map);
+  };
+  if (fd.blue)
+    dir > 0 ? go(k_rows_reg<+1, 1>) : go(k_rows_reg<-1, 1>);
+  else
+    dir > 0 ? go(k_rows_reg<+1, 0>) : go(k_rows_reg<-1, 0>);
+  SO3_LAUNCH_CHECK();
+}
+
+bool reg_order(const Geo &g) {
+  if (g.real) return false;
+  const FftDesc fa = fft_plan(g.nA), fg = fft_plan(2 * g.N - 1);
+  return fa.len == 1024 && fg.len == 1024 && fg.blue;
+}
+
+void launch_gamma_x2(const Geo &g, double2 *X2, int dir, double scale, hipStream_t st) {
+  const FftDesc fd = fft_plan(2 * g.N - 1);
+  if (!reg_order(g)) throw HipError("gamma in place on X2 needs the register path");
+  if (dir > 0) launch_gamma_tile<kGinv>(g, fd, X2, X2, scale, st);
+  else launch_gamma_tile<kGfwd>(g, fd, X2, X2, scale, st);
+}
+
+void launch_alpha_x2f(const Geo &g, const double2 *X2, double2 *f, int dir, double scale, hipStream_t st) {
+  if (!reg_order(g)) throw HipError("alpha X2 -> f needs the register path");
+  launch_rows_reg(fft_plan(g.nA), X2, f, (int64_t)g.nG * g.nB, g.nA, 0, dir, scale, g, kRowsX2F, st);
+}
+
+void launch_alpha_fx2(const Geo &g, const double2 *f, double2 *X2, int dir, double scale, hipStream_t st) {
+  if (!reg_order(g)) throw HipError("alpha f -> X2 needs the register path");
+  launch_rows_reg(fft_plan(g.nA), f, X2, (int64_t)g.nG * g.nB, g.nA, 0, dir, scale, g, kRowsFX2, st);
+}
+
 void launch_rows(double2 *X, int64_t nrows, int n, int ldr, int dir, double scale, hipStream_t st) {
   const FftDesc fd = fft_plan(n);
   if (nrows <= 0) return;
   if (fd.len == 1024) {
-    const size_t sh = reg_shared<kRegWaves>(fd.blue ? kRegBlueTab : kRegTw, 0);
-    auto go = [&](auto kern) {
-      const unsigned grid = reg_grid(kern, kThreads, sh, (nrows + kRegWaves - 1) / kRegWaves);
-      kern<<<grid, kThreads, sh, st>>>(fd, X, nrows, n, (int64_t)ldr, scale);
-    };
-    if (fd.blue)
-      dir > 0 ? go(k_rows_reg<+1, 1>) : go(k_rows_reg<-1, 1>);
-    else
-      dir > 0 ? go(k_rows_reg<+1, 0>) : go(k_rows_reg<-1, 0>);
-    SO3_LAUNCH_CHECK();
+    launch_rows_reg(fd, X, X, nrows, n, ldr, dir, scale, Geo{}, kRowsFlat, st);
     return;
   }
   const int ld = fft_ld(fd);
@@ -870,12 +1223,7 @@ void launch_alpha(const Geo &g, double2 *X2, int dir, double scale, hipStream_t 
 void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, double scale, hipStream_t st) {
   const FftDesc fd = fft_plan(2 * g.N - 1);
   if (fd.len == 1024 && fd.blue) {
-    const int64_t S = (int64_t)g.nB * g.nA;
-    const size_t sh = reg_shared<kRegWaves>(kRegBlueTab, 0);
-    const unsigned grid = reg_grid(k_inv_gamma_col, kThreads, sh, (S + kRegWaves - 1) / kRegWaves);
-    k_inv_gamma_col<<<grid, kThreads, sh, st>>>(
-        g, fd, X2, g.real ? nullptr : (double2 *)f, g.real ? (double *)f : nullptr, scale);
-    SO3_LAUNCH_CHECK();
+    launch_gamma_tile<kGxF>(g, fd, X2, f, scale, st);
     return;
   }
   const int ld = fft_ld(fd);
@@ -892,12 +1240,7 @@ void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, double scale, hi
 void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, double scale, hipStream_t st) {
   const FftDesc fd = fft_plan(2 * g.N - 1);
   if (fd.len == 1024 && fd.blue) {
-    const int64_t S = (int64_t)g.nB * g.nA;
-    const size_t sh = reg_shared<kRegWaves>(kRegBlueTab, 0);
-    const unsigned grid = reg_grid(k_fwd_gamma_col, kThreads, sh, (S + kRegWaves - 1) / kRegWaves);
-    k_fwd_gamma_col<<<grid, kThreads, sh, st>>>(
-        g, fd, g.real ? nullptr : (const double2 *)f, g.real ? (const double *)f : nullptr, X2, scale);
-    SO3_LAUNCH_CHECK();
+    launch_gamma_tile<kFxG>(g, fd, f, X2, scale, st);
     return;
   }
   const int ld = fft_ld(fd);
@@ -912,11 +1255,23 @@ void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, double scale, hi
 }
 
 void launch_fwd_beta(const Geo &g, const double2 *X2, double2 *Gp, NBase nb, const double2 *phase,
-                     const double2 *wr, int Pw, int adjoint, hipStream_t st) {
+                     const double2 *wr, const double2 *wr_even, const double2 *pole, int Pw, int adjoint,
+                     hipStream_t st) {
+  if (!adjoint && wr_even && pole && Pw == 1024 && g.L <= 256 && fft_plan(g.Pb).len == 1024 &&
+      fft_plan(g.Pb).blue) {
+    const FftDesc fb = fft_plan(g.Pb);
+    constexpr int W = kBetaW;
+    const size_t sh = reg_shared<W>(kRegBlueTab, (int64_t)g.nB * (2 * W + 1));
+    const int chunks = (g.nA + 2 * W - 1) / (2 * W);
+    const unsigned grid = reg_grid(k_fwd_beta_pair<W>, 64 * W, sh, (int64_t)g.nG * chunks);
+    k_fwd_beta_pair<W><<<grid, 64 * W, sh, st>>>(g, fb, X2, Gp, nb.base, phase, wr_even, pole);
+    SO3_LAUNCH_CHECK();
+    return;
+  }
   if (Pw == 1024 && g.L <= 256 && fft_plan(g.Pb).len == 1024 && fft_plan(g.Pb).blue) {
     const FftDesc fb = fft_plan(g.Pb), fw = fft_plan(Pw);
     constexpr int W = kBetaW;
-    const size_t sh = reg_shared<W>(kRegBlueTab, g.nB);
+    const size_t sh = reg_shared<W>(kRegBlueTab, (int64_t)g.nB * tile_ld<W>());
     const int chunks = (g.nA + W - 1) / W;
     const unsigned grid = reg_grid(k_fwd_beta_reg<W>, 64 * W, sh, (int64_t)g.nG * chunks);
     k_fwd_beta_reg<W><<<grid, 64 * W, sh, st>>>(g, fb, fw, X2, Gp, nb.base, phase, wr, adjoint);

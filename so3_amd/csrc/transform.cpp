@@ -36,6 +36,8 @@ struct Plan {
   double2 *phase_ai = nullptr;   // 2L-1: exp(-i m'' pi/(2L-1)), adjoint of the inverse
   double2 *phase_af = nullptr;   // 2L-1: exp(+i m' pi/(2L-1)) / Pb, adjoint of the forward
   double2 *wr = nullptr;         // Pw entries
+  double2 *wr_even = nullptr;    // Pw entries: the even taps of the weights only
+  double2 *pole = nullptr;       // 2L entries: folded responses of the pole samples
   int64_t *nbase = nullptr;      // 2*(2N-1): masked, then unmasked
   size_t x1 = 0, x2 = 0;         // workspace sizes (double2)
 };
@@ -145,20 +147,61 @@ Plan build_plan(const so3_parameters_t *p, Method method) {
       w[((q % Pw) + Pw) % Pw] = v;
     }
     // wr[r] = sum_q w[q] exp(+2 pi i q r / Pw); w has 2L-1 non-zero even taps + 2
-    std::vector<double2> wr(Pw);
+    // odd ones (q = +-1); wr_even leaves the odd ones out
+    std::vector<double2> wr(Pw), wre(Pw);
     for (int r = 0; r < Pw; ++r) {
-      cld acc(0, 0);
+      cld acc(0, 0), odd(0, 0);
       for (int q = -2 * (L - 1); q <= 2 * (L - 1); ++q) {
         const cld wq = w[((q % Pw) + Pw) % Pw];
         if (wq == cld(0, 0)) continue;
         long long k = ((long long)q * r) % Pw;
         if (k < 0) k += Pw;
         const long double a = 2.0L * kPiL * k / Pw;
-        acc += wq * cld(cosl(a), sinl(a));
+        const cld t = wq * cld(cosl(a), sinl(a));
+        acc += t;
+        if (q == 1 || q == -1) odd += t;
       }
       wr[r] = make_double2((double)acc.real(), (double)acc.imag());
+      const cld ev = acc - odd;
+      wre[r] = make_double2((double)ev.real(), (double)ev.imag());
     }
     P.wr = upload(wr);
+    P.wr_even = upload(wre);
+    // Pole samples (b = L-1 for MW; b = 0 and b = L for MW_SS) map to
+    // themselves under the extension, so an antisymmetric sequence
+    // ((-1)^{m+n} = -1) with a non-zero sample there is not antisymmetric.
+    // pole[j L + m'] is the folded (sign -1) forward-beta response to a unit
+    // sample at the j-th pole sample: G(m') - G(-m') for m' > 0, G(0) at 0,
+    // G(m') = 2 pi sum_k F(k) w(m' - k), F(k) = exp(-2 pi i k b0 / Pb) * phase(k).
+    if (L <= 256) {
+      std::vector<double2> pole(2 * L, make_double2(0.0, 0.0));
+      const int npole = g.ss ? 2 : 1;
+      for (int j = 0; j < npole; ++j) {
+        const int b0 = g.ss ? j * L : L - 1;
+        std::vector<cld> F(2 * L - 1), G(2 * L - 1);
+        for (int k = -(L - 1); k <= L - 1; ++k) {
+          long long e = ((long long)k * b0) % g.Pb;
+          if (e < 0) e += g.Pb;
+          const long double a = -2.0L * kPiL * e / g.Pb;
+          const long double ph = g.ss ? 0.0L : -kPiL * k / (2.0L * L - 1.0L);
+          F[k + L - 1] = cld(cosl(a), sinl(a)) * cld(cosl(ph), sinl(ph)) / (long double)g.Pb;
+        }
+        for (int mp = -(L - 1); mp <= L - 1; ++mp) {
+          cld acc(0, 0);
+          for (int k = -(L - 1); k <= L - 1; ++k) {
+            const int q = mp - k;
+            const cld wq = w[((q % Pw) + Pw) % Pw];
+            if (wq != cld(0, 0)) acc += F[k + L - 1] * wq;
+          }
+          G[mp + L - 1] = 2.0L * kPiL * acc;
+        }
+        for (int mp = 0; mp < L; ++mp) {
+          const cld t = mp == 0 ? G[L - 1] : G[mp + L - 1] - G[-mp + L - 1];
+          pole[j * L + mp] = make_double2((double)t.real(), (double)t.imag());
+        }
+      }
+      P.pole = upload(pole);
+    }
   }
   // per-n storage bases
   std::vector<int64_t> nb(2 * (2 * N - 1), kNoBlock);
@@ -330,8 +373,13 @@ void enqueue_inverse_locked(const Plan &P, const so3_parameters_t *p, const void
   NBase nb{P.nbase};
   timed(0, st, [&] { launch_contract_inv(g, (const double2 *)flmn, nb, dl, X1, st); });
   timed(1, st, [&] { launch_inv_beta(g, X1, X2, nb, P.phase_inv, st); });
-  timed(2, st, [&] { launch_alpha(g, X2, +1, 1.0, st); });
-  timed(3, st, [&] { launch_inv_gamma(g, X2, f, 1.0, st); });
+  if (reg_order(g)) {
+    timed(3, st, [&] { launch_gamma_x2(g, X2, +1, 1.0, st); });
+    timed(2, st, [&] { launch_alpha_x2f(g, X2, (double2 *)f, +1, 1.0, st); });
+  } else {
+    timed(2, st, [&] { launch_alpha(g, X2, +1, 1.0, st); });
+    timed(3, st, [&] { launch_inv_gamma(g, X2, f, 1.0, st); });
+  }
 }
 
 void enqueue_forward_locked(const Plan &P, const so3_parameters_t *p, Method method, const void *f, void *flmn,
@@ -340,9 +388,14 @@ void enqueue_forward_locked(const Plan &P, const so3_parameters_t *p, Method met
   double2 *Gp = ws, *X2 = ws + P.x1;
   const Geo &g = P.g;
   NBase nb{P.nbase};
-  timed(4, st, [&] { launch_fwd_gamma(g, f, X2, 2 * M_PI / (2.0 * g.N - 1.0), st); });
-  timed(5, st, [&] { launch_alpha(g, X2, -1, 1.0 / g.nA, st); });
-  timed(6, st, [&] { launch_fwd_beta(g, X2, Gp, nb, P.phase_fwd, P.wr, P.Pw, 0, st); });
+  if (reg_order(g)) {
+    timed(5, st, [&] { launch_alpha_fx2(g, (const double2 *)f, X2, -1, 1.0 / g.nA, st); });
+    timed(4, st, [&] { launch_gamma_x2(g, X2, -1, 2 * M_PI / (2.0 * g.N - 1.0), st); });
+  } else {
+    timed(4, st, [&] { launch_fwd_gamma(g, f, X2, 2 * M_PI / (2.0 * g.N - 1.0), st); });
+    timed(5, st, [&] { launch_alpha(g, X2, -1, 1.0 / g.nA, st); });
+  }
+  timed(6, st, [&] { launch_fwd_beta(g, X2, Gp, nb, P.phase_fwd, P.wr, P.wr_even, P.pole, P.Pw, 0, st); });
   timed(7, st, [&] {
     launch_fwd_zero(g, (double2 *)flmn, nb, method == kDirect, 0, (int)p->storage, (int)p->n_order, st);
   });
@@ -418,9 +471,14 @@ void run_adjoint_inverse(const so3_parameters_t *p, const void *f, void *flmn, h
   double2 *Gp = ws, *X2 = ws + P.x1;
   Geo g = P.g;
   NBase nb{P.nbase};
-  launch_fwd_gamma(g, f, X2, 1.0, st);
-  launch_alpha(g, X2, -1, 1.0, st);
-  launch_fwd_beta(g, X2, Gp, nb, P.phase_ai, P.wr, P.Pw, 1, st);
+  if (reg_order(g)) {
+    launch_alpha_fx2(g, (const double2 *)f, X2, -1, 1.0, st);
+    launch_gamma_x2(g, X2, -1, 1.0, st);
+  } else {
+    launch_fwd_gamma(g, f, X2, 1.0, st);
+    launch_alpha(g, X2, -1, 1.0, st);
+  }
+  launch_fwd_beta(g, X2, Gp, nb, P.phase_ai, P.wr, nullptr, nullptr, P.Pw, 1, st);
   launch_fwd_zero(g, (double2 *)flmn, nb, 1, 0, (int)p->storage, (int)p->n_order, st);
   g.clmode = 2;
   launch_contract_fwd(g, Gp, nb, dl, (double2 *)flmn, st);
@@ -439,8 +497,13 @@ void run_adjoint_forward(const so3_parameters_t *p, const void *flmn, void *f, h
   NBase nb{P.nbase};
   launch_contract_inv(g, (const double2 *)flmn, nb, dl, X1, st);
   launch_adjf_beta(g, X1, X2, nb, P.phase_af, P.wr, P.Pw, st);
-  launch_alpha(g, X2, +1, 1.0 / g.nA, st);
-  launch_inv_gamma(g, X2, f, 1.0 / (2.0 * g.N - 1.0), st);
+  if (reg_order(g)) {
+    launch_gamma_x2(g, X2, +1, 1.0 / (2.0 * g.N - 1.0), st);
+    launch_alpha_x2f(g, X2, (double2 *)f, +1, 1.0 / g.nA, st);
+  } else {
+    launch_alpha(g, X2, +1, 1.0 / g.nA, st);
+    launch_inv_gamma(g, X2, f, 1.0 / (2.0 * g.N - 1.0), st);
+  }
 }
 
 void release_all() {
@@ -452,6 +515,8 @@ void release_all() {
     (void)hipFree(kv.second.phase_ai);
     (void)hipFree(kv.second.phase_af);
     (void)hipFree(kv.second.wr);
+    (void)hipFree(kv.second.wr_even);
+    (void)hipFree(kv.second.pole);
     (void)hipFree(kv.second.nbase);
   }
   g_plans.clear();
