@@ -232,10 +232,11 @@ __global__ __launch_bounds__(256) void k_contract_inv(Geo g, const double2 *__re
     for (int a = 0; a < kXG; ++a) cv[a] = fac * buf[cvo + 2 * a * St::kRegD];
 #pragma unroll
     for (int t = 0; t < kNT; ++t) dy[t] = buf[1024 + (4 * w + t) * 64 + lane];
+    // tiles with m' > l or m' >= L were copied from the zero page: the
+    // matrix instructions run unconditionally, which keeps the accumulators
+    // in place (a branch per tile made the compiler move them every chunk)
 #pragma unroll
     for (int t = 0; t < kNT; ++t) {
-      const int mp0 = mpb + 16 * t;
-      if (mp0 >= g.L || mp0 > l0 + 3) continue;
 #pragma unroll
       for (int a = 0; a < kXG; ++a) acc[a][t] = mma(cv[a], buf[(4 * a + t) * 64 + lane] * dy[t], acc[a][t]);
     }
@@ -329,13 +330,15 @@ __global__ __launch_bounds__(256) void k_contract_fwd(Geo g, const double *__res
   const int e2 = 2 * (lane & 31);
   const int th = lane >> 5;
   const double *dsrc[4];  // DX (j = 0, 1), DY (j = 2, 3): chunk m0 / 4 at + 16 m0
+  int dlast[4];           // last m0 of the tile (m' <= l0 + 15), -1 if none
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int t = 2 * (j & 1) + th;
     const int l0 = lo + 16 * t;
     const int c = j < 2 ? x0 + w : y;
     const bool ok = (j < 2 ? c < g.L : yv) && l0 < g.L && c <= l0 + 15;
-    dsrc[j] = ok ? tf + offf[c] - 256 * tri(c >> 4) + 256 * tri(l0 >> 4) + e2 : nullptr;
+    dsrc[j] = ok ? tf + offf[c] - 256 * tri(c >> 4) + 256 * tri(l0 >> 4) + e2 : tf;
+    dlast[j] = ok ? l0 + 15 : -1;
   }
   const double *csrc[St::kCvI];
 #pragma unroll
@@ -354,7 +357,7 @@ __global__ __launch_bounds__(256) void k_contract_fwd(Geo g, const double *__res
   auto issue = [&](int m0, double *buf) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const double *src = dsrc[j] ? dsrc[j] + 16 * m0 : zero + e2;
+      const double *src = m0 <= dlast[j] ? dsrc[j] + 16 * m0 : zero + e2;
       const int row = j < 2 ? 4 * w + 2 * j : 1024 / 64 + 4 * w + 2 * (j - 2);
       dma16(src, buf + 64 * row);
     }
@@ -392,10 +395,10 @@ __global__ __launch_bounds__(256) void k_contract_fwd(Geo g, const double *__res
     for (int a = 0; a < kXG; ++a) cv[a] = used ? buf[cvo + 2 * a * St::kRegD] : 0.0;
 #pragma unroll
     for (int t = 0; t < kNT; ++t) dy[t] = buf[1024 + (4 * w + t) * 64 + lane];
+    // tiles with m' > l or l >= L were copied from the zero page: the matrix
+    // instructions run unconditionally (see the inverse)
 #pragma unroll
     for (int t = 0; t < kNT; ++t) {
-      const int l0 = lo + 16 * t;
-      if (l0 >= g.L || m0 > l0 + 15) continue;
 #pragma unroll
       for (int a = 0; a < kXG; ++a) acc[a][t] = mma(cv[a], buf[(4 * a + t) * 64 + lane] * dy[t], acc[a][t]);
     }

@@ -187,10 +187,12 @@ __device__ __forceinline__ void fft_G(double2 *v, double *T, const double2 *tw1,
 
 // DFT of length n <= 512 (Bluestein with inner length 1024) with sign DIR,
 // unnormalised, layout N in and out; entries at or beyond n must be zero on
-// entry and are undefined on exit.  tab: the LDS tables of reg_stage_blue<DIR>.
-template <int DIR>
-__device__ __forceinline__ void blue_reg(double2 *v, int n, const double2 *tab, double *T, int lane) {
-  const double2 *tw = tab, *bh = tab + kRegTw, *chirp = tab + kRegTw + 1024;
+// entry and are undefined on exit.  tw: the twiddles (LDS), chirp: n
+// entries; bh: the kernel spectrum, in register order (the LDS copy of
+// reg_stage_blue, BHREG = 1) or in natural order (FftDesc's, BHREG = 0).
+template <int DIR, int BHREG>
+__device__ __forceinline__ void blue_reg(double2 *v, int n, const double2 *tw, const double2 *bh,
+                                         const double2 *chirp, double *T, int lane) {
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int k = lane + 64 * q;
@@ -199,7 +201,7 @@ __device__ __forceinline__ void blue_reg(double2 *v, int n, const double2 *tab, 
   phase_fence();
   fft_F<-1>(v, T, tw, tw + 1024, lane);
 #pragma unroll
-  for (int c = 0; c < 16; ++c) v[c] = cmul(v[c], bh[c * 64 + lane]);
+  for (int c = 0; c < 16; ++c) v[c] = cmul(v[c], bh[BHREG ? c * 64 + lane : o_index(lane, c)]);
   phase_fence();
   fft_G<+1>(v, T, tw, tw + 1024, lane);
   phase_fence();
@@ -209,6 +211,12 @@ __device__ __forceinline__ void blue_reg(double2 *v, int n, const double2 *tab, 
     const int k = lane + 64 * q;
     if (k < n) v[q] = cmul(v[q], cscale(tw_load<DIR>(chirp, k), inv));
   }
+}
+
+// the same with every table in LDS (reg_stage_blue<DIR> layout)
+template <int DIR>
+__device__ __forceinline__ void blue_reg(double2 *v, int n, const double2 *tab, double *T, int lane) {
+  blue_reg<DIR, 1>(v, n, tab, tab + kRegTw, tab + kRegTw + 1024, T, lane);
 }
 
 }  // namespace so3g
