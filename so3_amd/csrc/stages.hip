@@ -208,26 +208,28 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))
   // src and dst are the same array in the in-place modes
   static_assert(kRegWaves == 4, "gamma tile of 4 columns");
   constexpr int DIR = (MODE == kGxF || MODE == kGinv) ? +1 : -1;
+  constexpr int kPF = 8;  // tile entries per thread: nG * 4 <= 2048
   extern __shared__ double2 lds[];
   const int wv = threadIdx.x >> 6;
   double2 *tab = lds, *tile = lds + kRegBlueTab;
   double *T = (double *)(lds + kRegBlueTab) + wv * kRegT;
   reg_stage_blue<DIR>(tab, fd);
-  __syncthreads();
   const int64_t S = (int64_t)g.nB * g.nA;
   const int64_t groups = (S + kRegWaves - 1) / kRegWaves;
   const int nG = 2 * g.N - 1;
   const int nout = (MODE == kFxG && g.real) ? g.N : nG;
   const double2 *X2s = (const double2 *)src;
   double2 *X2d = (double2 *)dst;
-  for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
-    const int lane = loop_lane();
-    const int64_t p0 = t * kRegWaves;
-    for (int idx = threadIdx.x; idx < nG * 4; idx += kThreads) {
+  // this thread's tile entries of group t (idx = threadIdx.x + i kThreads:
+  // n slot idx / 4, column idx % 4), loaded into registers one group ahead
+  auto fetch = [&](int64_t t, double2 (&P)[kPF]) {
+#pragma unroll
+    for (int i = 0; i < kPF; ++i) {
+      const int idx = threadIdx.x + i * kThreads;
       const int k = idx >> 2, c = idx & 3;
-      const int64_t p = p0 + c;
+      const int64_t p = t * kRegWaves + c;
       double2 x = make_double2(0.0, 0.0);
-      if (p < S) {
+      if (t < groups && k < nG && p < S) {
         const int b = (int)(p / g.nA), a = (int)(p % g.nA);
         if constexpr (MODE == kFxG) {
           x = g.real ? make_double2(((const double *)src)[k * S + p], 0.0) : ((const double2 *)src)[k * S + p];
@@ -239,7 +241,20 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))
           x = X2s[x2_row(g, k, b) * g.ldA + a];
         }
       }
-      tile[gtile(k, c)] = x;
+      P[i] = x;
+    }
+  };
+  double2 P[kPF];
+  int64_t t = xcd_tile(blockIdx.x, gridDim.x);
+  fetch(t, P);
+  __syncthreads();  // tables staged
+  for (; t < groups; t += gridDim.x) {
+    const int lane = loop_lane();
+    const int64_t p0 = t * kRegWaves;
+#pragma unroll
+    for (int i = 0; i < kPF; ++i) {
+      const int idx = threadIdx.x + i * kThreads;
+      if ((idx >> 2) < nG) tile[gtile(idx >> 2, idx & 3)] = P[i];
     }
     __syncthreads();
     double2 v[16];
@@ -249,6 +264,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))
       v[q] = k < nG ? tile[gtile(k, wv)] : make_double2(0.0, 0.0);
     }
     __syncthreads();  // the tile is read before the transposes reuse it
+    fetch(t + gridDim.x, P);  // the next group's loads run during the DFT
     blue_reg<DIR>(v, nG, tab, T, lane);
     __syncthreads();
 #pragma unroll
@@ -270,7 +286,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))
         X2d[x2_row(g, k, b) * g.ldA + a] = cscale(x, scale);
       }
     }
-    __syncthreads();  // the tile is read out before the next group's loads
+    __syncthreads();  // the tile is read out before the next group's entries
   }
 }
 
@@ -679,35 +695,54 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) 
   double2 *tab = lds, *tile = lds + kRegBlueTab;
   double *T = (double *)(lds + kRegBlueTab) + wv * kRegT;
   reg_stage_blue<+1>(tab, fd);
-  __syncthreads();
   const int L = g.L, Pb = g.Pb;
   const int mlo = -(L - 1) - g.ss;
   const int chunks = (g.nA + C - 1) / C;
   const int64_t groups = (int64_t)g.nG * chunks;
-  for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
+  // the X1 entries |m'| of this lane (k = lane + 64 q < Pb <= 512) for the
+  // pair of group t, loaded into registers one group ahead
+  constexpr int kQ = 8;
+  auto fetch = [&](int64_t t, double2 (&PA)[kQ], double2 (&PB)[kQ]) {
+    const int ln = threadIdx.x & 63;
+    const int nslot = (int)(t / chunks);
+    const int mA = mlo + (int)(t % chunks) * C + 2 * wv, mB = mA + 1;
+    const bool act = t < groups && nbase[n_of_slot(g, nslot) + g.N - 1] != kNoBlock;
+    const bool hA = act && mA >= -(L - 1) && mA <= L - 1;
+    const bool hB = act && mB >= -(L - 1) && mB <= L - 1;
+    const double2 *rA = X1 + ((int64_t)nslot * g.nA + (hA ? m_slot_of(g, mA) : 0)) * L;
+    const double2 *rB = X1 + ((int64_t)nslot * g.nA + (hB ? m_slot_of(g, mB) : 0)) * L;
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) {
+      const int k = ln + 64 * q;
+      const int mp = k <= L - 1 ? k : k - Pb;
+      const bool ok = k < Pb && mp >= -(L - 1);
+      const int am = mp < 0 ? -mp : mp;
+      PA[q] = ok && hA ? rA[am] : make_double2(0.0, 0.0);
+      PB[q] = ok && hB ? rB[am] : make_double2(0.0, 0.0);
+    }
+  };
+  double2 PA[kQ], PB[kQ];
+  int64_t t = xcd_tile(blockIdx.x, gridDim.x);
+  fetch(t, PA, PB);
+  __syncthreads();  // tables staged
+  for (; t < groups; t += gridDim.x) {
     const int lane = loop_lane();
     const int nslot = (int)(t / chunks);
     const int m0 = mlo + (int)(t % chunks) * C;
     const int n = n_of_slot(g, nslot);
-    const bool active = nbase[n + g.N - 1] != kNoBlock;
-    const int mA = m0 + 2 * wv, mB = mA + 1;
-    const bool hA = active && mA >= -(L - 1) && mA <= L - 1;
-    const bool hB = active && mB >= -(L - 1) && mB <= L - 1;
+    const int mA = m0 + 2 * wv;
     const bool negA = ((mA + n) & 1) != 0;  // s_A = -1 (then s_B = +1)
     const double sA = negA ? -1.0 : 1.0;
-    const double2 *rA = X1 + ((int64_t)nslot * g.nA + (hA ? m_slot_of(g, mA) : 0)) * L;
-    const double2 *rB = X1 + ((int64_t)nslot * g.nA + (hB ? m_slot_of(g, mB) : 0)) * L;
     double2 v[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int k = lane + 64 * q;
       double2 x = make_double2(0.0, 0.0);
-      if (k < Pb) {
+      if (q < kQ && k < Pb) {
         const int mp = k <= L - 1 ? k : k - Pb;
         if (mp >= -(L - 1)) {
-          const int am = mp < 0 ? -mp : mp;
-          double2 a = hA ? rA[am] : make_double2(0.0, 0.0);
-          double2 b = hB ? rB[am] : make_double2(0.0, 0.0);
+          double2 a = PA[q < kQ ? q : 0];
+          double2 b = PB[q < kQ ? q : 0];
           if (mp < 0) {
             if (negA) a = make_double2(-a.x, -a.y);
             else b = make_double2(-b.x, -b.y);
@@ -721,6 +756,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) 
       }
       v[q] = x;
     }
+    fetch(t + gridDim.x, PA, PB);  // the next group's loads run during the DFT
     blue_reg<+1>(v, Pb, tab, T, lane);
     // Z(b') for b < nB through the transpose buffer, real then imaginary parts
     double zr[kQB];
@@ -789,23 +825,44 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) 
   double2 *tw = lds, *tile = lds + kRegBlueTab;
   double *T = (double *)(lds + kRegBlueTab) + wv * kRegT;
   reg_stage_blue<-1>(tw, fb);
-  __syncthreads();
   const int L = g.L, Pb = g.Pb;
   const int mlo = -(L - 1) - g.ss;
   const int chunks = (g.nA + C - 1) / C;
   const int64_t groups = (int64_t)g.nG * chunks;
   const int C2 = 2 * g.NS;
-  for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
+  // this thread's tile entries of group t (idx = threadIdx.x + i 64 W:
+  // b = idx / C, column idx % C), loaded into registers one group ahead
+  constexpr int kPF = (257 * C + 64 * W - 1) / (64 * W);  // nB <= 257
+  auto fetch = [&](int64_t t, double2 (&P)[kPF]) {
+    const int nslot = (int)(t / chunks);
+    const int m0 = mlo + (int)(t % chunks) * C;
+    const bool act = t < groups && nbase[n_of_slot(g, nslot) + g.N - 1] != kNoBlock;
+#pragma unroll
+    for (int i = 0; i < kPF; ++i) {
+      const int idx = threadIdx.x + i * (64 * W);
+      const int b = idx / C, c = idx - b * C;
+      const int m = m0 + c;
+      P[i] = act && b < g.nB && m >= -(L - 1) && m <= L - 1 ? X2[x2_row(g, nslot, b) * g.ldA + m_slot_of(g, m)]
+                                                           : make_double2(0.0, 0.0);
+    }
+  };
+  double2 P[kPF];
+  int64_t t = xcd_tile(blockIdx.x, gridDim.x);
+  fetch(t, P);
+  __syncthreads();  // tables staged
+  for (; t < groups; t += gridDim.x) {
     const int lane = loop_lane();
     const int nslot = (int)(t / chunks);
     const int m0 = mlo + (int)(t % chunks) * C;
     const int n = n_of_slot(g, nslot);
-    if (nbase[n + g.N - 1] == kNoBlock) continue;  // block not transformed (whole workgroup)
-    for (int idx = threadIdx.x; idx < g.nB * C; idx += (64 * W)) {
-      const int b = idx / C, c = idx - b * C;
-      const int m = m0 + c;
-      tile[b * LD + c] = (m >= -(L - 1) && m <= L - 1) ? X2[x2_row(g, nslot, b) * g.ldA + m_slot_of(g, m)]
-                                                       : make_double2(0.0, 0.0);
+    if (nbase[n + g.N - 1] == kNoBlock) {  // block not transformed (whole workgroup)
+      fetch(t + gridDim.x, P);
+      continue;
+    }
+#pragma unroll
+    for (int i = 0; i < kPF; ++i) {
+      const int idx = threadIdx.x + i * (64 * W);
+      if (idx < g.nB * C) tile[(idx / C) * LD + idx % C] = P[i];
     }
     __syncthreads();
     const int mA = m0 + 2 * wv, mB = mA + 1;
@@ -839,6 +896,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) 
       v[q] = x;
     }
     __syncthreads();  // the tile is read before the transposes reuse it
+    fetch(t + gridDim.x, P);  // the next group's loads run during the DFTs
     blue_reg<-1>(v, Pb, tw, T, lane);
     // F~(m'') times the phase at m'' mod Pw, |m''| < L, and F~(+-1)
     double2 z1, zm1;
