@@ -27,9 +27,9 @@ namespace so3g {
 constexpr int kRegLd = 68;            // row stride (doubles) of the transpose buffer
 constexpr int kRegT = 16 * kRegLd;    // doubles per wavefront (8704 bytes)
 constexpr int kRegTw = 1024 + 64;     // double2 of the twiddle tables tw1 | tw2
-// double2 of the Bluestein tables: tw1 | tw2 | kernel spectrum (1024, in
-// register order: entry c * 64 + lane is the value at o_index(lane, c)) |
-// chirp (n <= 512 entries: a length-1024 Bluestein plan has n <= 512)
+// double2 of the Bluestein tables: tw1 | tw2 | kernel spectrum / 1024 (1024,
+// in register order: entry c * 64 + lane is the value at o_index(lane, c)) |
+// chirp (n <= 512 entries: a length-1024 Bluestein plan has 257 <= n <= 512)
 constexpr int kRegBlueTab = kRegTw + 1024 + 512;
 
 __device__ __forceinline__ int bitrev2(int a) { return ((a & 1) << 1) | (a >> 1); }
@@ -93,7 +93,9 @@ template <int DIR>
 __device__ __forceinline__ void reg_stage_blue(double2 *dst, const FftDesc &d) {
   const double2 *bh = DIR < 0 ? d.bhat_f : d.bhat_b;
   for (int i = threadIdx.x; i < kRegTw; i += blockDim.x) dst[i] = d.tws[i];
-  for (int i = threadIdx.x; i < 1024; i += blockDim.x) dst[kRegTw + i] = bh[o_index(i & 63, i >> 6)];
+  // 1/1024 of the convolution is exact in the spectrum (a power of two)
+  for (int i = threadIdx.x; i < 1024; i += blockDim.x)
+    dst[kRegTw + i] = cscale(bh[o_index(i & 63, i >> 6)], 1.0 / 1024.0);
   for (int i = threadIdx.x; i < d.n; i += blockDim.x) dst[kRegTw + 1024 + i] = d.chirp[i];
 }
 
@@ -185,38 +187,42 @@ __device__ __forceinline__ void fft_G(double2 *v, double *T, const double2 *tw1,
   dft16n<DIR>(v);
 }
 
-// DFT of length n <= 512 (Bluestein with inner length 1024) with sign DIR,
-// unnormalised, layout N in and out; entries at or beyond n must be zero on
-// entry and are undefined on exit.  tw: the twiddles (LDS), chirp: n
-// entries; bh: the kernel spectrum, in register order (the LDS copy of
-// reg_stage_blue, BHREG = 1) or in natural order (FftDesc's, BHREG = 0).
-template <int DIR, int BHREG>
+// DFT of length n (Bluestein with inner length 1024, so 257 <= n <= 512)
+// with sign DIR, unnormalised, layout N in and out; entries at or beyond n
+// must be zero on entry and are undefined on exit.  tw: the twiddles (LDS),
+// chirp: n entries; bh: the kernel spectrum, in register order and divided
+// by 1024 (the LDS copy of reg_stage_blue, BHREG = 1) or in natural order
+// (FftDesc's, BHREG = 0).  Slots q < 4 (k < 256) are always inside the
+// sequence and slots q >= 8 (k >= 512) never, so only q = 4..7 test k < n;
+// LOWREG tests every slot, which serialises the chirp loads and keeps fewer
+// registers live (for kernels near the register limit).
+template <int DIR, int BHREG, int LOWREG = 0>
 __device__ __forceinline__ void blue_reg(double2 *v, int n, const double2 *tw, const double2 *bh,
                                          const double2 *chirp, double *T, int lane) {
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
+  for (int q = 0; q < (LOWREG ? 16 : 8); ++q) {
     const int k = lane + 64 * q;
-    if (k < n) v[q] = cmul(v[q], tw_load<DIR>(chirp, k));
+    if ((!LOWREG && q < 4) || k < n) v[q] = cmul(v[q], tw_load<DIR>(chirp, k));
   }
   phase_fence();
   fft_F<-1>(v, T, tw, tw + 1024, lane);
 #pragma unroll
-  for (int c = 0; c < 16; ++c) v[c] = cmul(v[c], bh[BHREG ? c * 64 + lane : o_index(lane, c)]);
+  for (int c = 0; c < 16; ++c)
+    v[c] = BHREG ? cmul(v[c], bh[c * 64 + lane]) : cmul(v[c], cscale(bh[o_index(lane, c)], 1.0 / 1024.0));
   phase_fence();
   fft_G<+1>(v, T, tw, tw + 1024, lane);
   phase_fence();
-  constexpr double inv = 1.0 / 1024.0;
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
+  for (int q = 0; q < (LOWREG ? 16 : 8); ++q) {
     const int k = lane + 64 * q;
-    if (k < n) v[q] = cmul(v[q], cscale(tw_load<DIR>(chirp, k), inv));
+    if ((!LOWREG && q < 4) || k < n) v[q] = cmul(v[q], tw_load<DIR>(chirp, k));
   }
 }
 
 // the same with every table in LDS (reg_stage_blue<DIR> layout)
-template <int DIR>
+template <int DIR, int LOWREG = 0>
 __device__ __forceinline__ void blue_reg(double2 *v, int n, const double2 *tab, double *T, int lane) {
-  blue_reg<DIR, 1>(v, n, tab, tab + kRegTw, tab + kRegTw + 1024, T, lane);
+  blue_reg<DIR, 1, LOWREG>(v, n, tab, tab + kRegTw, tab + kRegTw + 1024, T, lane);
 }
 
 }  // namespace so3g

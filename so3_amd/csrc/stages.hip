@@ -897,7 +897,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) 
     }
     __syncthreads();  // the tile is read before the transposes reuse it
     fetch(t + gridDim.x, P);  // the next group's loads run during the DFTs
-    blue_reg<-1>(v, Pb, tw, T, lane);
+    blue_reg<-1, 1>(v, Pb, tw, T, lane);
     // F~(m'') times the phase at m'' mod Pw, |m''| < L, and F~(+-1)
     double2 z1, zm1;
     {
