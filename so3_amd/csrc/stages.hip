@@ -214,31 +214,41 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))
   double2 *tab = lds, *tile = lds + kRegBlueTab;
   double *T = (double *)(lds + kRegBlueTab) + wv * kRegT;
   reg_stage_blue<DIR>(tab, fd);
-  const int64_t S = (int64_t)g.nB * g.nA;
+  const int S = g.nB * g.nA;  // columns; f has S entries per g
   const int64_t groups = (S + kRegWaves - 1) / kRegWaves;
   const int nG = 2 * g.N - 1;
   const int nout = (MODE == kFxG && g.real) ? g.N : nG;
   const double2 *X2s = (const double2 *)src;
   double2 *X2d = (double2 *)dst;
-  // this thread's tile entries of group t (idx = threadIdx.x + i kThreads:
-  // n slot idx / 4, column idx % 4), loaded into registers one group ahead
+  // A thread always handles column c = threadIdx.x % 4 of a group and the n
+  // slots k = threadIdx.x / 4 + 64 i.  Column p of group t: X2 entry of slot
+  // k at x0 + k xs (x2_row), f entry at p + k S.
+  const int c = threadIdx.x & 3, k0 = threadIdx.x >> 2;
+  auto column = [&](int64_t t, int &p, int64_t &x0, int64_t &xs) {
+    p = (int)(t * kRegWaves) + c;
+    const int b = p / g.nA, a = p - b * g.nA;
+    x0 = p < S ? x2_row(g, 0, b) * g.ldA + a : 0;
+    xs = (int64_t)x2_w(g, b < g.nB ? b : 0) * g.ldA;
+  };
   auto fetch = [&](int64_t t, double2 (&P)[kPF]) {
+    int p;
+    int64_t x0, xs;
+    column(t, p, x0, xs);
+    const bool ok = t < groups && p < S;
 #pragma unroll
     for (int i = 0; i < kPF; ++i) {
-      const int idx = threadIdx.x + i * kThreads;
-      const int k = idx >> 2, c = idx & 3;
-      const int64_t p = t * kRegWaves + c;
+      const int k = k0 + 64 * i;
       double2 x = make_double2(0.0, 0.0);
-      if (t < groups && k < nG && p < S) {
-        const int b = (int)(p / g.nA), a = (int)(p % g.nA);
+      if (ok && k < nG) {
         if constexpr (MODE == kFxG) {
-          x = g.real ? make_double2(((const double *)src)[k * S + p], 0.0) : ((const double2 *)src)[k * S + p];
+          x = g.real ? make_double2(((const double *)src)[(int64_t)k * S + p], 0.0)
+                     : ((const double2 *)src)[(int64_t)k * S + p];
         } else if (MODE == kGxF && g.real) {
-          if (k == 0) x = make_double2(X2s[x2_row(g, 0, b) * g.ldA + a].x, 0.0);
-          else if (k < g.N) x = X2s[x2_row(g, k, b) * g.ldA + a];
-          else x = cconj(X2s[x2_row(g, nG - k, b) * g.ldA + a]);
+          if (k == 0) x = make_double2(X2s[x0].x, 0.0);
+          else if (k < g.N) x = X2s[x0 + k * xs];
+          else x = cconj(X2s[x0 + (nG - k) * xs]);
         } else {
-          x = X2s[x2_row(g, k, b) * g.ldA + a];
+          x = X2s[x0 + k * xs];
         }
       }
       P[i] = x;
@@ -250,11 +260,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))
   __syncthreads();  // tables staged
   for (; t < groups; t += gridDim.x) {
     const int lane = loop_lane();
-    const int64_t p0 = t * kRegWaves;
 #pragma unroll
     for (int i = 0; i < kPF; ++i) {
-      const int idx = threadIdx.x + i * kThreads;
-      if ((idx >> 2) < nG) tile[gtile(idx >> 2, idx & 3)] = P[i];
+      const int k = k0 + 64 * i;
+      if (k < nG) tile[gtile(k, c)] = P[i];
     }
     __syncthreads();
     double2 v[16];
@@ -273,17 +282,23 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))
       if (k < nout) tile[gtile(k, wv)] = v[q];
     }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < nout * 4; idx += kThreads) {
-      const int k = idx >> 2, c = idx & 3;
-      const int64_t p = p0 + c;
-      if (p >= S) continue;
-      const double2 x = tile[gtile(k, c)];
-      if constexpr (MODE == kGxF) {
-        if (g.real) ((double *)dst)[k * S + p] = x.x * scale;
-        else ((double2 *)dst)[k * S + p] = cscale(x, scale);
-      } else {
-        const int b = (int)(p / g.nA), a = (int)(p % g.nA);
-        X2d[x2_row(g, k, b) * g.ldA + a] = cscale(x, scale);
+    {
+      int p;
+      int64_t x0, xs;
+      column(t, p, x0, xs);
+      if (p < S) {
+#pragma unroll
+        for (int i = 0; i < kPF; ++i) {
+          const int k = k0 + 64 * i;
+          if (k >= nout) continue;
+          const double2 x = tile[gtile(k, c)];
+          if constexpr (MODE == kGxF) {
+            if (g.real) ((double *)dst)[(int64_t)k * S + p] = x.x * scale;
+            else ((double2 *)dst)[(int64_t)k * S + p] = cscale(x, scale);
+          } else {
+            X2d[x0 + k * xs] = cscale(x, scale);
+          }
+        }
       }
     }
     __syncthreads();  // the tile is read out before the next group's entries
