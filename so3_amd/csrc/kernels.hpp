@@ -45,13 +45,14 @@ struct NBase {
 // few 2 MB pages rather than 64 of them), a beta sequence (n slot, m) steps
 // one row per b inside a block, and alpha sees nG nB contiguous rows.
 constexpr int kX2Blk = 8;
+// (row indices stay below nG nB < 2^31: 32-bit arithmetic, b >= 0)
 __host__ __device__ inline int x2_w(const Geo &g, int b) {
-  const int r = g.nB - (b / kX2Blk) * kX2Blk;
+  const int r = g.nB - (int)((unsigned)b / kX2Blk) * kX2Blk;
   return r < kX2Blk ? r : kX2Blk;
 }
 __host__ __device__ inline int64_t x2_row(const Geo &g, int nslot, int b) {
-  const int h = b / kX2Blk;
-  return (int64_t)h * kX2Blk * g.nG + (int64_t)nslot * x2_w(g, b) + (b - h * kX2Blk);
+  const int h = (int)((unsigned)b / kX2Blk);
+  return (int64_t)(h * kX2Blk * g.nG + nslot * x2_w(g, b) + (int)((unsigned)b % kX2Blk));
 }
 void launch_contract_inv(const Geo &g, const double2 *flmn, NBase nb, const DlTables &dl, double2 *X1,
                          hipStream_t st);

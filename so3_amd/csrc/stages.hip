@@ -812,10 +812,14 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) 
       }
     }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < g.nB * C; idx += (64 * W)) {
-      const int b = idx / C, c = idx - b * C;
-      const int m = m0 + c;
-      if (m <= L - 1) X2[x2_row(g, nslot, b) * g.ldA + m_slot_of(g, m)] = tile[b * LD + c];
+    {
+      // a thread keeps column c = threadIdx.x % C, rows b = threadIdx.x / C + i 64 W / C
+      const int c = threadIdx.x % C, m = m0 + c;
+      if (m <= L - 1) {
+        double2 *col = X2 + m_slot_of(g, m);
+        for (int b = threadIdx.x / C; b < g.nB; b += 64 * W / C)
+          col[x2_row(g, nslot, b) * g.ldA] = tile[b * LD + c];
+      }
     }
     __syncthreads();  // the tile is read out before the next group's transposes
   }
@@ -850,15 +854,14 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) 
   constexpr int kPF = (257 * C + 64 * W - 1) / (64 * W);  // nB <= 257
   auto fetch = [&](int64_t t, double2 (&P)[kPF]) {
     const int nslot = (int)(t / chunks);
-    const int m0 = mlo + (int)(t % chunks) * C;
-    const bool act = t < groups && nbase[n_of_slot(g, nslot) + g.N - 1] != kNoBlock;
+    const int m = mlo + (int)(t % chunks) * C + (int)threadIdx.x % C;  // this thread's column
+    const bool act = t < groups && nbase[n_of_slot(g, nslot) + g.N - 1] != kNoBlock &&
+                     m >= -(L - 1) && m <= L - 1;
+    const double2 *col = X2 + (act ? m_slot_of(g, m) : 0);
 #pragma unroll
     for (int i = 0; i < kPF; ++i) {
-      const int idx = threadIdx.x + i * (64 * W);
-      const int b = idx / C, c = idx - b * C;
-      const int m = m0 + c;
-      P[i] = act && b < g.nB && m >= -(L - 1) && m <= L - 1 ? X2[x2_row(g, nslot, b) * g.ldA + m_slot_of(g, m)]
-                                                           : make_double2(0.0, 0.0);
+      const int b = (int)threadIdx.x / C + i * (64 * W / C);
+      P[i] = act && b < g.nB ? col[x2_row(g, nslot, b) * g.ldA] : make_double2(0.0, 0.0);
     }
   };
   double2 P[kPF];
@@ -876,8 +879,8 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) 
     }
 #pragma unroll
     for (int i = 0; i < kPF; ++i) {
-      const int idx = threadIdx.x + i * (64 * W);
-      if (idx < g.nB * C) tile[(idx / C) * LD + idx % C] = P[i];
+      const int b = (int)threadIdx.x / C + i * (64 * W / C);
+      if (b < g.nB) tile[b * LD + (int)threadIdx.x % C] = P[i];
     }
     __syncthreads();
     const int mA = m0 + 2 * wv, mB = mA + 1;
@@ -950,7 +953,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) 
     // convolution with the even weight taps
     fft_F<+1>(v, T, tw, tw + 1024, lane);
 #pragma unroll
-    for (int c = 0; c < 16; ++c) v[c] = cmul(v[c], wre[o_index(lane, c)]);
+    for (int c = 0; c < 16; ++c) v[c] = cmul(v[c], wre[c * 64 + lane]);  // register order
     fft_G<-1>(v, T, tw, tw + 1024, lane);
     constexpr double sc = 2.0 * kPi / 1024.0;
     // separate and fold: m' = lane + 64 j

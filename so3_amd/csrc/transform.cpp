@@ -166,6 +166,19 @@ Plan build_plan(const so3_parameters_t *p, Method method) {
       wre[r] = make_double2((double)ev.real(), (double)ev.imag());
     }
     P.wr = upload(wr);
+    // wr_even goes in the register order of the length-1024 register FFT
+    // (entry c * 64 + lane holds index (lane >> 2) + 16 c + 256 r(lane & 3),
+    // r the 2-bit reversal), the only place it is read; other lengths leave it
+    // in natural order (unused)
+    if (Pw == 1024) {
+      std::vector<double2> reg(Pw);
+      for (int c = 0; c < 16; ++c)
+        for (int lane = 0; lane < 64; ++lane) {
+          const int a = lane & 3, r = ((a & 1) << 1) | (a >> 1);
+          reg[c * 64 + lane] = wre[(lane >> 2) + 16 * c + 256 * r];
+        }
+      wre.swap(reg);
+    }
     P.wr_even = upload(wre);
     // Pole samples (b = L-1 for MW; b = 0 and b = L for MW_SS) map to
     // themselves under the extension, so an antisymmetric sequence
