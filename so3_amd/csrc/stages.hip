@@ -98,8 +98,12 @@ __global__ __launch_bounds__(kThreads) void k_rows(FftDesc fd, int K, int ld, do
 // + i * gridDim.x until none is left (the grid is what fits on the device at
 // once, reg_grid).
 constexpr int kRegWaves = kThreads / 64;
-// wavefronts per workgroup of the beta kernels, measured best at L = 256
+// wavefronts per workgroup of the beta kernels, measured best at L = 256:
+// 4 for the tiled ones (two workgroups per CU), 12 for the forward pair
+// kernel, whose wavefronts never synchronise (one workgroup per CU sharing
+// one copy of the tables, three wavefronts per SIMD)
 constexpr int kBetaW = 4;
+constexpr int kBetaFW = 12;
 
 // Workgroups go round-robin over the 8 XCDs, each with its own L2: block p
 // runs on XCD p % 8.  Renumbering block p to (p % 8) q + p / 8, q = n / 8,
@@ -695,8 +699,10 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) 
 //   A = (Z(b) + s_A Z(b')) / 2,   B = (Z(b) - s_A Z(b')) / (2 i).
 // Consecutive m have opposite signs, so a wavefront takes the pair
 // (m, m + 1); a group is 2W consecutive m from m_lo = -(L-1) - ss (the
-// MW_SS Nyquist column, zero, first) and leaves or enters through an LDS
-// tile of nB x (2W + 1) entries (128-byte HBM runs for W = 4).
+// MW_SS Nyquist column, zero, first).  The inverse kernel writes its group
+// through an LDS tile of nB x (2W + 1) entries (128-byte HBM runs for
+// W = 4); the forward kernel reads its samples straight from X2 (the pairs
+// of a workgroup share each line in L1) and has no workgroup barrier.
 __device__ __forceinline__ int m_slot_of(const Geo &g, int m) { return m >= 0 ? m : m + g.nA; }
 
 template <int W>
@@ -833,73 +839,52 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) 
 // antisymmetric member break its antisymmetry; they are left out of the
 // pair and added back through their folded responses (pole table).
 template <int W>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_fwd_beta_pair(
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_fwd_beta_pair(
     Geo g, FftDesc fb, const double2 *__restrict__ X2, double2 *__restrict__ Gp,
     const int64_t *__restrict__ nbase, const double2 *__restrict__ phase, const double2 *__restrict__ wre,
     const double2 *__restrict__ pole) {
-  constexpr int C = 2 * W, LD = C + 1;
+  constexpr int C = 2 * W;
   constexpr double kPi = 3.141592653589793238462643383279502884;
   extern __shared__ double2 lds[];
   const int wv = threadIdx.x >> 6;
-  double2 *tw = lds, *tile = lds + kRegBlueTab;
+  double2 *tw = lds;
   double *T = (double *)(lds + kRegBlueTab) + wv * kRegT;
   reg_stage_blue<-1>(tw, fb);
+  __syncthreads();  // tables staged; no other barrier: the wavefronts run independently
   const int L = g.L, Pb = g.Pb;
   const int mlo = -(L - 1) - g.ss;
   const int chunks = (g.nA + C - 1) / C;
   const int64_t groups = (int64_t)g.nG * chunks;
   const int C2 = 2 * g.NS;
-  // this thread's tile entries of group t (idx = threadIdx.x + i 64 W:
-  // b = idx / C, column idx % C), loaded into registers one group ahead
-  constexpr int kPF = (257 * C + 64 * W - 1) / (64 * W);  // nB <= 257
-  auto fetch = [&](int64_t t, double2 (&P)[kPF]) {
-    const int nslot = (int)(t / chunks);
-    const int m = mlo + (int)(t % chunks) * C + (int)threadIdx.x % C;  // this thread's column
-    const bool act = t < groups && nbase[n_of_slot(g, nslot) + g.N - 1] != kNoBlock &&
-                     m >= -(L - 1) && m <= L - 1;
-    const double2 *col = X2 + (act ? m_slot_of(g, m) : 0);
-#pragma unroll
-    for (int i = 0; i < kPF; ++i) {
-      const int b = (int)threadIdx.x / C + i * (64 * W / C);
-      P[i] = act && b < g.nB ? col[x2_row(g, nslot, b) * g.ldA] : make_double2(0.0, 0.0);
-    }
-  };
-  double2 P[kPF];
-  int64_t t = xcd_tile(blockIdx.x, gridDim.x);
-  fetch(t, P);
-  __syncthreads();  // tables staged
-  for (; t < groups; t += gridDim.x) {
+  for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
     const int lane = loop_lane();
     const int nslot = (int)(t / chunks);
     const int m0 = mlo + (int)(t % chunks) * C;
     const int n = n_of_slot(g, nslot);
-    if (nbase[n + g.N - 1] == kNoBlock) {  // block not transformed (whole workgroup)
-      fetch(t + gridDim.x, P);
-      continue;
-    }
-#pragma unroll
-    for (int i = 0; i < kPF; ++i) {
-      const int b = (int)threadIdx.x / C + i * (64 * W / C);
-      if (b < g.nB) tile[b * LD + (int)threadIdx.x % C] = P[i];
-    }
-    __syncthreads();
+    if (nbase[n + g.N - 1] == kNoBlock) continue;  // block not transformed
     const int mA = m0 + 2 * wv, mB = mA + 1;
     const bool negA = ((mA + n) & 1) != 0;
     const double sA = negA ? -1.0 : 1.0;
-    // pole samples of the antisymmetric member
-    const int cx = negA ? 2 * wv : 2 * wv + 1;
-    const double2 p0 = tile[(g.ss ? 0 : L - 1) * LD + cx];
-    const double2 p1 = g.ss ? tile[L * LD + cx] : make_double2(0.0, 0.0);
+    // the beta samples of the pair straight from X2: the pairs of a
+    // workgroup share every 128-byte line, so most loads hit in L1
+    const bool hA = mA >= -(L - 1) && mA <= L - 1, hB = mB >= -(L - 1) && mB <= L - 1;
+    const double2 *colA = X2 + (hA ? m_slot_of(g, mA) : 0), *colB = X2 + (hB ? m_slot_of(g, mB) : 0);
+    // pole samples of the antisymmetric member (the same address in every lane)
+    const double2 *colX = negA ? colA : colB;
+    const bool hX = negA ? hA : hB;
+    const double2 p0 = hX ? colX[x2_row(g, nslot, g.ss ? 0 : L - 1) * g.ldA] : make_double2(0.0, 0.0);
+    const double2 p1 = hX && g.ss ? colX[x2_row(g, nslot, L) * g.ldA] : make_double2(0.0, 0.0);
     double2 v[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int k = lane + 64 * q;
       double2 x = make_double2(0.0, 0.0);
-      if (k < Pb) {
+      if (q < 8 && k < Pb) {  // Pb <= 512
         // MW: F(2L-2-b) for b >= L; MW_SS: F(2L-b) for b > L
         const bool direct = g.ss ? k <= L : k < L;
         const int kr = direct ? k : (g.ss ? 2 * L - k : 2 * L - 2 - k);
-        double2 a = tile[kr * LD + 2 * wv], b = tile[kr * LD + 2 * wv + 1];
+        const int64_t off = x2_row(g, nslot, kr) * g.ldA;
+        double2 a = hA ? colA[off] : make_double2(0.0, 0.0), b = hB ? colB[off] : make_double2(0.0, 0.0);
         if (!direct) {
           if (negA) a = make_double2(-a.x, -a.y);
           else b = make_double2(-b.x, -b.y);
@@ -913,8 +898,6 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) 
       }
       v[q] = x;
     }
-    __syncthreads();  // the tile is read before the transposes reuse it
-    fetch(t + gridDim.x, P);  // the next group's loads run during the DFTs
     blue_reg<-1, 1>(v, Pb, tw, T, lane);
     // F~(m'') times the phase at m'' mod Pw, |m''| < L, and F~(+-1)
     double2 z1, zm1;
@@ -1018,7 +1001,6 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) 
         }
       }
     }
-    __syncthreads();  // transposes done before the next group's tile
   }
 }
 
@@ -1336,8 +1318,8 @@ void launch_fwd_beta(const Geo &g, const double2 *X2, double2 *Gp, NBase nb, con
   if (!adjoint && wr_even && pole && Pw == 1024 && g.L <= 256 && fft_plan(g.Pb).len == 1024 &&
       fft_plan(g.Pb).blue) {
     const FftDesc fb = fft_plan(g.Pb);
-    constexpr int W = kBetaW;
-    const size_t sh = reg_shared<W>(kRegBlueTab, (int64_t)g.nB * (2 * W + 1));
+    constexpr int W = kBetaFW;
+    const size_t sh = reg_shared<W>(kRegBlueTab, 0);
     const int chunks = (g.nA + 2 * W - 1) / (2 * W);
     const unsigned grid = reg_grid(k_fwd_beta_pair<W>, 64 * W, sh, (int64_t)g.nG * chunks);
     k_fwd_beta_pair<W><<<grid, 64 * W, sh, st>>>(g, fb, X2, Gp, nb.base, phase, wr_even, pole);
