@@ -47,6 +47,49 @@ __device__ __forceinline__ d4 mma(double a, double b, d4 c) {
 
 __host__ __device__ inline int64_t tri(int64_t k) { return k * (k + 1) / 2; }
 
+// Block order.  A workgroup reads the Delta columns of its 4 x and 4 y; the
+// same columns serve every workgroup with the same x group (or y group) and
+// lane block, so those should run on one XCD at about the same time and meet
+// in its L2 (in plain grid order 2/3 of the reads missed L2).  Workgroups go
+// round-robin over the 8 XCDs (block p on XCD p % 8), an XCD holding about
+// 64 at once, so the work is cut into tiles of 8 x 8 (x group, y group) of
+// one lane block and each XCD runs whole tiles back to back: tile T on XCD
+// T % 8.  Only tiles with x >= y hold work (xg >= yg); they are numbered row
+// by row (yt, then xt >= yt), lane block (z) outermost, so neighbouring tiles
+// of one row, which carry similar work, land on different XCDs.
+struct CBlock {
+  int xg, yg, z;
+  bool ok;
+};
+__host__ __device__ inline int cblock_tiles(int nxg, int nyg) {  // per lane block
+  const int tx = (nxg + 7) / 8, ty = (nyg + 7) / 8;
+  int n = 0;
+  for (int yt = 0; yt < ty; ++yt) n += tx > yt ? tx - yt : 0;
+  return n;
+}
+__host__ __device__ inline int64_t cblock_grid(int nxg, int nyg, int nz) {
+  const int64_t nt = (int64_t)nz * cblock_tiles(nxg, nyg);
+  return 64 * ((nt + 7) / 8 * 8);
+}
+__device__ __forceinline__ CBlock cblock(int nxg, int nyg, int nz) {
+  const unsigned p = blockIdx.x, k = p >> 3;
+  const int T = (int)((p & 7) + 8 * (k >> 6)), w = (int)(k & 63);
+  const int tx = (nxg + 7) / 8, ty = (nyg + 7) / 8;
+  const int per = cblock_tiles(nxg, nyg);
+  CBlock b;
+  b.z = T / per;
+  int r = T - b.z * per, yt = 0;
+  while (yt < ty && r >= tx - yt) {  // row yt holds tiles xt = yt .. tx - 1
+    r -= tx - yt;
+    ++yt;
+  }
+  const int xt = yt + r;
+  b.xg = xt * 8 + (w & 7);
+  b.yg = yt * 8 + (w >> 3);
+  b.ok = b.z < nz && yt < ty && b.xg < nxg && b.yg < nyg;
+  return b;
+}
+
 // Operand-row map of the coefficient matrix: lane i supplies row rho = i & 15
 // of coef^T at reduction offset i >> 4.  Row rho = 4 j + q of the product
 // lands in accumulator register j of the lanes with i >> 4 == q, so
@@ -124,10 +167,11 @@ __global__ __launch_bounds__(256) void k_contract_inv(Geo g, const double2 *__re
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nmb = (g.L + 63) >> 6;
-  const int X = blockIdx.x / nmb, mb = blockIdx.x % nmb;
   const int Np = min(g.N, g.L);
-  const int x0 = X * kXG, y0 = blockIdx.y * 4;
-  if (y0 >= Np || x0 + kXG - 1 < y0 || x0 >= g.L) return;  // whole workgroup
+  const CBlock cb = cblock((g.L + kXG - 1) / kXG, (Np + 3) / 4, nmb);
+  const int mb = cb.z;
+  const int x0 = cb.xg * kXG, y0 = cb.yg * 4;
+  if (!cb.ok || y0 >= Np || x0 + kXG - 1 < y0 || x0 >= g.L) return;  // whole workgroup
   const int y = y0 + w;
   const bool yv = y < Np;
   const int mpb = mb * 64;
@@ -305,14 +349,15 @@ __global__ __launch_bounds__(256) void k_contract_fwd(Geo g, const double *__res
   __shared__ __attribute__((aligned(16))) double sm[kRing * kBufD];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  // l blocks are the slowest grid dimension, highest first: their work grows
-  // with l, so the longest workgroups start first and the tail is short
+  // l blocks are the outermost tile dimension, highest first: their work
+  // grows with l, so the longest workgroups start first and the tail is short
   const int nlb = (g.L + 63) >> 6;
-  const int X = blockIdx.x, lb = nlb - 1 - (int)blockIdx.z;
   const int Np = min(g.N, g.L);
-  const int x0 = X * kXG, y0 = blockIdx.y * 4;
+  const CBlock cb = cblock((g.L + kXG - 1) / kXG, (Np + 3) / 4, nlb);
+  const int lb = nlb - 1 - cb.z;
+  const int x0 = cb.xg * kXG, y0 = cb.yg * 4;
   const int lo = lb * 64;
-  if (y0 >= Np || x0 + kXG - 1 < y0 || x0 >= g.L || max(x0, y0) > lo + 63) return;
+  if (!cb.ok || y0 >= Np || x0 + kXG - 1 < y0 || x0 >= g.L || max(x0, y0) > lo + 63) return;
   const int y = y0 + w;
   const bool yv = y < Np;
 
@@ -447,7 +492,7 @@ void launch_contract_inv(const Geo &g, const double2 *flmn, NBase nb, const DlTa
                          hipStream_t st) {
   const int Np = std::min(g.N, g.L);
   const int nmb = (g.L + 63) / 64;
-  dim3 grid(nmb * ((g.L + kXG - 1) / kXG), (Np + 3) / 4);
+  const unsigned grid = (unsigned)cblock_grid((g.L + kXG - 1) / kXG, (Np + 3) / 4, nmb);
   if (g.NS == 2)
     k_contract_inv<2><<<grid, 256, 0, st>>>(g, flmn, nb.base, dl.ti, dl.offi, dl.nmt, zero_page(), X1);
   else
@@ -463,7 +508,7 @@ void launch_contract_fwd(const Geo &g, const double2 *Gp, NBase nb, const DlTabl
                          double2 *flmn, hipStream_t st) {
   const int Np = std::min(g.N, g.L);
   const int nlb = (g.L + 63) / 64;
-  dim3 grid((g.L + kXG - 1) / kXG, (Np + 3) / 4, nlb);
+  const unsigned grid = (unsigned)cblock_grid((g.L + kXG - 1) / kXG, (Np + 3) / 4, nlb);
   if (g.NS == 2)
     k_contract_fwd<2><<<grid, 256, 0, st>>>(g, (const double *)Gp, nb.base, dl.tf, dl.offf, zero_page(), flmn);
   else
