@@ -820,11 +820,13 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) 
     __syncthreads();
     {
       // a thread keeps column c = threadIdx.x % C, rows b = threadIdx.x / C + i 64 W / C
+      // (the MW_SS Nyquist column m = -L holds no data: exact zeros)
       const int c = threadIdx.x % C, m = m0 + c;
       if (m <= L - 1) {
         double2 *col = X2 + m_slot_of(g, m);
+        const bool data = m >= -(L - 1);
         for (int b = threadIdx.x / C; b < g.nB; b += 64 * W / C)
-          col[x2_row(g, nslot, b) * g.ldA] = tile[b * LD + c];
+          col[x2_row(g, nslot, b) * g.ldA] = data ? tile[b * LD + c] : make_double2(0.0, 0.0);
       }
     }
     __syncthreads();  // the tile is read out before the next group's transposes
