@@ -193,7 +193,10 @@ __device__ __forceinline__ void fft_G(double2 *v, double *T, const double2 *tw1,
 // chirp: n entries; bh: the kernel spectrum, in register order and divided
 // by 1024 (the LDS copy of reg_stage_blue, BHREG = 1) or in natural order
 // (FftDesc's, BHREG = 0).  Slots q < 4 (k < 256) are always inside the
-// sequence and slots q >= 8 (k >= 512) never, so only q = 4..7 test k < n;
+// sequence and slots q >= 8 (k >= 512) never, so only q = 4..7 test k < n.
+// Slots q >= 8 are set to constant zero before the first transform, so its
+// first radix-4 step has two zero inputs; callers that read only slots
+// q < 8 afterwards let the compiler drop half of the last radix-4 step;
 // LOWREG tests every slot, which serialises the chirp loads and keeps fewer
 // registers live (for kernels near the register limit).
 template <int DIR, int BHREG, int LOWREG = 0>
@@ -204,6 +207,8 @@ __device__ __forceinline__ void blue_reg(double2 *v, int n, const double2 *tw, c
     const int k = lane + 64 * q;
     if ((!LOWREG && q < 4) || k < n) v[q] = cmul(v[q], tw_load<DIR>(chirp, k));
   }
+#pragma unroll
+  for (int q = 8; q < 16; ++q) v[q] = make_double2(0.0, 0.0);
   phase_fence();
   fft_F<-1>(v, T, tw, tw + 1024, lane);
 #pragma unroll

@@ -98,6 +98,10 @@ __global__ __launch_bounds__(kThreads) void k_rows(FftDesc fd, int K, int ld, do
 // + i * gridDim.x until none is left (the grid is what fits on the device at
 // once, reg_grid).
 constexpr int kRegWaves = kThreads / 64;
+// wavefronts per workgroup of the rows kernel, whose wavefronts never
+// synchronise: one workgroup per CU sharing one copy of the tables, three
+// wavefronts per SIMD
+constexpr int kRowsWaves = 12;
 // wavefronts per workgroup of the beta kernels, measured best at L = 256:
 // 4 for the tiled ones (two workgroups per CU), 12 for the forward pair
 // kernel, whose wavefronts never synchronise (one workgroup per CU sharing
@@ -143,7 +147,7 @@ size_t reg_shared(int tabs, int64_t tile) {
 enum RowsMap { kRowsFlat = 0, kRowsX2F = 1, kRowsFX2 = 2 };
 
 template <int DIR, int BLUE>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_rows_reg(
+__global__ __launch_bounds__(64 * kRowsWaves) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_rows_reg(
     FftDesc fd, const double2 *src, double2 *dst, int64_t nrows, int n, int64_t rs, double scale, Geo g,
     int map) {
   extern __shared__ double2 lds[];
@@ -154,10 +158,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))
   if constexpr (BLUE) reg_stage_blue<DIR>(tw, fd);
   else reg_stage_tw(tw, fd);
   __syncthreads();
-  const int64_t groups = (nrows + kRegWaves - 1) / kRegWaves;
+  const int64_t groups = (nrows + kRowsWaves - 1) / kRowsWaves;
   for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
     const int lane = loop_lane();
-    const int64_t r = t * kRegWaves + wv;
+    const int64_t r = t * kRowsWaves + wv;
     if (r >= nrows) break;  // the whole wavefront, last group
     int64_t ri = r * rs, ro = r * rs;
     if (map != kRowsFlat) {
@@ -177,7 +181,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))
     if constexpr (BLUE) {
       blue_reg<DIR>(v, n, tw, T, lane);
 #pragma unroll
-      for (int q = 0; q < 16; ++q) {
+      for (int q = 0; q < 8; ++q) {  // n <= 512
         const int k = lane + 64 * q;
         if (k < n) row[k] = cscale(v[q], scale);
       }
@@ -281,7 +285,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))
     blue_reg<DIR>(v, nG, tab, T, lane);
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
+    for (int q = 0; q < 8; ++q) {  // nG <= 512
       const int k = lane + 64 * q;
       if (k < nout) tile[gtile(k, wv)] = v[q];
     }
@@ -610,7 +614,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) 
     {
       double t[16];
 #pragma unroll
-      for (int q = 0; q < 16; ++q) T[lane + 64 * q] = v[q].x;
+      for (int q = 0; q < 8; ++q) T[lane + 64 * q] = v[q].x;
       __builtin_amdgcn_wave_barrier();
       phase_fence();
 #pragma unroll
@@ -622,7 +626,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) 
       __builtin_amdgcn_wave_barrier();
       phase_fence();
 #pragma unroll
-      for (int q = 0; q < 16; ++q) T[lane + 64 * q] = v[q].y;
+      for (int q = 0; q < 8; ++q) T[lane + 64 * q] = v[q].y;
       __builtin_amdgcn_wave_barrier();
       phase_fence();
 #pragma unroll
@@ -780,9 +784,10 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) 
     fetch(t + gridDim.x, PA, PB);  // the next group's loads run during the DFT
     blue_reg<+1>(v, Pb, tab, T, lane);
     // Z(b') for b < nB through the transpose buffer, real then imaginary parts
+    // (b' < Pb <= 512: slots q < 8)
     double zr[kQB];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) T[lane + 64 * q] = v[q].x;
+    for (int q = 0; q < 8; ++q) T[lane + 64 * q] = v[q].x;
     __builtin_amdgcn_wave_barrier();
     phase_fence();
 #pragma unroll
@@ -794,7 +799,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) 
     __builtin_amdgcn_wave_barrier();
     phase_fence();
 #pragma unroll
-    for (int q = 0; q < 16; ++q) T[lane + 64 * q] = v[q].y;
+    for (int q = 0; q < 8; ++q) T[lane + 64 * q] = v[q].y;
     __builtin_amdgcn_wave_barrier();
     phase_fence();
     double2 oA[kQB], oB[kQB];
@@ -906,7 +911,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(3, 3))) 
     {
       double t[16];
 #pragma unroll
-      for (int q = 0; q < 16; ++q) T[lane + 64 * q] = v[q].x;
+      for (int q = 0; q < 8; ++q) T[lane + 64 * q] = v[q].x;
       __builtin_amdgcn_wave_barrier();
       phase_fence();
 #pragma unroll
@@ -919,7 +924,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(3, 3))) 
       __builtin_amdgcn_wave_barrier();
       phase_fence();
 #pragma unroll
-      for (int q = 0; q < 16; ++q) T[lane + 64 * q] = v[q].y;
+      for (int q = 0; q < 8; ++q) T[lane + 64 * q] = v[q].y;
       __builtin_amdgcn_wave_barrier();
       phase_fence();
 #pragma unroll
@@ -1219,10 +1224,10 @@ void launch_inv_beta(const Geo &g, const double2 *X1, double2 *X2, NBase nb, con
 // rows of length n on the register path (fft_plan(n).len == 1024)
 static void launch_rows_reg(const FftDesc &fd, const double2 *src, double2 *dst, int64_t nrows, int n,
                             int64_t rs, int dir, double scale, const Geo &g, int map, hipStream_t st) {
-  const size_t sh = reg_shared<kRegWaves>(fd.blue ? kRegBlueTab : kRegTw, 0);
+  const size_t sh = reg_shared<kRowsWaves>(fd.blue ? kRegBlueTab : kRegTw, 0);
   auto go = [&](auto kern) {
-    const unsigned grid = reg_grid(kern, kThreads, sh, (nrows + kRegWaves - 1) / kRegWaves);
-    kern<<<grid, kThreads, sh, st>>>(fd, src, dst, nrows, n, rs, scale, g, map);
+    const unsigned grid = reg_grid(kern, 64 * kRowsWaves, sh, (nrows + kRowsWaves - 1) / kRowsWaves);
+    kern<<<grid, 64 * kRowsWaves, sh, st>>>(fd, src, dst, nrows, n, rs, scale, g, map);
   };
   if (fd.blue)
     dir > 0 ? go(k_rows_reg<+1, 1>) : go(k_rows_reg<-1, 1>);
