@@ -967,6 +967,17 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(3, 3))) 
     phase_fence();
     const bool sAok = mA >= -(L - 1) && mA <= L - 1, sBok = mB >= -(L - 1) && mB <= L - 1;
     const int an = n < 0 ? -n : n, sn = n < 0;
+    // G' row of each member (wave-uniform) when it has a single combination
+    bool one[2], flip[2];
+    double2 *row[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int m = e ? mB : mA;
+      const int am = m < 0 ? -m : m, sm = m < 0;
+      one[e] = am != 0 && an != 0;
+      flip[e] = sm != sn;
+      row[e] = Gp + (((int64_t)an * L + am) * C2 + sm + 2 * sn) * L;
+    }
 #pragma unroll
     for (int j = 0; j < kFold; ++j) {
       const int mp = lane + 64 * j;
@@ -993,10 +1004,15 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(3, 3))) 
         if (negA) GA = cadd(GA, corr);
         else GB = cadd(GB, corr);
       }
+      if (mp >= L) continue;
       for (int e = 0; e < 2; ++e) {
-        const int m = e ? mB : mA;
-        if (mp >= L || !(e ? sBok : sAok)) continue;
+        if (!(e ? sBok : sAok)) continue;
         const double2 G0 = e ? GB : GA;
+        if (one[e]) {  // |m|, |n| > 0: the one combination (sign m, sign n)
+          row[e][mp] = (flip[e] && (mp & 1)) ? make_double2(-G0.x, -G0.y) : G0;
+          continue;
+        }
+        const int m = e ? mB : mA;
         const int am = m < 0 ? -m : m, sm = m < 0;
         double2 *dst = Gp + ((int64_t)an * L + am) * C2 * L;
         for (int cm = 0; cm < 2; ++cm) {
