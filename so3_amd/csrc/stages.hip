@@ -1027,6 +1027,202 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(3, 3))) 
   }
 }
 
+// ---------------------------------------------------------------- paired beta, LDS engine
+// The paired beta stages (see above) for the other DFT lengths: a
+// workgroup transforms K pairs, 2K consecutive m from m_lo, in the LDS
+// Stockham engine.  Reference counterparts as for k_inv_beta / k_fwd_beta.
+__global__ __launch_bounds__(kThreads) void k_inv_beta_gp(Geo g, FftDesc fd, int K, int ld,
+                                                          const double2 *__restrict__ X1,
+                                                          double2 *__restrict__ X2,
+                                                          const int64_t *__restrict__ nbase,
+                                                          const double2 *__restrict__ phase) {
+  extern __shared__ double2 lds[];
+  double2 *A = lds, *tw = lds + K * ld;
+  fft_load_twiddles(tw, fd);
+  const int L = g.L, Pb = g.Pb, C = 2 * K;
+  const int mlo = -(L - 1) - g.ss;
+  const int chunks = (g.nA + C - 1) / C;
+  const int nslot = blockIdx.x / chunks;
+  const int m0 = mlo + (blockIdx.x % chunks) * C;
+  const int n = n_of_slot(g, nslot);
+  const bool active = nbase[n + g.N - 1] != kNoBlock;
+  for (int idx = threadIdx.x; idx < K * Pb; idx += blockDim.x) {
+    const int s = idx / Pb, k = idx - s * Pb;
+    const int mA = m0 + 2 * s, mB = mA + 1;
+    const bool hA = active && mA >= -(L - 1) && mA <= L - 1;
+    const bool hB = active && mB >= -(L - 1) && mB <= L - 1;
+    const bool negA = ((mA + n) & 1) != 0;
+    const int mp = k <= L - 1 ? k : k - Pb;
+    double2 v = make_double2(0.0, 0.0);
+    if (mp >= -(L - 1)) {
+      const int am = mp < 0 ? -mp : mp;
+      double2 a = hA ? X1[((int64_t)nslot * g.nA + m_slot_of(g, mA)) * L + am] : make_double2(0.0, 0.0);
+      double2 b = hB ? X1[((int64_t)nslot * g.nA + m_slot_of(g, mB)) * L + am] : make_double2(0.0, 0.0);
+      if (mp < 0) {
+        if (negA) a = make_double2(-a.x, -a.y);
+        else b = make_double2(-b.x, -b.y);
+      }
+      if (mp == 0) {  // the antisymmetric member vanishes at m' = 0
+        if (negA) a = make_double2(0.0, 0.0);
+        else b = make_double2(0.0, 0.0);
+      }
+      v = cmul(make_double2(a.x - b.y, a.y + b.x), phase[k]);
+    }
+    A[s * ld + k] = v;
+  }
+  __syncthreads();
+  fft_run<+1>(A, K, ld, fd, tw);
+  for (int idx = threadIdx.x; idx < K * g.nB; idx += blockDim.x) {
+    const int b = idx / K, s = idx - b * K;
+    const int mA = m0 + 2 * s, mB = mA + 1;
+    const double sA = ((mA + n) & 1) ? -1.0 : 1.0;
+    const int bp = g.ss ? (b == 0 ? 0 : Pb - b) : Pb - 1 - b;
+    const double2 zb = A[s * ld + b], zp = A[s * ld + bp];
+    const double2 su = make_double2(zb.x + sA * zp.x, zb.y + sA * zp.y);
+    const double2 df = make_double2(zb.x - sA * zp.x, zb.y - sA * zp.y);
+    double2 *row = X2 + x2_row(g, nslot, b) * g.ldA;
+    // (the MW_SS Nyquist column m = -L holds no data: exact zeros)
+    if (mA <= L - 1)
+      row[m_slot_of(g, mA)] = mA >= -(L - 1) ? make_double2(0.5 * su.x, 0.5 * su.y) : make_double2(0.0, 0.0);
+    if (mB <= L - 1) row[m_slot_of(g, mB)] = make_double2(0.5 * df.y, -0.5 * df.x);
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_fwd_beta_gp(Geo g, FftDesc fb, FftDesc fw, int K, int ld,
+                                                          const double2 *__restrict__ X2,
+                                                          double2 *__restrict__ Gp,
+                                                          const int64_t *__restrict__ nbase,
+                                                          const double2 *__restrict__ phase,
+                                                          const double2 *__restrict__ wre,
+                                                          const double2 *__restrict__ pole, int Pw) {
+  constexpr double kPi = 3.141592653589793238462643383279502884;
+  extern __shared__ double2 lds[];
+  // one twiddle table serves both transforms when their lengths agree;
+  // F~(+-1) of every pair after the twiddles
+  double2 *A = lds, *twb = lds + K * ld, *tww = fw.len == fb.len ? twb : twb + fb.len;
+  double2 *zpm = (fw.len == fb.len ? twb + fb.len : tww + fw.len);
+  const int L = g.L, Pb = g.Pb, C = 2 * K;
+  const int mlo = -(L - 1) - g.ss;
+  const int chunks = (g.nA + C - 1) / C;
+  const int nslot = blockIdx.x / chunks;
+  const int m0 = mlo + (blockIdx.x % chunks) * C;
+  const int n = n_of_slot(g, nslot);
+  if (nbase[n + g.N - 1] == kNoBlock) return;  // block not transformed
+  fft_load_twiddles(twb, fb);
+  if (tww != twb) fft_load_twiddles(tww, fw);
+  for (int idx = threadIdx.x; idx < K * Pb; idx += blockDim.x) {
+    const int k = idx / K, s = idx - k * K;
+    const int mA = m0 + 2 * s, mB = mA + 1;
+    const bool hA = mA >= -(L - 1) && mA <= L - 1, hB = mB >= -(L - 1) && mB <= L - 1;
+    const bool negA = ((mA + n) & 1) != 0;
+    // MW: F(2L-2-b) for b >= L; MW_SS: F(2L-b) for b > L
+    const bool direct = g.ss ? k <= L : k < L;
+    const int kr = direct ? k : (g.ss ? 2 * L - k : 2 * L - 2 - k);
+    const double2 *row = X2 + x2_row(g, nslot, kr) * g.ldA;
+    double2 a = hA ? row[m_slot_of(g, mA)] : make_double2(0.0, 0.0);
+    double2 b = hB ? row[m_slot_of(g, mB)] : make_double2(0.0, 0.0);
+    if (!direct) {
+      if (negA) a = make_double2(-a.x, -a.y);
+      else b = make_double2(-b.x, -b.y);
+    }
+    const bool pk = g.ss ? (k == 0 || k == L) : (k == L - 1);
+    if (pk) {  // pole samples of the antisymmetric member go through the pole table
+      if (negA) a = make_double2(0.0, 0.0);
+      else b = make_double2(0.0, 0.0);
+    }
+    A[s * ld + k] = make_double2(a.x - b.y, a.y + b.x);
+  }
+  __syncthreads();
+  fft_run<-1>(A, K, ld, fb, twb);
+  for (int s = threadIdx.x; s < K; s += blockDim.x) {
+    zpm[2 * s] = cmul(A[s * ld + 1], phase[L]);
+    zpm[2 * s + 1] = cmul(A[s * ld + Pb - 1], phase[L - 2]);
+  }
+  // F~(m'') = DFT/Pb * phase, placed at m'' mod Pw (zero elsewhere); the
+  // remap runs through registers because it widens the sequence in place
+  {
+    constexpr int kMaxPer = kFftElems * 2;
+    double2 v[kMaxPer];
+#pragma unroll
+    for (int b = 0; b < kMaxPer; ++b) {
+      const int idx = threadIdx.x + b * kThreads;
+      v[b] = make_double2(0.0, 0.0);
+      if (idx < K * Pw) {
+        const int s = idx / Pw, r = idx - s * Pw;
+        const int mpp = r <= L - 1 ? r : r - Pw;
+        if (mpp >= -(L - 1)) v[b] = cmul(A[s * ld + (mpp < 0 ? mpp + Pb : mpp)], phase[mpp + L - 1]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < kMaxPer; ++b) {
+      const int idx = threadIdx.x + b * kThreads;
+      if (idx < K * Pw) {
+        const int s = idx / Pw, r = idx - s * Pw;
+        A[s * ld + r] = v[b];
+      }
+    }
+  }
+  __syncthreads();
+  // convolution with the even weight taps
+  fft_run<+1>(A, K, ld, fw, tww);
+  for (int idx = threadIdx.x; idx < K * Pw; idx += blockDim.x) {
+    const int s = idx / Pw, r = idx - s * Pw;
+    A[s * ld + r] = cmul(A[s * ld + r], wre[r]);
+  }
+  __syncthreads();
+  fft_run<-1>(A, K, ld, fw, tww);
+  const double sc = kTwoPi / (double)Pw;
+  const int C2 = 2 * g.NS;
+  const int an = n < 0 ? -n : n, sn = n < 0;
+  for (int idx = threadIdx.x; idx < K * L; idx += blockDim.x) {
+    const int s = idx / L, mp = idx - s * L;
+    const int mA = m0 + 2 * s, mB = mA + 1;
+    const bool negA = ((mA + n) & 1) != 0;
+    const double sA = negA ? -1.0 : 1.0;
+    const double2 zp = cscale(A[s * ld + mp], sc), zm = cscale(A[s * ld + (mp == 0 ? 0 : Pw - mp)], sc);
+    double2 GA, GB;
+    if (mp > 0) {
+      GA = make_double2(zp.x + sA * zm.x, zp.y + sA * zm.y);
+      GB = make_double2(zp.y - sA * zm.y, -(zp.x - sA * zm.x));
+    } else {
+      const double pp = kPi * kPi;
+      const double2 z1 = zpm[2 * s], zm1 = zpm[2 * s + 1];
+      const double2 dz = make_double2(z1.x - zm1.x, z1.y - zm1.y);
+      if (!negA) {
+        GA = zp;
+        GB = make_double2(pp * dz.x, pp * dz.y);
+      } else {
+        GA = make_double2(-pp * dz.y, pp * dz.x);
+        GB = make_double2(zp.y, -zp.x);
+      }
+    }
+    // pole samples of the antisymmetric member, through their responses
+    const int mX = negA ? mA : mB;
+    if (mX >= -(L - 1) && mX <= L - 1) {
+      const double2 *colX = X2 + m_slot_of(g, mX);
+      double2 corr = cmul(colX[x2_row(g, nslot, g.ss ? 0 : L - 1) * g.ldA], pole[mp]);
+      if (g.ss) corr = cadd(corr, cmul(colX[x2_row(g, nslot, L) * g.ldA], pole[L + mp]));
+      if (negA) GA = cadd(GA, corr);
+      else GB = cadd(GB, corr);
+    }
+    for (int e = 0; e < 2; ++e) {
+      const int m = e ? mB : mA;
+      if (m < -(L - 1) || m > L - 1) continue;  // MW_SS Nyquist column / past the last m
+      const double2 G0 = e ? GB : GA;
+      const int am = m < 0 ? -m : m, sm = m < 0;
+      double2 *dst = Gp + ((int64_t)an * L + am) * C2 * L + mp;
+      for (int cm = 0; cm < 2; ++cm) {
+        if (cm != sm && am != 0) continue;
+        for (int cn = 0; cn < g.NS; ++cn) {
+          if (cn != sn && an != 0) continue;
+          dst[(int64_t)(cm + 2 * cn) * L] = ((cm != cn) && (mp & 1)) ? make_double2(-G0.x, -G0.y) : G0;
+        }
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------- adjoint of the forward: beta
 __global__ __launch_bounds__(kThreads) void k_adjf_beta(Geo g, FftDesc fb, FftDesc fw, int K, int ld,
                                                         const double2 *__restrict__ X1,
@@ -1231,9 +1427,9 @@ void launch_inv_beta(const Geo &g, const double2 *X1, double2 *X2, NBase nb, con
   const int tb = fd.len * (int)sizeof(double2);
   const int K = choose_k(ld, fft_capacity(fd), fd.len, tb, 8);
   const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
-  set_lds(k_inv_beta, sh);
-  const int chunks = (g.nA + K - 1) / K;
-  k_inv_beta<<<g.nG * chunks, kThreads, sh, st>>>(g, fd, K, ld, X1, X2, nb.base, phase);
+  set_lds(k_inv_beta_gp, sh);
+  const int chunks = (g.nA + 2 * K - 1) / (2 * K);  // K pairs per workgroup
+  k_inv_beta_gp<<<g.nG * chunks, kThreads, sh, st>>>(g, fd, K, ld, X1, X2, nb.base, phase);
   SO3_LAUNCH_CHECK();
 }
 
@@ -1336,16 +1532,16 @@ void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, double scale, hi
 }
 
 void launch_fwd_beta(const Geo &g, const double2 *X2, double2 *Gp, NBase nb, const double2 *phase,
-                     const double2 *wr, const double2 *wr_even, const double2 *pole, int Pw, int adjoint,
-                     hipStream_t st) {
-  if (!adjoint && wr_even && pole && Pw == 1024 && g.L <= 256 && fft_plan(g.Pb).len == 1024 &&
+                     const double2 *wr, const double2 *wr_even, const double2 *wr_even_reg,
+                     const double2 *pole, int Pw, int adjoint, hipStream_t st) {
+  if (!adjoint && wr_even_reg && pole && Pw == 1024 && g.L <= 256 && fft_plan(g.Pb).len == 1024 &&
       fft_plan(g.Pb).blue) {
     const FftDesc fb = fft_plan(g.Pb);
     constexpr int W = kBetaFW;
     const size_t sh = reg_shared<W>(kRegBlueTab, 0);
     const int chunks = (g.nA + 2 * W - 1) / (2 * W);
     const unsigned grid = reg_grid(k_fwd_beta_pair<W>, 64 * W, sh, (int64_t)g.nG * chunks);
-    k_fwd_beta_pair<W><<<grid, 64 * W, sh, st>>>(g, fb, X2, Gp, nb.base, phase, wr_even, pole);
+    k_fwd_beta_pair<W><<<grid, 64 * W, sh, st>>>(g, fb, X2, Gp, nb.base, phase, wr_even_reg, pole);
     SO3_LAUNCH_CHECK();
     return;
   }
@@ -1363,6 +1559,17 @@ void launch_fwd_beta(const Geo &g, const double2 *X2, double2 *Gp, NBase nb, con
   const FftDesc fw = fft_plan(Pw);
   const int ld = std::max(fft_ld(fb), fft_ld(fw));
   const int tb = (fb.len + (fw.len == fb.len ? 0 : fw.len)) * (int)sizeof(double2);
+  if (!adjoint && wr_even && pole) {
+    // K pairs per workgroup, F~(+-1) of each after the tables
+    const int K = choose_k2(ld, fb, fw, tb + 16 * (int)sizeof(double2), 8);
+    const size_t sh = (size_t)K * ld * sizeof(double2) + tb + 2 * K * sizeof(double2);
+    set_lds(k_fwd_beta_gp, sh);
+    const int chunks = (g.nA + 2 * K - 1) / (2 * K);
+    k_fwd_beta_gp<<<g.nG * chunks, kThreads, sh, st>>>(g, fb, fw, K, ld, X2, Gp, nb.base, phase, wr_even, pole,
+                                                       Pw);
+    SO3_LAUNCH_CHECK();
+    return;
+  }
   const int K = choose_k2(ld, fb, fw, tb, 8);
   const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
   set_lds(k_fwd_beta, sh);
