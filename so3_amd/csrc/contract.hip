@@ -259,11 +259,15 @@ __global__ __launch_bounds__(256) void k_contract_inv(Geo g, const double2 *__re
   const int nch = (lend - lbeg) >> 2;
   for (int r = 0; r < kRing - 1 && r < nch; ++r) issue(lbeg + 4 * r, sm + r * kBufD);
   for (int q = 0; q < nch; ++q) {
-    if (q + kRing - 1 < nch) issue(lbeg + 4 * (q + kRing - 1), sm + ((q + kRing - 1) % kRing) * kBufD);
-    wait_ring(min(nch - 1 - q, kRing - 1), St::kInstr);
-    // this wavefront's copies of chunk q have landed; the barrier makes
-    // everyone's visible (no fence: it would wait for the chunks in flight)
+    // this wavefront's copies of chunk q have landed (those of chunk q + 1
+    // may still be in flight) and its reads of chunk q - 1 have returned; the
+    // one barrier per chunk then makes chunk q visible to everyone and frees
+    // the buffer of chunk q - 1, which the copies of chunk q + 2 refill (no
+    // fence: it would wait for the chunk in flight)
+    wait_ring(min(nch - 1 - q, kRing - 2), St::kInstr);
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
     __builtin_amdgcn_s_barrier();
+    if (q + kRing - 1 < nch) issue(lbeg + 4 * (q + kRing - 1), sm + ((q + kRing - 1) % kRing) * kBufD);
     const double *buf = sm + (q % kRing) * kBufD;
     const int l0 = lbeg + 4 * q;
     // c_l and the (-1)^l of the mixed-sign combinations on this lane's row
@@ -284,10 +288,6 @@ __global__ __launch_bounds__(256) void k_contract_inv(Geo g, const double2 *__re
 #pragma unroll
       for (int a = 0; a < kXG; ++a) acc[a][t] = mma(cv[a], buf[(4 * a + t) * 64 + lane] * dy[t], acc[a][t]);
     }
-    // every read of this buffer has returned (its values fed the matrix
-    // instructions) before any wavefront refills it
-    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
-    __builtin_amdgcn_s_barrier();
   }
   if (!yv) return;
 
@@ -428,11 +428,11 @@ __global__ __launch_bounds__(256) void k_contract_fwd(Geo g, const double *__res
   const int nch = (min(lo + 63, g.L - 1) >> 2) + 1;
   for (int r = 0; r < kRing - 1 && r < nch; ++r) issue(4 * r, sm + r * kBufD);
   for (int q = 0; q < nch; ++q) {
-    if (q + kRing - 1 < nch) issue(4 * (q + kRing - 1), sm + ((q + kRing - 1) % kRing) * kBufD);
-    wait_ring(min(nch - 1 - q, kRing - 1), St::kInstr);
-    // this wavefront's copies of chunk q have landed; the barrier makes
-    // everyone's visible (no fence: it would wait for the chunks in flight)
+    // one barrier per chunk, as in the inverse
+    wait_ring(min(nch - 1 - q, kRing - 2), St::kInstr);
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
     __builtin_amdgcn_s_barrier();
+    if (q + kRing - 1 < nch) issue(4 * (q + kRing - 1), sm + ((q + kRing - 1) % kRing) * kBufD);
     const double *buf = sm + (q % kRing) * kBufD;
     const int m0 = 4 * q;
     double cv[kXG], dy[kNT];
@@ -447,10 +447,6 @@ __global__ __launch_bounds__(256) void k_contract_fwd(Geo g, const double *__res
 #pragma unroll
       for (int a = 0; a < kXG; ++a) acc[a][t] = mma(cv[a], buf[(4 * a + t) * 64 + lane] * dy[t], acc[a][t]);
     }
-    // every read of this buffer has returned (its values fed the matrix
-    // instructions) before any wavefront refills it
-    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
-    __builtin_amdgcn_s_barrier();
   }
   if (!yv) return;
 
