@@ -17,6 +17,7 @@
 
 #include "fft_plan.hpp"
 #include "fft_reg.hpp"
+#include "fft_reg32.hpp"
 #include "kernels.hpp"
 
 namespace so3g {
@@ -146,17 +147,16 @@ size_t reg_shared(int tabs, int64_t tile) {
 //   kRowsFX2  : f row r -> X2 row x2_row(k, b)
 enum RowsMap { kRowsFlat = 0, kRowsX2F = 1, kRowsFX2 = 2 };
 
-template <int DIR, int BLUE>
+// Rows of length n = 1024 (no Bluestein): one row per wavefront in the
+// layouts of fft_reg.hpp
+template <int DIR>
 __global__ __launch_bounds__(64 * kRowsWaves) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_rows_reg(
-    FftDesc fd, const double2 *src, double2 *dst, int64_t nrows, int n, int64_t rs, double scale, Geo g,
-    int map) {
+    FftDesc fd, const double2 *src, double2 *dst, int64_t nrows, int64_t rs, double scale, Geo g, int map) {
   extern __shared__ double2 lds[];
   const int wv = threadIdx.x >> 6;
-  constexpr int kTabs = BLUE ? kRegBlueTab : kRegTw;
   double2 *tw = lds;
-  double *T = (double *)(lds + kTabs) + wv * kRegT;
-  if constexpr (BLUE) reg_stage_blue<DIR>(tw, fd);
-  else reg_stage_tw(tw, fd);
+  double *T = (double *)(lds + kRegTw) + wv * kRegT;
+  reg_stage_tw(tw, fd);
   __syncthreads();
   const int64_t groups = (nrows + kRowsWaves - 1) / kRowsWaves;
   for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
@@ -174,21 +174,59 @@ __global__ __launch_bounds__(64 * kRowsWaves) __attribute__((amdgpu_waves_per_eu
     double2 *row = dst + ro;
     double2 v[16];
 #pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = in[lane + 64 * q];
+    fft_F<DIR>(v, T, tw, tw + 1024, lane);
+#pragma unroll
+    for (int c = 0; c < 16; ++c) row[o_index(lane, c)] = cscale(v[c], scale);
+  }
+}
+
+// Rows of a Bluestein length n in [257, 512] on the two-sequence register
+// FFT (fft_reg32.hpp): wavefront w of group t takes rows 2 (t W + w) and the
+// next one, one per half-wavefront, so each access instruction moves two
+// runs of 32 consecutive entries.  Two wavefronts per SIMD (a lane holds 32
+// complex values), one workgroup per CU sharing one copy of the tables.
+constexpr int kRows32Waves = 8;
+
+template <int DIR, int ODD>
+__global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_rows_r32(
+    FftDesc fd, const double2 *src, double2 *dst, int64_t nrows, int n, int64_t rs, double scale, Geo g,
+    int map) {
+  extern __shared__ double2 lds[];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar index math
+  double2 *tab = lds;
+  double *T = (double *)(lds + kR32BlueTab) + wv * kR32T;
+  r32_stage_blue<DIR>(tab, fd);
+  __syncthreads();
+  const int64_t groups = (nrows + 2 * kRows32Waves - 1) / (2 * kRows32Waves);
+  // offsets of this lane's row of group t in src (IN = 1) or dst; a missing
+  // second row of the last group repeats the first one (both halves then
+  // write the same values)
+  auto row_at = [&](int64_t t, int lane, bool IN) {
+    const int64_t r0 = (t * kRows32Waves + wv) * 2;
+    const int h = r0 + 1 < nrows ? lane >> 5 : 0;
+    if (map == kRowsFlat) return (r0 + h) * rs;
+    int k = (int)(r0 / g.nB), b = (int)(r0 % g.nB) + h;
+    if (b == g.nB) b = 0, ++k;
+    const bool x2 = (map == kRowsX2F) == IN;
+    return x2 ? x2_row(g, k, b) * g.ldA : (r0 + h) * g.nA;
+  };
+  for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
+    if ((t * kRows32Waves + wv) * 2 >= nrows) break;  // the whole wavefront, last group
+    const int lane = loop_lane(), l = lane & 31;
+    const double2 *in = src + row_at(t, lane, true);
+    double2 v[32];
+#pragma unroll
     for (int q = 0; q < 16; ++q) {
-      const int k = lane + 64 * q;
-      v[q] = k < n ? in[k] : make_double2(0.0, 0.0);
+      const int k = l + 32 * q;
+      v[q] = in[q < 8 ? k : min(k, n - 1)];  // past n: cleared by the chirp
     }
-    if constexpr (BLUE) {
-      blue_reg<DIR>(v, n, tw, T, lane);
+    blue32<DIR, ODD>(v, n, tab, T, lane);
+    double2 *row = dst + row_at(t, lane, false);
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {  // n <= 512
-        const int k = lane + 64 * q;
-        if (k < n) row[k] = cscale(v[q], scale);
-      }
-    } else {
-      fft_F<DIR>(v, T, tw, tw + 1024, lane);
-#pragma unroll
-      for (int c = 0; c < 16; ++c) row[o_index(lane, c)] = cscale(v[c], scale);
+    for (int q = 0; q < 16; ++q) {
+      const int k = l + 32 * q;
+      if (q < 8 || k < n) row[k] = cscale(v[q], scale);  // n > 256
     }
   }
 }
@@ -1436,15 +1474,25 @@ void launch_inv_beta(const Geo &g, const double2 *X1, double2 *X2, NBase nb, con
 // rows of length n on the register path (fft_plan(n).len == 1024)
 static void launch_rows_reg(const FftDesc &fd, const double2 *src, double2 *dst, int64_t nrows, int n,
                             int64_t rs, int dir, double scale, const Geo &g, int map, hipStream_t st) {
-  const size_t sh = reg_shared<kRowsWaves>(fd.blue ? kRegBlueTab : kRegTw, 0);
-  auto go = [&](auto kern) {
-    const unsigned grid = reg_grid(kern, 64 * kRowsWaves, sh, (nrows + kRowsWaves - 1) / kRowsWaves);
-    kern<<<grid, 64 * kRowsWaves, sh, st>>>(fd, src, dst, nrows, n, rs, scale, g, map);
-  };
-  if (fd.blue)
-    dir > 0 ? go(k_rows_reg<+1, 1>) : go(k_rows_reg<-1, 1>);
-  else
-    dir > 0 ? go(k_rows_reg<+1, 0>) : go(k_rows_reg<-1, 0>);
+  if (fd.blue) {
+    const size_t sh = sizeof(double2) * kR32BlueTab + sizeof(double) * kRows32Waves * kR32T;
+    auto go = [&](auto kern) {
+      const int64_t groups = (nrows + 2 * kRows32Waves - 1) / (2 * kRows32Waves);
+      const unsigned grid = reg_grid(kern, 64 * kRows32Waves, sh, groups);
+      kern<<<grid, 64 * kRows32Waves, sh, st>>>(fd, src, dst, nrows, n, rs, scale, g, map);
+    };
+    if (n & 1)
+      dir > 0 ? go(k_rows_r32<+1, 1>) : go(k_rows_r32<-1, 1>);
+    else
+      dir > 0 ? go(k_rows_r32<+1, 0>) : go(k_rows_r32<-1, 0>);
+  } else {
+    const size_t sh = reg_shared<kRowsWaves>(kRegTw, 0);
+    auto go = [&](auto kern) {
+      const unsigned grid = reg_grid(kern, 64 * kRowsWaves, sh, (nrows + kRowsWaves - 1) / kRowsWaves);
+      kern<<<grid, 64 * kRowsWaves, sh, st>>>(fd, src, dst, nrows, rs, scale, g, map);
+    };
+    dir > 0 ? go(k_rows_reg<+1>) : go(k_rows_reg<-1>);
+  }
   SO3_LAUNCH_CHECK();
 }
 
