@@ -69,9 +69,9 @@ def stage_model(p, A, T):
 
 
 # kernel that carries each stage (names as rocprofv3 reports them)
-STAGE_KERNEL = {"contract_inv": "k_contract_inv", "beta_inv": "k_inv_beta_pair", "alpha_inv": "k_rows_r32<1, 1>",
-                "gamma_inv": "k_gamma_tile<2>", "gamma_fwd": "k_gamma_tile<3>", "alpha_fwd": "k_rows_r32<-1, 1>",
-                "beta_fwd": "k_fwd_beta_pair", "contract_fwd": "k_contract_fwd"}
+STAGE_KERNEL = {"contract_inv": "k_contract_inv", "beta_inv": "k_inv_beta_r32", "alpha_inv": "k_rows_r32<1, 1>",
+                "gamma_inv": "k_gamma_r32<1>", "gamma_fwd": "k_gamma_r32<-1>", "alpha_fwd": "k_rows_r32<-1, 1>",
+                "beta_fwd": "k_fwd_beta_r32", "contract_fwd": "k_contract_fwd"}
 
 
 def measured_traffic(kernel):

@@ -77,12 +77,11 @@ void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, double scale, hi
 void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, double scale, hipStream_t st);
 // adjoint != 0: zero-fill instead of the beta extension and no weights (the
 // beta stage of the adjoint of the inverse).  wr_even (the weight spectrum
-// without the two odd taps, natural order; wr_even_reg the same in the
-// register FFT's order, Pw = 1024 only) and pole (the folded responses of
+// without the two odd taps, natural order) and pole (the folded responses of
 // the pole samples, 2L entries) serve the paired kernels; null for the
 // adjoint.
 void launch_fwd_beta(const Geo &g, const double2 *X2, double2 *Gp, NBase nb, const double2 *phase,
-                     const double2 *wr, const double2 *wr_even, const double2 *wr_even_reg,
+                     const double2 *wr, const double2 *wr_even,
                      const double2 *pole, int Pw, int adjoint, hipStream_t st);
 // beta stage of the adjoint of the forward transform (so3_adjoint.c:730-905):
 // m' < 0 by symmetry, weight convolution, phase exp(+i m' pi/(2L-1)),

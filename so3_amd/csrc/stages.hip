@@ -14,6 +14,7 @@
 //                  the conjugate domain of length Pw >= 4L-3
 //                  (so3_core.c:1366-1492), folded to m' >= 0.
 #include <algorithm>
+#include <type_traits>
 
 #include "fft_plan.hpp"
 #include "fft_reg.hpp"
@@ -103,12 +104,8 @@ constexpr int kRegWaves = kThreads / 64;
 // synchronise: one workgroup per CU sharing one copy of the tables, three
 // wavefronts per SIMD
 constexpr int kRowsWaves = 12;
-// wavefronts per workgroup of the beta kernels, measured best at L = 256:
-// 4 for the tiled ones (two workgroups per CU), 12 for the forward pair
-// kernel, whose wavefronts never synchronise (one workgroup per CU sharing
-// one copy of the tables, three wavefronts per SIMD)
+// wavefronts per workgroup of the tiled beta kernels (two workgroups per CU)
 constexpr int kBetaW = 4;
-constexpr int kBetaFW = 12;
 
 // Workgroups go round-robin over the 8 XCDs, each with its own L2: block p
 // runs on XCD p % 8.  Renumbering block p to (p % 8) q + p / 8, q = n / 8,
@@ -348,6 +345,49 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))
       }
     }
     __syncthreads();  // the tile is read out before the next group's entries
+  }
+}
+
+// Gamma in place on X2 (complex) on the two-sequence register FFT: the
+// columns p = b nA + a are numbered as in k_gamma_tile; wavefront w of group
+// t takes columns 2 (t W + w) and the next one, one per half-wavefront, and
+// loads and stores their nG n slots straight from X2.  The W wavefronts of a
+// workgroup start together on 2 W consecutive columns, whose entries of one
+// slot share 128-byte lines, so most of each line is used from L1 instead of
+// being staged through an LDS tile (no room beside eight transpose buffers).
+template <int DIR>
+__global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gamma_r32(
+    Geo g, FftDesc fd, double2 *X2, double scale) {
+  extern __shared__ double2 lds[];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  double2 *tab = lds;
+  double *T = (double *)(lds + kR32BlueTab) + wv * kR32T;
+  r32_stage_blue<DIR>(tab, fd);
+  __syncthreads();
+  const int S = g.nB * g.nA, nG = g.nG;
+  const int64_t groups = (S + 2 * kRows32Waves - 1) / (2 * kRows32Waves);
+  for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
+    const int p0 = (int)(t * kRows32Waves + wv) * 2;
+    if (p0 >= S) break;  // the whole wavefront, last group
+    const int lane = loop_lane(), l = lane & 31;
+    // a missing second column of the last group repeats the first one (both
+    // halves then write the same values)
+    const int p = p0 + (p0 + 1 < S ? lane >> 5 : 0);
+    const int b = p / g.nA, a = p - b * g.nA;
+    double2 *col = X2 + x2_row(g, 0, b) * g.ldA + a;
+    const int64_t xs = (int64_t)x2_w(g, b) * g.ldA;
+    double2 v[32];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int k = l + 32 * q;
+      v[q] = col[(q < 8 ? k : min(k, nG - 1)) * xs];  // past nG: cleared by the chirp
+    }
+    blue32<DIR, 1>(v, nG, tab, T, lane);  // nG = 2N - 1 is odd
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int k = l + 32 * q;
+      if (q < 8 || k < nG) col[k * xs] = cscale(v[q], scale);  // nG > 256
+    }
   }
 }
 
@@ -739,140 +779,129 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) 
 // F(-m'') = s F(m'') (forward).  Two sequences A, B of opposite signs share
 // one DFT of A + i B and separate afterwards:
 //   A = (Z(b) + s_A Z(b')) / 2,   B = (Z(b) - s_A Z(b')) / (2 i).
-// Consecutive m have opposite signs, so a wavefront takes the pair
-// (m, m + 1); a group is 2W consecutive m from m_lo = -(L-1) - ss (the
-// MW_SS Nyquist column, zero, first).  The inverse kernel writes its group
-// through an LDS tile of nB x (2W + 1) entries (128-byte HBM runs for
-// W = 4); the forward kernel reads its samples straight from X2 (the pairs
-// of a workgroup share each line in L1) and has no workgroup barrier.
+// Consecutive m have opposite signs, so a sequence of the register FFT
+// takes the pair (m, m + 1); groups of consecutive m start at
+// m_lo = -(L-1) - ss (the MW_SS Nyquist column, zero, first).
 __device__ __forceinline__ int m_slot_of(const Geo &g, int m) { return m >= 0 ? m : m + g.nA; }
 
-template <int W>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_inv_beta_pair(
+// Inverse beta, paired, on the two-sequence register FFT: half-wavefront h
+// of wavefront w takes the pair (mA, mA + 1), mA = m0 + 4 w + 2 h, so a
+// group covers 4 W consecutive m of one n slot.  X1 rows are read straight
+// (the |m'| runs are contiguous), Z(b') comes through the wavefront's
+// transpose buffer, and each lane stores its A and B entries straight into
+// X2 (the 4 W consecutive m of a group fill 4 W consecutive entries of each
+// X2 row), with no workgroup barrier.  ODD: Pb odd (MW).
+template <int ODD>
+__global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_inv_beta_r32(
     Geo g, FftDesc fd, const double2 *__restrict__ X1, double2 *__restrict__ X2,
     const int64_t *__restrict__ nbase, const double2 *__restrict__ phase) {
-  constexpr int C = 2 * W, LD = C + 1;
-  constexpr int kQB = 5;  // b = lane + 64 q < nB <= 257
+  constexpr int C = 4 * kRows32Waves;  // m per group
+  constexpr int kQB = 9;               // b = l + 32 j < nB <= 257
   extern __shared__ double2 lds[];
-  const int wv = threadIdx.x >> 6;
-  double2 *tab = lds, *tile = lds + kRegBlueTab;
-  double *T = (double *)(lds + kRegBlueTab) + wv * kRegT;
-  reg_stage_blue<+1>(tab, fd);
-  const int L = g.L, Pb = g.Pb;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  double2 *tab = lds;
+  double *T = (double *)(lds + kR32BlueTab) + wv * kR32T;
+  r32_stage_blue<+1>(tab, fd);
+  __syncthreads();
+  const int L = g.L, Pb = g.Pb;  // 129 <= L <= 256
   const int mlo = -(L - 1) - g.ss;
   const int chunks = (g.nA + C - 1) / C;
   const int64_t groups = (int64_t)g.nG * chunks;
-  // the X1 entries |m'| of this lane (k = lane + 64 q < Pb <= 512) for the
-  // pair of group t, loaded into registers one group ahead
-  constexpr int kQ = 8;
-  auto fetch = [&](int64_t t, double2 (&PA)[kQ], double2 (&PB)[kQ]) {
-    const int ln = threadIdx.x & 63;
+  const int nBf = g.nB & ~(kX2Blk - 1);  // rows b < nBf lie in full X2 blocks
+  for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
+    const int lane = loop_lane(), l = lane & 31, h = lane >> 5;
     const int nslot = (int)(t / chunks);
-    const int mA = mlo + (int)(t % chunks) * C + 2 * wv, mB = mA + 1;
-    const bool act = t < groups && nbase[n_of_slot(g, nslot) + g.N - 1] != kNoBlock;
-    const bool hA = act && mA >= -(L - 1) && mA <= L - 1;
-    const bool hB = act && mB >= -(L - 1) && mB <= L - 1;
-    const double2 *rA = X1 + ((int64_t)nslot * g.nA + (hA ? m_slot_of(g, mA) : 0)) * L;
-    const double2 *rB = X1 + ((int64_t)nslot * g.nA + (hB ? m_slot_of(g, mB) : 0)) * L;
-#pragma unroll
-    for (int q = 0; q < kQ; ++q) {
-      const int k = ln + 64 * q;
-      const int mp = k <= L - 1 ? k : k - Pb;
-      const bool ok = k < Pb && mp >= -(L - 1);
-      const int am = mp < 0 ? -mp : mp;
-      PA[q] = ok && hA ? rA[am] : make_double2(0.0, 0.0);
-      PB[q] = ok && hB ? rB[am] : make_double2(0.0, 0.0);
-    }
-  };
-  double2 PA[kQ], PB[kQ];
-  int64_t t = xcd_tile(blockIdx.x, gridDim.x);
-  fetch(t, PA, PB);
-  __syncthreads();  // tables staged
-  for (; t < groups; t += gridDim.x) {
-    const int lane = loop_lane();
-    const int nslot = (int)(t / chunks);
-    const int m0 = mlo + (int)(t % chunks) * C;
     const int n = n_of_slot(g, nslot);
-    const int mA = m0 + 2 * wv;
-    const bool negA = ((mA + n) & 1) != 0;  // s_A = -1 (then s_B = +1)
-    const double sA = negA ? -1.0 : 1.0;
-    double2 v[16];
+    const int m0 = mlo + (int)(t % chunks) * C + 4 * wv;
+    const int mA = m0 + 2 * h, mB = mA + 1;
+    const bool act = nbase[n + g.N - 1] != kNoBlock;
+    // outputs: columns past m = L - 1 are not stored, the MW_SS Nyquist
+    // column m = -L holds zeros
+    const bool stA = mA <= L - 1, stB = mB <= L - 1;
+    const bool dA = act && mA >= -(L - 1), dB = act && mB >= -(L - 1);
+    const int cA = m_slot_of(g, stA ? mA : 0), cB = m_slot_of(g, stB ? mB : 0);
+    // X2 row of b = l + 32 j
+    const int64_t rb = (int64_t)((l >> 3) * kX2Blk * g.nG + nslot * kX2Blk + (l & 7));
+    auto row_of = [&](int j) {
+      const int b = l + 32 * j;
+      return X2 + (b < nBf ? rb + (int64_t)j * 32 * g.nG : x2_row(g, nslot, b)) * g.ldA;
+    };
+    if (!act) {  // block n not in the transform: zeros
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int k = lane + 64 * q;
-      double2 x = make_double2(0.0, 0.0);
-      if (q < kQ && k < Pb) {
-        const int mp = k <= L - 1 ? k : k - Pb;
-        if (mp >= -(L - 1)) {
-          double2 a = PA[q < kQ ? q : 0];
-          double2 b = PB[q < kQ ? q : 0];
-          if (mp < 0) {
-            if (negA) a = make_double2(-a.x, -a.y);
-            else b = make_double2(-b.x, -b.y);
-          }
-          if (mp == 0) {  // the antisymmetric member vanishes at m' = 0
-            if (negA) a = make_double2(0.0, 0.0);
-            else b = make_double2(0.0, 0.0);
-          }
-          x = cmul(make_double2(a.x - b.y, a.y + b.x), phase[k]);
-        }
+      for (int j = 0; j < kQB; ++j) {
+        if (l + 32 * j >= g.nB) continue;
+        double2 *row = row_of(j);
+        if (stA) row[cA] = make_double2(0.0, 0.0);
+        if (stB) row[cB] = make_double2(0.0, 0.0);
       }
-      v[q] = x;
+      continue;
     }
-    fetch(t + gridDim.x, PA, PB);  // the next group's loads run during the DFT
-    blue_reg<+1>(v, Pb, tab, T, lane);
-    // Z(b') for b < nB through the transpose buffer, real then imaginary parts
-    // (b' < Pb <= 512: slots q < 8)
+    // A member outside [-(L-1), L-1] reads an in-range row of the same
+    // parity: the pair keeps its symmetries, so the separation of the other
+    // member stays exact, and the member's own outputs are not stored.
+    auto fixm = [&](int m) { return m < -(L - 1) ? m + 2 : m > L - 1 ? (L - 1) - ((m - (L - 1)) & 1) : m; };
+    const double2 *rA = X1 + ((int64_t)nslot * g.nA + m_slot_of(g, fixm(mA))) * L;
+    const double2 *rB = X1 + ((int64_t)nslot * g.nA + m_slot_of(g, fixm(mB))) * L;
+    const bool negA = ((m0 + n) & 1) != 0;  // s_A = -1 (then s_B = +1); wave-uniform
+    double2 v[32];
+    // entry k = l + 32 q holds m' = k (k <= L - 1) or k - Pb; k >= Pb only
+    // needs to be finite (the chirp clears it).  q < 4: k < 128 < L; q >= 8:
+    // k >= 256 >= L.
+    auto prep = [&](auto NEG) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int k = q < 8 ? l + 32 * q : min(l + 32 * q, Pb - 1);
+        const int mp = q < 4 ? k : q >= 8 ? k - Pb : (k <= L - 1 ? k : k - Pb);
+        const int am = mp < 0 ? -mp : mp;
+        double2 a = rA[am], b = rB[am];
+        if (q >= 4 && mp < 0) {  // X1(-m') = s X1(m')
+          if (NEG) a = make_double2(-a.x, -a.y);
+          else b = make_double2(-b.x, -b.y);
+        }
+        if (q == 0 && l == 0) {  // the antisymmetric member vanishes at m' = 0
+          if (NEG) a = make_double2(0.0, 0.0);
+          else b = make_double2(0.0, 0.0);
+        }
+        if (!ODD && q >= 4 && q <= 8 && k == L) a = b = make_double2(0.0, 0.0);  // MW_SS: m' = -L
+        v[q] = cmul(make_double2(a.x - b.y, a.y + b.x), phase[k]);
+      }
+    };
+    if (negA) prep(std::true_type{});
+    else prep(std::false_type{});
+    blue32<+1, ODD>(v, Pb, tab, T, lane);
+    // Z(b') for b < nB through the transpose buffer (row q holds register q
+    // of the 64 lanes): b' = Pb - 1 - b (MW) or Pb - b (MW_SS, b > 0), so
+    // entry j of a lane sits at (c >> 5 - j) 64 + 32 h + (c & 31),
+    // c = b' + 32 j
+    const int c = ODD ? Pb - 1 - l : Pb - l;
+    const int zb = (c >> 5) * 64 + 32 * h + (c & 31);
+    auto z_at = [&](int j) { return (!ODD && j == 0 && l == 0) ? 32 * h : zb - 64 * j; };
     double zr[kQB];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) T[lane + 64 * q] = v[q].x;
+    for (int q = 0; q < 16; ++q) T[q * 64 + lane] = v[q].x;
     __builtin_amdgcn_wave_barrier();
-    phase_fence();
+    r32_fence();
+#pragma unroll
+    for (int j = 0; j < kQB; ++j) zr[j] = T[max(z_at(j), 0)];
+    __builtin_amdgcn_wave_barrier();
+    r32_fence();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) T[q * 64 + lane] = v[q].y;
+    __builtin_amdgcn_wave_barrier();
+    r32_fence();
+    const double sA = negA ? -1.0 : 1.0;
 #pragma unroll
     for (int j = 0; j < kQB; ++j) {
-      const int b = lane + 64 * j;
-      const int bp = g.ss ? (b == 0 ? 0 : Pb - b) : Pb - 1 - b;
-      zr[j] = b < g.nB ? T[bp] : 0.0;
-    }
-    __builtin_amdgcn_wave_barrier();
-    phase_fence();
-#pragma unroll
-    for (int q = 0; q < 8; ++q) T[lane + 64 * q] = v[q].y;
-    __builtin_amdgcn_wave_barrier();
-    phase_fence();
-    double2 oA[kQB], oB[kQB];
-#pragma unroll
-    for (int j = 0; j < kQB; ++j) {
-      const int b = lane + 64 * j;
-      const int bp = g.ss ? (b == 0 ? 0 : Pb - b) : Pb - 1 - b;
-      const double zi = b < g.nB ? T[bp] : 0.0;
+      const double zi = T[max(z_at(j), 0)];
+      if (l + 32 * j >= g.nB) continue;
       const double2 su = make_double2(v[j].x + sA * zr[j], v[j].y + sA * zi);
       const double2 df = make_double2(v[j].x - sA * zr[j], v[j].y - sA * zi);
-      oA[j] = make_double2(0.5 * su.x, 0.5 * su.y);
-      oB[j] = make_double2(0.5 * df.y, -0.5 * df.x);
+      double2 *row = row_of(j);
+      if (stA) row[cA] = dA ? make_double2(0.5 * su.x, 0.5 * su.y) : make_double2(0.0, 0.0);
+      if (stB) row[cB] = dB ? make_double2(0.5 * df.y, -0.5 * df.x) : make_double2(0.0, 0.0);
     }
-    __syncthreads();  // every wavefront is done with its transpose buffer
-#pragma unroll
-    for (int j = 0; j < kQB; ++j) {
-      const int b = lane + 64 * j;
-      if (b < g.nB) {
-        tile[b * LD + 2 * wv] = oA[j];
-        tile[b * LD + 2 * wv + 1] = oB[j];
-      }
-    }
-    __syncthreads();
-    {
-      // a thread keeps column c = threadIdx.x % C, rows b = threadIdx.x / C + i 64 W / C
-      // (the MW_SS Nyquist column m = -L holds no data: exact zeros)
-      const int c = threadIdx.x % C, m = m0 + c;
-      if (m <= L - 1) {
-        double2 *col = X2 + m_slot_of(g, m);
-        const bool data = m >= -(L - 1);
-        for (int b = threadIdx.x / C; b < g.nB; b += 64 * W / C)
-          col[x2_row(g, nslot, b) * g.ldA] = data ? tile[b * LD + c] : make_double2(0.0, 0.0);
-      }
-    }
-    __syncthreads();  // the tile is read out before the next group's transposes
+    __builtin_amdgcn_wave_barrier();
+    r32_fence();  // the buffer is read before the next group's transposes
   }
 }
 
@@ -883,129 +912,142 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) 
 // The pole samples (b = L-1 for MW, b = 0 and L for MW_SS) of the
 // antisymmetric member break its antisymmetry; they are left out of the
 // pair and added back through their folded responses (pole table).
-template <int W>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_fwd_beta_pair(
+// On the two-sequence register FFT, half-wavefront h of wavefront w takes the pair
+// (mA, mA + 1), mA = m0 + 4 w + 2 h.  In layout P the remap to the
+// convolution index keeps m'' >= 0 in the lane's own registers 0..7 and
+// gathers m'' < 0 (registers 24..31) through the transpose buffer; the fold
+// G(m') + s G(-m') takes G(m') from the lane's own registers and G(-m')
+// through the buffer.  The even weight spectrum is real and even, and is
+// read from global memory (L1): LDS holds the Bluestein tables and eight
+// transpose buffers.  No workgroup barrier.  ODD: Pb odd (MW).
+template <int ODD>
+__global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_fwd_beta_r32(
     Geo g, FftDesc fb, const double2 *__restrict__ X2, double2 *__restrict__ Gp,
     const int64_t *__restrict__ nbase, const double2 *__restrict__ phase, const double2 *__restrict__ wre,
     const double2 *__restrict__ pole) {
-  constexpr int C = 2 * W;
+  constexpr int C = 4 * kRows32Waves;  // m per group
   constexpr double kPi = 3.141592653589793238462643383279502884;
   extern __shared__ double2 lds[];
-  const int wv = threadIdx.x >> 6;
-  double2 *tw = lds;
-  double *T = (double *)(lds + kRegBlueTab) + wv * kRegT;
-  reg_stage_blue<-1>(tw, fb);
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  double2 *tab = lds;
+  double *T = (double *)(lds + kR32BlueTab) + wv * kR32T;
+  r32_stage_blue<-1>(tab, fb);
   __syncthreads();  // tables staged; no other barrier: the wavefronts run independently
-  const int L = g.L, Pb = g.Pb;
+  const int L = g.L, Pb = g.Pb;  // 129 <= L <= 256
   const int mlo = -(L - 1) - g.ss;
   const int chunks = (g.nA + C - 1) / C;
   const int64_t groups = (int64_t)g.nG * chunks;
   const int C2 = 2 * g.NS;
   for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
-    const int lane = loop_lane();
     const int nslot = (int)(t / chunks);
-    const int m0 = mlo + (int)(t % chunks) * C;
     const int n = n_of_slot(g, nslot);
     if (nbase[n + g.N - 1] == kNoBlock) continue;  // block not transformed
-    const int mA = m0 + 2 * wv, mB = mA + 1;
-    const bool negA = ((mA + n) & 1) != 0;
+    const int lane = loop_lane(), l = lane & 31, h = lane >> 5;
+    const int m0 = mlo + (int)(t % chunks) * C + 4 * wv;
+    const int mA = m0 + 2 * h, mB = mA + 1;
+    const bool negA = ((m0 + n) & 1) != 0;  // s_A = -1 (then s_B = +1); wave-uniform
     const double sA = negA ? -1.0 : 1.0;
-    // the beta samples of the pair straight from X2: the pairs of a
-    // workgroup share every 128-byte line, so most loads hit in L1
-    const bool hA = mA >= -(L - 1) && mA <= L - 1, hB = mB >= -(L - 1) && mB <= L - 1;
-    const double2 *colA = X2 + (hA ? m_slot_of(g, mA) : 0), *colB = X2 + (hB ? m_slot_of(g, mB) : 0);
-    // pole samples of the antisymmetric member (the same address in every lane)
+    // a member outside [-(L-1), L-1] reads an in-range column (the extension
+    // imposes its symmetry; its outputs are not stored)
+    auto fixm = [&](int m) { return m < -(L - 1) ? m + 2 : m > L - 1 ? (L - 1) - ((m - (L - 1)) & 1) : m; };
+    const double2 *colA = X2 + m_slot_of(g, fixm(mA)), *colB = X2 + m_slot_of(g, fixm(mB));
+    // pole samples of the antisymmetric member (one address per half-wavefront)
     const double2 *colX = negA ? colA : colB;
-    const bool hX = negA ? hA : hB;
-    const double2 p0 = hX ? colX[x2_row(g, nslot, g.ss ? 0 : L - 1) * g.ldA] : make_double2(0.0, 0.0);
-    const double2 p1 = hX && g.ss ? colX[x2_row(g, nslot, L) * g.ldA] : make_double2(0.0, 0.0);
-    double2 v[16];
+    const double2 p0 = colX[x2_row(g, nslot, ODD ? L - 1 : 0) * g.ldA];
+    const double2 p1 = ODD ? make_double2(0.0, 0.0) : colX[x2_row(g, nslot, L) * g.ldA];
+    double2 v[32];
+    // extended samples: MW F(2L-2-b) for b >= L, MW_SS F(2L-b) for b > L;
+    // entries k >= Pb only need to be finite (the chirp clears them)
+    auto prep = [&](auto NEG) {
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int k = lane + 64 * q;
-      double2 x = make_double2(0.0, 0.0);
-      if (q < 8 && k < Pb) {  // Pb <= 512
-        // MW: F(2L-2-b) for b >= L; MW_SS: F(2L-b) for b > L
-        const bool direct = g.ss ? k <= L : k < L;
-        const int kr = direct ? k : (g.ss ? 2 * L - k : 2 * L - 2 - k);
+      for (int q = 0; q < 16; ++q) {
+        const int k = q < 8 ? l + 32 * q : min(l + 32 * q, Pb - 1);
+        const bool direct = q < 4 || (ODD ? k < L : k <= L);
+        const int kr = direct ? k : (ODD ? 2 * L - 2 - k : 2 * L - k);
         const int64_t off = x2_row(g, nslot, kr) * g.ldA;
-        double2 a = hA ? colA[off] : make_double2(0.0, 0.0), b = hB ? colB[off] : make_double2(0.0, 0.0);
+        double2 a = colA[off], b = colB[off];
         if (!direct) {
-          if (negA) a = make_double2(-a.x, -a.y);
+          if (NEG) a = make_double2(-a.x, -a.y);
           else b = make_double2(-b.x, -b.y);
         }
-        const bool pk = g.ss ? (k == 0 || k == L) : (k == L - 1);
-        if (pk) {
-          if (negA) a = make_double2(0.0, 0.0);
+        const bool pk = ODD ? (q >= 4 && k == L - 1) : ((q == 0 && l == 0) || (q >= 4 && k == L));
+        if (pk) {  // the antisymmetric member's pole samples go through the pole table
+          if (NEG) a = make_double2(0.0, 0.0);
           else b = make_double2(0.0, 0.0);
         }
-        x = make_double2(a.x - b.y, a.y + b.x);
+        v[q] = make_double2(a.x - b.y, a.y + b.x);
       }
-      v[q] = x;
-    }
-    blue_reg<-1, 1>(v, Pb, tw, T, lane);
-    // F~(m'') times the phase at m'' mod Pw, |m''| < L, and F~(+-1)
+    };
+    if (negA) prep(std::true_type{});
+    else prep(std::false_type{});
+    blue32<-1, ODD>(v, Pb, tab, T, lane);
+    // remap: u(r) = F~(m'') phase(m''), m'' = r (r <= L-1) or r - 1024
+    // (r >= 1024 - (L-1)), zero otherwise; F~(+-1) for the m' = 0 fold.
+    // Entry s of F~ sits at (s >> 5) 64 + 32 h + (s & 31) of the buffer.
     double2 z1, zm1;
     {
-      double t[16];
+      const int d = l + Pb - 1024;  // s = d + 32 q for r = l + 32 q >= 768
+      const int gb = (d >> 5) * 64 + 32 * h + (d & 31);
+      const int i1 = 32 * h + 1, im1 = ((Pb - 1) >> 5) * 64 + 32 * h + ((Pb - 1) & 31);
+      double gr[8];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) T[lane + 64 * q] = v[q].x;
+      for (int q = 0; q < 16; ++q) T[q * 64 + lane] = v[q].x;
       __builtin_amdgcn_wave_barrier();
-      phase_fence();
+      r32_fence();
 #pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const int r = lane + 64 * q;
-        const int mpp = r <= L - 1 ? r : r - 1024;
-        t[q] = mpp >= -(L - 1) ? T[mpp < 0 ? mpp + Pb : mpp] : 0.0;
+      for (int q = 24; q < 32; ++q) gr[q - 24] = T[gb + 64 * q];
+      const double r1 = T[i1], rm1 = T[im1];
+      __builtin_amdgcn_wave_barrier();
+      r32_fence();
+#pragma unroll
+      for (int q = 0; q < 16; ++q) T[q * 64 + lane] = v[q].y;
+      __builtin_amdgcn_wave_barrier();
+      r32_fence();
+#pragma unroll
+      for (int q = 24; q < 32; ++q) {
+        const int r = l + 32 * q;
+        const bool ok = r >= 1024 - (L - 1);
+        const double2 x = make_double2(gr[q - 24], T[gb + 64 * q]);
+        v[q] = ok ? cmul(x, phase[r - 1024 + L - 1]) : make_double2(0.0, 0.0);
       }
-      const double r1 = T[1], rm1 = T[Pb - 1];
+      z1 = cmul(make_double2(r1, T[i1]), phase[L]);
+      zm1 = cmul(make_double2(rm1, T[im1]), phase[L - 2]);
       __builtin_amdgcn_wave_barrier();
-      phase_fence();
+      r32_fence();
 #pragma unroll
-      for (int q = 0; q < 8; ++q) T[lane + 64 * q] = v[q].y;
-      __builtin_amdgcn_wave_barrier();
-      phase_fence();
-#pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const int r = lane + 64 * q;
-        const int mpp = r <= L - 1 ? r : r - 1024;
-        double2 x = make_double2(0.0, 0.0);
-        if (mpp >= -(L - 1)) x = cmul(make_double2(t[q], T[mpp < 0 ? mpp + Pb : mpp]), phase[mpp + L - 1]);
-        v[q] = x;
+      for (int q = 0; q < 8; ++q) {
+        const int r = l + 32 * q;
+        v[q] = r <= L - 1 ? cmul(v[q], phase[min(r, L - 1) + L - 1]) : make_double2(0.0, 0.0);
       }
-      z1 = cmul(make_double2(r1, T[1]), phase[L]);
-      zm1 = cmul(make_double2(rm1, T[Pb - 1]), phase[L - 2]);
-      __builtin_amdgcn_wave_barrier();
-      phase_fence();
+#pragma unroll
+      for (int q = 8; q < 24; ++q) v[q] = make_double2(0.0, 0.0);
     }
-    // convolution with the even weight taps
-    fft_F<+1>(v, T, tw, tw + 1024, lane);
-#pragma unroll
-    for (int c = 0; c < 16; ++c) v[c] = cmul(v[c], wre[c * 64 + lane]);  // register order
-    fft_G<-1>(v, T, tw, tw + 1024, lane);
+    // convolution with the even weight taps (real spectrum, natural order)
+    fft32<+1>(v, T, tab, lane, [&](int k, double2 &z) { z = cscale(z, wre[l + 32 * k].x); });
+    fft32<-1>(v, T, tab, lane, NoPost());
+    // separate and fold, m' = l + 32 j < L: G(m') is register j of the lane,
+    // G(-m') = G(1024 - m') comes through the buffer (rows 0 and 24..31)
     constexpr double sc = 2.0 * kPi / 1024.0;
-    // separate and fold: m' = lane + 64 j
-    constexpr int kFold = 4;  // L <= 256
-    double zpr[kFold], zmr[kFold];
+    const int zb = 31 * 64 + 32 * h + 32 - l;  // entry of 1024 - m' for l > 0, minus 64 (31 - j)
+    auto zm_at = [&](int j) { return l == 0 ? (j == 0 ? 32 * h : (32 - j) * 64 + 32 * h) : zb - 64 * j; };
+    double zmr[8];
+    T[lane] = v[0].x;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) T[lane + 64 * q] = v[q].x;
+    for (int q = 24; q < 32; ++q) T[q * 64 + lane] = v[q].x;
     __builtin_amdgcn_wave_barrier();
-    phase_fence();
+    r32_fence();
 #pragma unroll
-    for (int j = 0; j < kFold; ++j) {
-      const int mp = lane + 64 * j;
-      zpr[j] = T[mp];
-      zmr[j] = T[(1024 - mp) & 1023];
-    }
+    for (int j = 0; j < 8; ++j) zmr[j] = T[zm_at(j)];
     __builtin_amdgcn_wave_barrier();
-    phase_fence();
+    r32_fence();
+    T[lane] = v[0].y;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) T[lane + 64 * q] = v[q].y;
+    for (int q = 24; q < 32; ++q) T[q * 64 + lane] = v[q].y;
     __builtin_amdgcn_wave_barrier();
-    phase_fence();
-    const bool sAok = mA >= -(L - 1) && mA <= L - 1, sBok = mB >= -(L - 1) && mB <= L - 1;
+    r32_fence();
+    const bool okA = mA >= -(L - 1) && mA <= L - 1, okB = mB >= -(L - 1) && mB <= L - 1;
     const int an = n < 0 ? -n : n, sn = n < 0;
-    // G' row of each member (wave-uniform) when it has a single combination
+    // G' row of each member when it has a single combination (sign m, sign n)
     bool one[2], flip[2];
     double2 *row[2];
 #pragma unroll
@@ -1017,10 +1059,12 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(3, 3))) 
       row[e] = Gp + (((int64_t)an * L + am) * C2 + sm + 2 * sn) * L;
     }
 #pragma unroll
-    for (int j = 0; j < kFold; ++j) {
-      const int mp = lane + 64 * j;
-      const double2 zp = make_double2(zpr[j] * sc, T[mp] * sc);
-      const double2 zm = make_double2(zmr[j] * sc, T[(1024 - mp) & 1023] * sc);
+    for (int j = 0; j < 8; ++j) {
+      const int mp = l + 32 * j;
+      const double zmi = T[zm_at(j)];
+      if (mp >= L) continue;
+      const double2 zp = make_double2(v[j].x * sc, v[j].y * sc);
+      const double2 zm = make_double2(zmr[j] * sc, zmi * sc);
       double2 GA, GB;
       if (mp > 0) {
         GA = make_double2(zp.x + sA * zm.x, zp.y + sA * zm.y);
@@ -1036,17 +1080,15 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(3, 3))) 
           GB = make_double2(zp.y, -zp.x);
         }
       }
-      if (mp < L) {
-        double2 corr = cmul(p0, pole[mp]);
-        if (g.ss) corr = cadd(corr, cmul(p1, pole[L + mp]));
-        if (negA) GA = cadd(GA, corr);
-        else GB = cadd(GB, corr);
-      }
-      if (mp >= L) continue;
+      double2 corr = cmul(p0, pole[mp]);
+      if (!ODD) corr = cadd(corr, cmul(p1, pole[L + mp]));
+      if (negA) GA = cadd(GA, corr);
+      else GB = cadd(GB, corr);
+#pragma unroll
       for (int e = 0; e < 2; ++e) {
-        if (!(e ? sBok : sAok)) continue;
+        if (!(e ? okB : okA)) continue;
         const double2 G0 = e ? GB : GA;
-        if (one[e]) {  // |m|, |n| > 0: the one combination (sign m, sign n)
+        if (one[e]) {
           row[e][mp] = (flip[e] && (mp & 1)) ? make_double2(-G0.x, -G0.y) : G0;
           continue;
         }
@@ -1062,6 +1104,8 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(3, 3))) 
         }
       }
     }
+    __builtin_amdgcn_wave_barrier();
+    r32_fence();  // the buffer is read before the next group's transposes
   }
 }
 
@@ -1444,11 +1488,14 @@ void launch_inv_beta(const Geo &g, const double2 *X1, double2 *X2, NBase nb, con
                      hipStream_t st) {
   const FftDesc fd = fft_plan(g.Pb);
   if (fd.len == 1024 && fd.blue && g.nB <= 257) {
-    constexpr int W = kBetaW;
-    const size_t sh = reg_shared<W>(kRegBlueTab, (int64_t)g.nB * (2 * W + 1));
-    const int chunks = (g.nA + 2 * W - 1) / (2 * W);
-    const unsigned grid = reg_grid(k_inv_beta_pair<W>, 64 * W, sh, (int64_t)g.nG * chunks);
-    k_inv_beta_pair<W><<<grid, 64 * W, sh, st>>>(g, fd, X1, X2, nb.base, phase);
+    constexpr int C = 4 * kRows32Waves;
+    const size_t sh = sizeof(double2) * kR32BlueTab + sizeof(double) * kRows32Waves * kR32T;
+    const int chunks = (g.nA + C - 1) / C;
+    auto go = [&](auto kern) {
+      const unsigned grid = reg_grid(kern, 64 * kRows32Waves, sh, (int64_t)g.nG * chunks);
+      kern<<<grid, 64 * kRows32Waves, sh, st>>>(g, fd, X1, X2, nb.base, phase);
+    };
+    (g.Pb & 1) ? go(k_inv_beta_r32<1>) : go(k_inv_beta_r32<0>);
     SO3_LAUNCH_CHECK();
     return;
   }
@@ -1505,8 +1552,16 @@ bool reg_order(const Geo &g) {
 void launch_gamma_x2(const Geo &g, double2 *X2, int dir, double scale, hipStream_t st) {
   const FftDesc fd = fft_plan(2 * g.N - 1);
   if (!reg_order(g)) throw HipError("gamma in place on X2 needs the register path");
-  if (dir > 0) launch_gamma_tile<kGinv>(g, fd, X2, X2, scale, st);
-  else launch_gamma_tile<kGfwd>(g, fd, X2, X2, scale, st);
+  {
+    const size_t sh = sizeof(double2) * kR32BlueTab + sizeof(double) * kRows32Waves * kR32T;
+    const int64_t S = (int64_t)g.nB * g.nA;
+    auto go = [&](auto kern) {
+      const unsigned grid = reg_grid(kern, 64 * kRows32Waves, sh, (S + 2 * kRows32Waves - 1) / (2 * kRows32Waves));
+      kern<<<grid, 64 * kRows32Waves, sh, st>>>(g, fd, X2, scale);
+    };
+    dir > 0 ? go(k_gamma_r32<+1>) : go(k_gamma_r32<-1>);
+    SO3_LAUNCH_CHECK();
+  }
 }
 
 void launch_alpha_x2f(const Geo &g, const double2 *X2, double2 *f, int dir, double scale, hipStream_t st) {
@@ -1580,16 +1635,19 @@ void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, double scale, hi
 }
 
 void launch_fwd_beta(const Geo &g, const double2 *X2, double2 *Gp, NBase nb, const double2 *phase,
-                     const double2 *wr, const double2 *wr_even, const double2 *wr_even_reg,
+                     const double2 *wr, const double2 *wr_even,
                      const double2 *pole, int Pw, int adjoint, hipStream_t st) {
-  if (!adjoint && wr_even_reg && pole && Pw == 1024 && g.L <= 256 && fft_plan(g.Pb).len == 1024 &&
+  if (!adjoint && wr_even && pole && Pw == 1024 && g.L <= 256 && fft_plan(g.Pb).len == 1024 &&
       fft_plan(g.Pb).blue) {
     const FftDesc fb = fft_plan(g.Pb);
-    constexpr int W = kBetaFW;
-    const size_t sh = reg_shared<W>(kRegBlueTab, 0);
-    const int chunks = (g.nA + 2 * W - 1) / (2 * W);
-    const unsigned grid = reg_grid(k_fwd_beta_pair<W>, 64 * W, sh, (int64_t)g.nG * chunks);
-    k_fwd_beta_pair<W><<<grid, 64 * W, sh, st>>>(g, fb, X2, Gp, nb.base, phase, wr_even_reg, pole);
+    constexpr int C = 4 * kRows32Waves;
+    const size_t sh = sizeof(double2) * kR32BlueTab + sizeof(double) * kRows32Waves * kR32T;
+    const int chunks = (g.nA + C - 1) / C;
+    auto go = [&](auto kern) {
+      const unsigned grid = reg_grid(kern, 64 * kRows32Waves, sh, (int64_t)g.nG * chunks);
+      kern<<<grid, 64 * kRows32Waves, sh, st>>>(g, fb, X2, Gp, nb.base, phase, wr_even, pole);
+    };
+    (g.Pb & 1) ? go(k_fwd_beta_r32<1>) : go(k_fwd_beta_r32<0>);
     SO3_LAUNCH_CHECK();
     return;
   }

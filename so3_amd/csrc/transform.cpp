@@ -37,7 +37,6 @@ struct Plan {
   double2 *phase_af = nullptr;   // 2L-1: exp(+i m' pi/(2L-1)) / Pb, adjoint of the forward
   double2 *wr = nullptr;         // Pw entries
   double2 *wr_even = nullptr;    // Pw entries: the even taps of the weights only
-  double2 *wr_even_reg = nullptr;  // the same in the register FFT's order (Pw = 1024)
   double2 *pole = nullptr;       // 2L entries: folded responses of the pole samples
   int64_t *nbase = nullptr;      // 2*(2N-1): masked, then unmasked
   size_t x1 = 0, x2 = 0;         // workspace sizes (double2)
@@ -168,18 +167,6 @@ Plan build_plan(const so3_parameters_t *p, Method method) {
     }
     P.wr = upload(wr);
     P.wr_even = upload(wre);
-    // and in the register order of the length-1024 register FFT (entry
-    // c * 64 + lane holds index (lane >> 2) + 16 c + 256 r(lane & 3), r the
-    // 2-bit reversal)
-    if (Pw == 1024) {
-      std::vector<double2> reg(Pw);
-      for (int c = 0; c < 16; ++c)
-        for (int lane = 0; lane < 64; ++lane) {
-          const int a = lane & 3, r = ((a & 1) << 1) | (a >> 1);
-          reg[c * 64 + lane] = wre[(lane >> 2) + 16 * c + 256 * r];
-        }
-      P.wr_even_reg = upload(reg);
-    }
     // Pole samples (b = L-1 for MW; b = 0 and b = L for MW_SS) map to
     // themselves under the extension, so an antisymmetric sequence
     // ((-1)^{m+n} = -1) with a non-zero sample there is not antisymmetric.
@@ -409,7 +396,7 @@ void enqueue_forward_locked(const Plan &P, const so3_parameters_t *p, Method met
     timed(5, st, [&] { launch_alpha(g, X2, -1, 1.0 / g.nA, st); });
   }
   timed(6, st, [&] {
-    launch_fwd_beta(g, X2, Gp, nb, P.phase_fwd, P.wr, P.wr_even, P.wr_even_reg, P.pole, P.Pw, 0, st);
+    launch_fwd_beta(g, X2, Gp, nb, P.phase_fwd, P.wr, P.wr_even, P.pole, P.Pw, 0, st);
   });
   timed(7, st, [&] {
     launch_fwd_zero(g, (double2 *)flmn, nb, method == kDirect, 0, (int)p->storage, (int)p->n_order, st);
@@ -493,7 +480,7 @@ void run_adjoint_inverse(const so3_parameters_t *p, const void *f, void *flmn, h
     launch_fwd_gamma(g, f, X2, 1.0, st);
     launch_alpha(g, X2, -1, 1.0, st);
   }
-  launch_fwd_beta(g, X2, Gp, nb, P.phase_ai, P.wr, nullptr, nullptr, nullptr, P.Pw, 1, st);
+  launch_fwd_beta(g, X2, Gp, nb, P.phase_ai, P.wr, nullptr, nullptr, P.Pw, 1, st);
   launch_fwd_zero(g, (double2 *)flmn, nb, 1, 0, (int)p->storage, (int)p->n_order, st);
   g.clmode = 2;
   launch_contract_fwd(g, Gp, nb, dl, (double2 *)flmn, st);
@@ -531,7 +518,6 @@ void release_all() {
     (void)hipFree(kv.second.phase_af);
     (void)hipFree(kv.second.wr);
     (void)hipFree(kv.second.wr_even);
-    (void)hipFree(kv.second.wr_even_reg);
     (void)hipFree(kv.second.pole);
     (void)hipFree(kv.second.nbase);
   }
