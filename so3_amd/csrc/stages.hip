@@ -1023,7 +1023,14 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
       for (int q = 8; q < 24; ++q) v[q] = make_double2(0.0, 0.0);
     }
     // convolution with the even weight taps (real spectrum, natural order)
-    fft32<+1>(v, T, tab, lane, [&](int k, double2 &z) { z = cscale(z, wre[l + 32 * k].x); });
+    // (the first of the four output groups of the last DFT-32, k % 4 == 0,
+    // is loaded before the transform, whose first stages hide the latency)
+    double wk[8];
+#pragma unroll
+    for (int k2 = 0; k2 < 8; ++k2) wk[k2] = wre[l + 128 * k2].x;
+    fft32<+1>(v, T, tab, lane, [&](int k, double2 &z) {
+      z = cscale(z, (k & 3) == 0 ? wk[k >> 2] : wre[l + 32 * k].x);
+    });
     fft32<-1>(v, T, tab, lane, NoPost());
     // separate and fold, m' = l + 32 j < L: G(m') is register j of the lane,
     // G(-m') = G(1024 - m') comes through the buffer (rows 0 and 24..31)
