@@ -367,6 +367,9 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
   const int S = g.nB * g.nA, nG = g.nG;
   const int64_t groups = (S + 2 * kRows32Waves - 1) / (2 * kRows32Waves);
   for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
+    // one barrier per group keeps the wavefronts that share 128-byte lines
+    // in step, so a line is still in L2 when its last user reads it
+    __syncthreads();
     const int p0 = (int)(t * kRows32Waves + wv) * 2;
     if (p0 >= S) break;  // the whole wavefront, last group
     const int lane = loop_lane(), l = lane & 31;
@@ -790,7 +793,7 @@ __device__ __forceinline__ int m_slot_of(const Geo &g, int m) { return m >= 0 ? 
 // (the |m'| runs are contiguous), Z(b') comes through the wavefront's
 // transpose buffer, and each lane stores its A and B entries straight into
 // X2 (the 4 W consecutive m of a group fill 4 W consecutive entries of each
-// X2 row), with no workgroup barrier.  ODD: Pb odd (MW).
+// X2 row).  ODD: Pb odd (MW).
 template <int ODD>
 __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_inv_beta_r32(
     Geo g, FftDesc fd, const double2 *__restrict__ X1, double2 *__restrict__ X2,
@@ -809,6 +812,9 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
   const int64_t groups = (int64_t)g.nG * chunks;
   const int nBf = g.nB & ~(kX2Blk - 1);  // rows b < nBf lie in full X2 blocks
   for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
+    // one barrier per group keeps the wavefronts that share 128-byte lines
+    // in step (their partial-line stores meet in L2)
+    __syncthreads();
     const int lane = loop_lane(), l = lane & 31, h = lane >> 5;
     const int nslot = (int)(t / chunks);
     const int n = n_of_slot(g, nslot);
@@ -919,7 +925,7 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
 // G(m') + s G(-m') takes G(m') from the lane's own registers and G(-m')
 // through the buffer.  The even weight spectrum is real and even, and is
 // read from global memory (L1): LDS holds the Bluestein tables and eight
-// transpose buffers.  No workgroup barrier.  ODD: Pb odd (MW).
+// transpose buffers.  ODD: Pb odd (MW).
 template <int ODD>
 __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_fwd_beta_r32(
     Geo g, FftDesc fb, const double2 *__restrict__ X2, double2 *__restrict__ Gp,
@@ -932,13 +938,16 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
   double2 *tab = lds;
   double *T = (double *)(lds + kR32BlueTab) + wv * kR32T;
   r32_stage_blue<-1>(tab, fb);
-  __syncthreads();  // tables staged; no other barrier: the wavefronts run independently
+  __syncthreads();  // tables staged
   const int L = g.L, Pb = g.Pb;  // 129 <= L <= 256
   const int mlo = -(L - 1) - g.ss;
   const int chunks = (g.nA + C - 1) / C;
   const int64_t groups = (int64_t)g.nG * chunks;
   const int C2 = 2 * g.NS;
   for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
+    // one barrier per group keeps the wavefronts that share 128-byte lines
+    // in step, so a line is still in L2 when its last user reads it
+    __syncthreads();
     const int nslot = (int)(t / chunks);
     const int n = n_of_slot(g, nslot);
     if (nbase[n + g.N - 1] == kNoBlock) continue;  // block not transformed
