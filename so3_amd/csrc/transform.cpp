@@ -75,7 +75,8 @@ Geo make_geo(const so3_parameters_t *p, Method method) {
   g.ss = (method == kViaSsht && p->sampling_scheme == SO3_SAMPLING_MW_SS) ? 1 : 0;
   g.real = p->reality ? 1 : 0;
   g.nA = g.ss ? 2 * g.L : 2 * g.L - 1;
-  // X2 rows unpadded (padding rows to 8 entries was measured slower overall)
+  // X2 rows unpadded: padding rows to 8 entries (a power-of-two stride at
+  // L = 256) was measured slower overall, the gamma columns most
   g.ldA = g.nA;
   g.nB = g.ss ? g.L + 1 : g.L;
   g.nG = g.real ? g.N : 2 * g.N - 1;
