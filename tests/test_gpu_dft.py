@@ -44,3 +44,31 @@ def test_dft(n, direction):
     ref = np.fft.fft(x, axis=1) if direction < 0 else np.fft.ifft(x, axis=1) * n
     err = np.abs(y - ref).max() / np.abs(ref).max()
     assert err < 5e-15 * max(1.0, np.log2(n)), err
+
+
+# Lengths with a Bluestein inner length of 1024 (257 <= n <= 512) run on the
+# two-sequence register FFT, two rows per wavefront: odd and even n (the
+# chirp's half table has sign (-1)^n), and row counts that leave the last
+# wavefront with one row (1, 17) or a whole group (2).
+@pytest.mark.parametrize("n", [257, 258, 300, 383, 384, 401, 447, 500, 509, 510])
+@pytest.mark.parametrize("howmany", [1, 2, 17])
+@pytest.mark.parametrize("direction", [-1, 1])
+def test_dft_register_lengths(n, howmany, direction):
+    lib()
+    h = _hip()
+    rng = np.random.default_rng(1000 * n + howmany)
+    x = rng.standard_normal((howmany, n)) + 1j * rng.standard_normal((howmany, n))
+    x = np.ascontiguousarray(x)
+    d = ctypes.c_void_p()
+    assert h.hipMalloc(ctypes.byref(d), x.nbytes) == 0
+    try:
+        assert h.hipMemcpy(d, x.ctypes.data, x.nbytes, 1) == 0
+        A.dft_device(d.value, n, howmany, direction)
+        assert h.hipDeviceSynchronize() == 0
+        y = np.empty_like(x)
+        assert h.hipMemcpy(y.ctypes.data, d, x.nbytes, 2) == 0
+    finally:
+        h.hipFree(d)
+    ref = np.fft.fft(x, axis=1) if direction < 0 else np.fft.ifft(x, axis=1) * n
+    err = np.abs(y - ref).max() / np.abs(ref).max()
+    assert err < 5e-15 * np.log2(n), err

@@ -1,4 +1,5 @@
-"""Full-size (L = N = 256) transforms on the GPU against brute-force synthesis.
+"""Full-size (L = N = 256 and 129 <= L, N <= 256) transforms on the GPU
+against brute-force synthesis.
 
 At this size the DFT stages run their register-resident 1024-point paths
 (Bluestein inner length of 2L - 1 = 511), which the small parity cases do not
@@ -41,13 +42,29 @@ REAL_COEFS = [(255, 255, 255), (255, -3, 254), (200, -17, 33), (128, 0, 0), (97,
               (1, 1, 1), (254, -200, 7), (60, 5, 60)]
 
 
-@pytest.mark.parametrize("real,ss", [(0, 0), (1, 0), (0, 1), (1, 1)])
-def test_l256_sparse_inverse_and_round_trip(oracle, real, ss):
-    L = N = 256
+def _fit(coefs, L, N, real):
+    """The coefficient list mapped into 0 <= l < L, |m| <= l, |n| <= min(l, N-1)."""
+    out = []
+    for (l, m, n) in coefs:
+        l = min(l, L - 1)
+        m = max(-l, min(l, m))
+        nn = min(abs(n), l, N - 1)
+        n = nn if (real or n >= 0) else -nn
+        if (l, m, n) not in out:
+            out.append((l, m, n))
+    return out
+
+
+# L = N = 256 and sizes with every Bluestein length between 257 and 512 on the
+# register FFT: alpha and beta of 2L - 1 (MW) or 2L (MW_SS), gamma of 2N - 1
+@pytest.mark.parametrize("L,N,real,ss", [(256, 256, 0, 0), (256, 256, 1, 0), (256, 256, 0, 1), (256, 256, 1, 1),
+                                         (150, 140, 0, 0), (170, 131, 0, 1), (129, 129, 0, 0),
+                                         (211, 180, 1, 0)])
+def test_sparse_inverse_and_round_trip(oracle, L, N, real, ss):
     p = P.SO3Parameters(L=L, N=N, reality=real, sampling_scheme=ss, n_order=P.SO3_N_ORDER_ZERO_FIRST)
-    rng = np.random.default_rng(7 + real)
+    rng = np.random.default_rng(7 + real + L)
     flmn = np.zeros(P.flmn_size(p), dtype=np.complex128)
-    coefs = REAL_COEFS if real else COEFS
+    coefs = _fit(REAL_COEFS if real else COEFS, L, N, real)
     vals = {}
     for (l, m, n) in coefs:
         v = rng.uniform(-1, 1) + 1j * rng.uniform(-1, 1)
