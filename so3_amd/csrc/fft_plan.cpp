@@ -46,6 +46,32 @@ void host_fft_pow2(std::vector<cld> &x, int sign) {
   }
 }
 
+// in-place FFT of length 2^k or 3 * 2^k in long double: three interleaved
+// power-of-two transforms and one radix-3 combination for the latter
+void host_fft(std::vector<cld> &x, int sign) {
+  const size_t n = x.size();
+  if ((n & (n - 1)) == 0) {
+    host_fft_pow2(x, sign);
+    return;
+  }
+  const size_t m = n / 3;
+  std::vector<cld> part[3];
+  for (int r = 0; r < 3; ++r) {
+    part[r].resize(m);
+    for (size_t j = 0; j < m; ++j) part[r][j] = x[3 * j + r];
+    host_fft_pow2(part[r], sign);
+  }
+  for (size_t k = 0; k < n; ++k) {
+    cld acc(0, 0);
+    for (int r = 0; r < 3; ++r) {
+      cld w = root((long long)(r * k), (long long)n);
+      if (sign > 0) w = std::conj(w);
+      acc += w * part[r][k % m];
+    }
+    x[k] = acc;
+  }
+}
+
 bool factor(int n, std::vector<int> &rad) {
   rad.clear();
   int m = n;
@@ -94,6 +120,9 @@ FftDesc build(int n) {
   } else {
     int M = 1;
     while (M < 2 * n - 1) M *= 2;
+    // 3 * 2^k when that suffices and one workgroup holds it (fewer points
+    // than the next power of two); n in [257, 512] keeps M = 1024
+    if (!(n >= 257 && n <= 512) && M >= 4 && 3 * (M / 4) >= 2 * n - 1 && 3 * (M / 4) <= 1536) M = 3 * (M / 4);
     d.blue = 1;
     d.len = M;
     factor(M, rad);
@@ -110,8 +139,8 @@ FftDesc build(int n) {
         bb[M - j] = chirp[j];
       }
     }
-    host_fft_pow2(bf, -1);
-    host_fft_pow2(bb, -1);
+    host_fft(bf, -1);
+    host_fft(bb, -1);
     d.chirp = upload(chirp);
     d.bhat_f = upload(bf);
     d.bhat_b = upload(bb);
