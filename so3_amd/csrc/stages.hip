@@ -1624,7 +1624,7 @@ void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, double scale, hi
   }
   const int ld = fft_ld(fd);
   const int tb = fd.len * (int)sizeof(double2);
-  const int K = choose_k(ld, fft_capacity(fd), fd.len, tb, 16);
+  const int K = choose_k(ld, fft_capacity(fd), fd.len, tb, 64);
   const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
   set_lds(k_inv_gamma, sh);
   const int64_t S = (int64_t)g.nB * g.nA;
@@ -1641,7 +1641,7 @@ void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, double scale, hi
   }
   const int ld = fft_ld(fd);
   const int tb = fd.len * (int)sizeof(double2);
-  const int K = choose_k(ld, fft_capacity(fd), fd.len, tb, 16);
+  const int K = choose_k(ld, fft_capacity(fd), fd.len, tb, 64);
   const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
   set_lds(k_fwd_gamma, sh);
   const int64_t S = (int64_t)g.nB * g.nA;
