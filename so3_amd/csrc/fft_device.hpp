@@ -265,10 +265,18 @@ __device__ void fft_inner(double2 *X, int S, int ld, const FftDesc &d, const dou
   }
 }
 
-// Copy the twiddle table of d into LDS (call before the first fft_run;
-// the caller synchronises).
-__device__ inline void fft_load_twiddles(double2 *tw_lds, const FftDesc &d) {
+// Twiddle tables of lengths from kTwGlobal up are read from global memory,
+// where they stay in L1/L2: in LDS (32 KB and more) they left room for one
+// workgroup per CU.  tw_count: the table's LDS entries.
+constexpr int kTwGlobal = 2048;
+__host__ __device__ inline int tw_count(const FftDesc &d) { return d.len >= kTwGlobal ? 0 : d.len; }
+
+// Copy the twiddle table of d into LDS, or not (tw_count); returns the table
+// fft_run reads.  Call before the first fft_run; the caller synchronises.
+__device__ inline const double2 *fft_load_twiddles(double2 *tw_lds, const FftDesc &d) {
+  if (d.len >= kTwGlobal) return d.tw;
   for (int k = threadIdx.x; k < d.len; k += kFftThreads) tw_lds[k] = d.tw[k];
+  return tw_lds;
 }
 
 // Full DFT of length d.n in place: input at X[s*ld + j], j < d.n, output

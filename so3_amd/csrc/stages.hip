@@ -41,8 +41,8 @@ __global__ __launch_bounds__(kThreads) void k_inv_beta(Geo g, FftDesc fd, int K,
                                                        const int64_t *__restrict__ nbase,
                                                        const double2 *__restrict__ phase) {
   extern __shared__ double2 lds[];
-  double2 *A = lds, *tw = lds + K * ld;
-  fft_load_twiddles(tw, fd);
+  double2 *A = lds;
+  const double2 *tw = fft_load_twiddles(lds + K * ld, fd);
   const int chunks = (g.nA + K - 1) / K;
   const int nslot = blockIdx.x / chunks;
   const int ms0 = (blockIdx.x % chunks) * K;
@@ -77,8 +77,8 @@ template <int DIR>
 __global__ __launch_bounds__(kThreads) void k_rows(FftDesc fd, int K, int ld, double2 *__restrict__ X,
                                                    int64_t nrows, int n, int64_t rs, double scale) {
   extern __shared__ double2 lds[];
-  double2 *A = lds, *tw = lds + K * ld;
-  fft_load_twiddles(tw, fd);
+  double2 *A = lds;
+  const double2 *tw = fft_load_twiddles(lds + K * ld, fd);
   const int64_t r0 = (int64_t)blockIdx.x * K;
   for (int idx = threadIdx.x; idx < K * n; idx += blockDim.x) {
     const int s = idx / n, k = idx - s * n;
@@ -459,8 +459,8 @@ __global__ __launch_bounds__(kThreads) void k_inv_gamma(Geo g, FftDesc fd, int K
                                                         double2 *__restrict__ fc,
                                                         double *__restrict__ fr, double scale) {
   extern __shared__ double2 lds[];
-  double2 *A = lds, *tw = lds + K * ld;
-  fft_load_twiddles(tw, fd);
+  double2 *A = lds;
+  const double2 *tw = fft_load_twiddles(lds + K * ld, fd);
   // columns p = b nA + a of f; column (b, a) of X2 holds n slot k at
   // x2_row(k, b) ldA + a
   const int64_t S = (int64_t)g.nB * g.nA;
@@ -503,8 +503,8 @@ __global__ __launch_bounds__(kThreads) void k_fwd_gamma(Geo g, FftDesc fd, int K
                                                         const double *__restrict__ fr,
                                                         double2 *__restrict__ X2, double sc) {
   extern __shared__ double2 lds[];
-  double2 *A = lds, *tw = lds + K * ld;
-  fft_load_twiddles(tw, fd);
+  double2 *A = lds;
+  const double2 *tw = fft_load_twiddles(lds + K * ld, fd);
   // columns p = b nA + a of f map to columns (b, a) of X2
   const int64_t S = (int64_t)g.nB * g.nA;
   const int64_t p0 = (int64_t)blockIdx.x * K;
@@ -536,14 +536,14 @@ __global__ __launch_bounds__(kThreads) void k_fwd_beta(Geo g, FftDesc fb, FftDes
                                                        int adjoint) {
   extern __shared__ double2 lds[];
   // one twiddle table serves both transforms when their lengths agree
-  double2 *A = lds, *twb = lds + K * ld, *tww = fw.len == fb.len ? twb : twb + fb.len;
+  double2 *A = lds, *twb = lds + K * ld, *tww = fw.len == fb.len ? twb : twb + tw_count(fb);
   const int chunks = (g.nA + K - 1) / K;
   const int nslot = blockIdx.x / chunks;
   const int ms0 = (blockIdx.x % chunks) * K;
   const int n = n_of_slot(g, nslot);
   if (nbase[n + g.N - 1] == kNoBlock) return;  // block not transformed
-  fft_load_twiddles(twb, fb);
-  if (tww != twb) fft_load_twiddles(tww, fw);
+  const double2 *twbp = fft_load_twiddles(twb, fb);
+  const double2 *twwp = fw.len == fb.len ? twbp : fft_load_twiddles(tww, fw);
   const int Pb = g.Pb, L = g.L;
   for (int idx = threadIdx.x; idx < K * Pb; idx += blockDim.x) {
     const int k = idx / K, s = idx - k * K;
@@ -564,7 +564,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_beta(Geo g, FftDesc fb, FftDes
     A[s * ld + k] = v;
   }
   __syncthreads();
-  fft_run<-1>(A, K, ld, fb, twb);
+  fft_run<-1>(A, K, ld, fb, twbp);
   // F~(m'') = DFT/Pb * phase, placed at m'' mod Pw (zero elsewhere); the
   // remap runs through registers because it widens the sequence in place
   {
@@ -593,13 +593,13 @@ __global__ __launch_bounds__(kThreads) void k_fwd_beta(Geo g, FftDesc fb, FftDes
   __syncthreads();
   double sc = 1.0;
   if (!adjoint) {
-    fft_run<+1>(A, K, ld, fw, tww);
+    fft_run<+1>(A, K, ld, fw, twwp);
     for (int idx = threadIdx.x; idx < K * Pw; idx += blockDim.x) {
       const int s = idx / Pw, r = idx - s * Pw;
       A[s * ld + r] = cmul(A[s * ld + r], wr[r]);
     }
     __syncthreads();
-    fft_run<-1>(A, K, ld, fw, tww);
+    fft_run<-1>(A, K, ld, fw, twwp);
     sc = kTwoPi / (double)Pw;
   }
   const double2 *R3 = A;
@@ -1135,8 +1135,8 @@ __global__ __launch_bounds__(kThreads) void k_inv_beta_gp(Geo g, FftDesc fd, int
                                                           const int64_t *__restrict__ nbase,
                                                           const double2 *__restrict__ phase) {
   extern __shared__ double2 lds[];
-  double2 *A = lds, *tw = lds + K * ld;
-  fft_load_twiddles(tw, fd);
+  double2 *A = lds;
+  const double2 *tw = fft_load_twiddles(lds + K * ld, fd);
   const int L = g.L, Pb = g.Pb, C = 2 * K;
   const int mlo = -(L - 1) - g.ss;
   const int chunks = (g.nA + C - 1) / C;
@@ -1197,8 +1197,8 @@ __global__ __launch_bounds__(kThreads) void k_fwd_beta_gp(Geo g, FftDesc fb, Fft
   extern __shared__ double2 lds[];
   // one twiddle table serves both transforms when their lengths agree;
   // F~(+-1) of every pair after the twiddles
-  double2 *A = lds, *twb = lds + K * ld, *tww = fw.len == fb.len ? twb : twb + fb.len;
-  double2 *zpm = (fw.len == fb.len ? twb + fb.len : tww + fw.len);
+  double2 *A = lds, *twb = lds + K * ld, *tww = fw.len == fb.len ? twb : twb + tw_count(fb);
+  double2 *zpm = (fw.len == fb.len ? twb + tw_count(fb) : tww + tw_count(fw));
   const int L = g.L, Pb = g.Pb, C = 2 * K;
   const int mlo = -(L - 1) - g.ss;
   const int chunks = (g.nA + C - 1) / C;
@@ -1206,8 +1206,8 @@ __global__ __launch_bounds__(kThreads) void k_fwd_beta_gp(Geo g, FftDesc fb, Fft
   const int m0 = mlo + (blockIdx.x % chunks) * C;
   const int n = n_of_slot(g, nslot);
   if (nbase[n + g.N - 1] == kNoBlock) return;  // block not transformed
-  fft_load_twiddles(twb, fb);
-  if (tww != twb) fft_load_twiddles(tww, fw);
+  const double2 *twbp = fft_load_twiddles(twb, fb);
+  const double2 *twwp = fw.len == fb.len ? twbp : fft_load_twiddles(tww, fw);
   for (int idx = threadIdx.x; idx < K * Pb; idx += blockDim.x) {
     const int k = idx / K, s = idx - k * K;
     const int mA = m0 + 2 * s, mB = mA + 1;
@@ -1231,7 +1231,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_beta_gp(Geo g, FftDesc fb, Fft
     A[s * ld + k] = make_double2(a.x - b.y, a.y + b.x);
   }
   __syncthreads();
-  fft_run<-1>(A, K, ld, fb, twb);
+  fft_run<-1>(A, K, ld, fb, twbp);
   for (int s = threadIdx.x; s < K; s += blockDim.x) {
     zpm[2 * s] = cmul(A[s * ld + 1], phase[L]);
     zpm[2 * s + 1] = cmul(A[s * ld + Pb - 1], phase[L - 2]);
@@ -1263,13 +1263,13 @@ __global__ __launch_bounds__(kThreads) void k_fwd_beta_gp(Geo g, FftDesc fb, Fft
   }
   __syncthreads();
   // convolution with the even weight taps
-  fft_run<+1>(A, K, ld, fw, tww);
+  fft_run<+1>(A, K, ld, fw, twwp);
   for (int idx = threadIdx.x; idx < K * Pw; idx += blockDim.x) {
     const int s = idx / Pw, r = idx - s * Pw;
     A[s * ld + r] = cmul(A[s * ld + r], wre[r]);
   }
   __syncthreads();
-  fft_run<-1>(A, K, ld, fw, tww);
+  fft_run<-1>(A, K, ld, fw, twwp);
   const double sc = kTwoPi / (double)Pw;
   const int C2 = 2 * g.NS;
   const int an = n < 0 ? -n : n, sn = n < 0;
@@ -1330,15 +1330,15 @@ __global__ __launch_bounds__(kThreads) void k_adjf_beta(Geo g, FftDesc fb, FftDe
                                                         const double2 *__restrict__ wr, int Pw) {
   extern __shared__ double2 lds[];
   // one twiddle table serves both transforms when their lengths agree
-  double2 *A = lds, *twb = lds + K * ld, *tww = fw.len == fb.len ? twb : twb + fb.len;
+  double2 *A = lds, *twb = lds + K * ld, *tww = fw.len == fb.len ? twb : twb + tw_count(fb);
   const int chunks = (g.nA + K - 1) / K;
   const int nslot = blockIdx.x / chunks;
   const int ms0 = (blockIdx.x % chunks) * K;
   const int n = n_of_slot(g, nslot);
   const bool active = nbase[n + g.N - 1] != kNoBlock;
   const int Pb = g.Pb, L = g.L;
-  fft_load_twiddles(twb, fb);
-  if (tww != twb) fft_load_twiddles(tww, fw);
+  const double2 *twbp = fft_load_twiddles(twb, fb);
+  const double2 *twwp = fw.len == fb.len ? twbp : fft_load_twiddles(tww, fw);
   // G(m') for |m'| <= L-1 at m' mod Pw, F(-m') = (-1)^{m+n} F(m')
   for (int idx = threadIdx.x; idx < K * Pw; idx += blockDim.x) {
     const int s = idx / Pw, r = idx - s * Pw;
@@ -1353,13 +1353,13 @@ __global__ __launch_bounds__(kThreads) void k_adjf_beta(Geo g, FftDesc fb, FftDe
     A[s * ld + r] = v;
   }
   __syncthreads();
-  fft_run<+1>(A, K, ld, fw, tww);
+  fft_run<+1>(A, K, ld, fw, twwp);
   for (int idx = threadIdx.x; idx < K * Pw; idx += blockDim.x) {
     const int s = idx / Pw, r = idx - s * Pw;
     A[s * ld + r] = cmul(A[s * ld + r], wr[r]);
   }
   __syncthreads();
-  fft_run<-1>(A, K, ld, fw, tww);
+  fft_run<-1>(A, K, ld, fw, twwp);
   // x 4 pi^2 / Pw, phase exp(+i m' pi/(2L-1)) / Pb, to m' mod Pb (in place
   // through registers: the sequence shrinks from Pw to Pb)
   {
@@ -1388,7 +1388,7 @@ __global__ __launch_bounds__(kThreads) void k_adjf_beta(Geo g, FftDesc fb, FftDe
     }
   }
   __syncthreads();
-  fft_run<+1>(A, K, ld, fb, twb);
+  fft_run<+1>(A, K, ld, fb, twbp);
   // adjoint of the extension: out(b) = E(b) + (-1)^{m+n} E(2L-2-b), b < L-1
   for (int idx = threadIdx.x; idx < K * g.nB; idx += blockDim.x) {
     const int b = idx / K, s = idx - b * K;
@@ -1525,7 +1525,7 @@ void launch_inv_beta(const Geo &g, const double2 *X1, double2 *X2, NBase nb, con
     return;
   }
   const int ld = fft_ld(fd);
-  const int tb = fd.len * (int)sizeof(double2);
+  const int tb = tw_count(fd) * (int)sizeof(double2);
   const int K = choose_k(ld, fft_capacity(fd), fd.len, tb, 8);
   const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
   set_lds(k_inv_beta_gp, sh);
@@ -1598,7 +1598,7 @@ void launch_rows(double2 *X, int64_t nrows, int n, int ldr, int dir, double scal
     return;
   }
   const int ld = fft_ld(fd);
-  const int tb = fd.len * (int)sizeof(double2);
+  const int tb = tw_count(fd) * (int)sizeof(double2);
   const int K = choose_k(ld, fft_capacity(fd), fd.len, tb, 8);
   const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
   const unsigned grid = (unsigned)((nrows + K - 1) / K);
@@ -1623,7 +1623,7 @@ void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, double scale, hi
     return;
   }
   const int ld = fft_ld(fd);
-  const int tb = fd.len * (int)sizeof(double2);
+  const int tb = tw_count(fd) * (int)sizeof(double2);
   const int K = choose_k(ld, fft_capacity(fd), fd.len, tb, 64);
   const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
   set_lds(k_inv_gamma, sh);
@@ -1640,7 +1640,7 @@ void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, double scale, hi
     return;
   }
   const int ld = fft_ld(fd);
-  const int tb = fd.len * (int)sizeof(double2);
+  const int tb = tw_count(fd) * (int)sizeof(double2);
   const int K = choose_k(ld, fft_capacity(fd), fd.len, tb, 64);
   const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
   set_lds(k_fwd_gamma, sh);
@@ -1680,7 +1680,7 @@ void launch_fwd_beta(const Geo &g, const double2 *X2, double2 *Gp, NBase nb, con
   const FftDesc fb = fft_plan(g.Pb);
   const FftDesc fw = fft_plan(Pw);
   const int ld = std::max(fft_ld(fb), fft_ld(fw));
-  const int tb = (fb.len + (fw.len == fb.len ? 0 : fw.len)) * (int)sizeof(double2);
+  const int tb = (tw_count(fb) + (fw.len == fb.len ? 0 : tw_count(fw))) * (int)sizeof(double2);
   if (!adjoint && wr_even && pole) {
     // K pairs per workgroup, F~(+-1) of each after the tables
     const int K = choose_k2(ld, fb, fw, tb + 16 * (int)sizeof(double2), 8);
@@ -1705,7 +1705,7 @@ void launch_adjf_beta(const Geo &g, const double2 *X1, double2 *X2, NBase nb, co
   const FftDesc fb = fft_plan(g.Pb);
   const FftDesc fw = fft_plan(Pw);
   const int ld = std::max(fft_ld(fb), fft_ld(fw));
-  const int tb = (fb.len + (fw.len == fb.len ? 0 : fw.len)) * (int)sizeof(double2);
+  const int tb = (tw_count(fb) + (fw.len == fb.len ? 0 : tw_count(fw))) * (int)sizeof(double2);
   const int K = choose_k2(ld, fb, fw, tb, 8);
   const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
   set_lds(k_adjf_beta, sh);
