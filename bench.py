@@ -189,20 +189,21 @@ def batched_bench(A, P, torch, reps=5):
     }
 
 
-def serving_bench(A, rates=(50, 100, 200), count=120):
+def serving_bench(A, rates=(50, 100, 200, 400, 800), count=120):
     """The mixed request stream (Poisson arrivals) through the serving loop at
     several rates: throughput and p50 / p99 latency per rate."""
     from so3_amd import serve
     A.cache_limit(96 << 30)
     srv = serve.Server(hbm_budget=48 << 30, lanes=4, max_batch=32, streams=3)
-    fn = serve.random_inputs(5)
     out = {"workload": "Poisson arrivals; 70%% fwd+inv pairs (L log-uniform [16,512], N in [1,min(L,32)]), "
                        "30%% s2toso3 + inverse (L log-uniform [64,1024], N in {4,8}); MW complex (seed 5); "
-                       "%d requests per rate; tables and plans resident (built before timing)" % count,
+                       "%d requests per rate; tables and plans resident (built before timing); inputs "
+                       "drawn ahead in pinned host memory, copied to HBM inside the loop" % count,
            "rates": {}}
     for rate in rates:
         reqs = serve.request_mix(count, rate)
         srv.prepare(reqs)
+        fn = serve.pinned_inputs(reqs, serve.random_inputs(5))
         warm = [serve.Request(r.rid, r.kind, r.p, 0.0) for r in reqs[:8]]
         srv.run(warm, fn)
         srv.run(reqs, fn)

@@ -92,12 +92,18 @@ class Server:
         bufs = {"f": f, "flmn_out": out}
         spec = {"p": p, "kind": r.kind, "f": f.data_ptr(), "flmn_out": out.data_ptr()}
         data = input_fn(r)
+
+        def dev(x):  # pinned host tensors copy asynchronously on the current stream
+            if isinstance(x, torch.Tensor):
+                return x.to(self.dev, non_blocking=True)
+            return torch.as_tensor(x, device=self.dev)
+
         if r.kind == A.ITEM_PAIR:
-            bufs["flmn_in"] = torch.as_tensor(data["flmn"], device=self.dev)
+            bufs["flmn_in"] = dev(data["flmn"])
             spec["flmn_in"] = bufs["flmn_in"].data_ptr()
         else:
-            bufs["flm"] = torch.as_tensor(data["flm"], device=self.dev)
-            bufs["glm"] = torch.as_tensor(data["glm"], device=self.dev)
+            bufs["flm"] = dev(data["flm"])
+            bufs["glm"] = dev(data["glm"])
             spec["flmn_in"] = bufs["flm"].data_ptr()
             spec["glm"] = bufs["glm"].data_ptr()
         return spec, bufs
@@ -176,6 +182,19 @@ def random_inputs(device_rng_seed=0):
         return {"flm": rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n),
                 "glm": rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)}
     return fn
+
+
+def pinned_inputs(requests, input_fn):
+    """input_fn over `requests` drawn ahead of time into pinned host memory,
+    as clients hand their data over: the serving loop then only enqueues
+    asynchronous copies (drawing the larger inputs in the loop took longer
+    than transforming them)."""
+    import torch
+    cache = {}
+    for r in requests:
+        if r.rid not in cache:
+            cache[r.rid] = {k: torch.from_numpy(v).pin_memory() for k, v in input_fn(r).items()}
+    return lambda r: cache[r.rid]
 
 
 def latency_stats(requests):
