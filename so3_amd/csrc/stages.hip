@@ -787,6 +787,107 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) 
 // m_lo = -(L-1) - ss (the MW_SS Nyquist column, zero, first).
 __device__ __forceinline__ int m_slot_of(const Geo &g, int m) { return m >= 0 ? m : m + g.nA; }
 
+// Gamma of a real signal between X2 (n slots 0..N-1) and f (2N-1 real
+// samples per column) on the two-sequence register FFT.  The samples are
+// real, so one complex DFT serves two columns A, B: half-wavefront h of
+// wavefront w takes columns p = 4 (t W + w) + 2 h and p + 1.
+//   inverse (c2r): the Hermitian extension of A + i B (slot 0 real, slots
+//     k >= N from conj(slot 2N-1-k)); f_A, f_B are the real and imaginary
+//     parts of the DFT;
+//   forward (r2c): the DFT Y of f_A + i f_B separates into
+//     X_A = (Y(k) + conj Y(-k)) / 2, X_B = (Y(k) - conj Y(-k)) / (2 i),
+//     Y(-k) through the wavefront's transpose buffer.
+// The group barrier keeps the wavefronts that share lines in step (see
+// k_gamma_r32).
+template <int DIR>
+__global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gamma_real_r32(
+    Geo g, FftDesc fd, double2 *X2, double *f, double scale) {
+  extern __shared__ double2 lds[];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  double2 *tab = lds;
+  double *T = (double *)(lds + kR32BlueTab) + wv * kR32T;
+  r32_stage_blue<DIR>(tab, fd);
+  __syncthreads();
+  const int S = g.nB * g.nA, N = g.N, nf = 2 * g.N - 1;  // 257 <= nf <= 511
+  const int64_t groups = (S + 4 * kRows32Waves - 1) / (4 * kRows32Waves);
+  for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
+    __syncthreads();
+    const int lane = loop_lane(), l = lane & 31, h = lane >> 5;
+    const int pA = (int)(t * kRows32Waves + wv) * 4 + 2 * h, pB = pA + 1;
+    const bool okA = pA < S, okB = pB < S;  // a column past the last one repeats it, unstored
+    const int qA = okA ? pA : S - 1, qB = okB ? pB : S - 1;
+    const int bA = qA / g.nA, aA = qA - bA * g.nA, bB = qB / g.nA, aB = qB - bB * g.nA;
+    double2 *cA = X2 + x2_row(g, 0, bA) * g.ldA + aA, *cB = X2 + x2_row(g, 0, bB) * g.ldA + aB;
+    const int64_t sA = (int64_t)x2_w(g, bA) * g.ldA, sB = (int64_t)x2_w(g, bB) * g.ldA;
+    double2 v[32];
+    if (DIR > 0) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int k = min(l + 32 * q, nf - 1);  // past nf: cleared by the chirp
+        const bool dir = k < N;
+        const int kk = dir ? k : nf - k;
+        double2 a = cA[kk * sA], b = cB[kk * sB];
+        if (!dir) a.y = -a.y, b.y = -b.y;
+        if (k == 0) a.y = 0.0, b.y = 0.0;  // slot 0 of a real signal is real
+        v[q] = make_double2(a.x - b.y, a.y + b.x);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int k = min(l + 32 * q, nf - 1);
+        v[q] = make_double2(f[(int64_t)k * S + qA], f[(int64_t)k * S + qB]);
+      }
+    }
+    blue32<DIR, 1>(v, nf, tab, T, lane);  // nf odd
+    if (DIR > 0) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) asm volatile("" : "+v"(v[q].x), "+v"(v[q].y));
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int k = l + 32 * q;
+        if (q < 8 || k < nf) {  // nf > 256
+          if (okA) f[(int64_t)k * S + pA] = v[q].x * scale;
+          if (okB) f[(int64_t)k * S + pB] = v[q].y * scale;
+        }
+      }
+    } else {
+      // Y(-k) = Y(nf - k) for k < N through the buffer (row q holds register q
+      // of the 64 lanes); entry j of a lane sits at (c >> 5 - j) 64 + 32 h + (c & 31),
+      // c = nf - l; k = 0 reads Y(0)
+      const int c = nf - l;
+      const int zb = (c >> 5) * 64 + 32 * h + (c & 31);
+      auto z_at = [&](int j) { return (j == 0 && l == 0) ? 32 * h : zb - 64 * j; };
+      double zr[8];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) T[q * 64 + lane] = v[q].x;
+      __builtin_amdgcn_wave_barrier();
+      r32_fence();
+#pragma unroll
+      for (int j = 0; j < 8; ++j) zr[j] = T[max(z_at(j), 0)];
+      __builtin_amdgcn_wave_barrier();
+      r32_fence();
+#pragma unroll
+      for (int q = 0; q < 16; ++q) T[q * 64 + lane] = v[q].y;
+      __builtin_amdgcn_wave_barrier();
+      r32_fence();
+      const double hs = 0.5 * scale;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {  // k = l + 32 j < N <= 256
+        const double zi = T[max(z_at(j), 0)];
+        const int k = l + 32 * j;
+        if (k >= N) continue;
+        // X_A = (Y + conj Y(-k)) / 2, X_B = (Y - conj Y(-k)) / (2 i)
+        const double2 xa = make_double2(hs * (v[j].x + zr[j]), hs * (v[j].y - zi));
+        const double2 xb = make_double2(hs * (v[j].y + zi), -hs * (v[j].x - zr[j]));
+        if (okA) cA[k * sA] = xa;
+        if (okB) cB[k * sB] = xb;
+      }
+      __builtin_amdgcn_wave_barrier();
+      r32_fence();  // the buffer is read before the next group's transposes
+    }
+  }
+}
+
 // Inverse beta, paired, on the two-sequence register FFT: half-wavefront h
 // of wavefront w takes the pair (mA, mA + 1), mA = m0 + 4 w + 2 h, so a
 // group covers 4 W consecutive m of one n slot.  X1 rows are read straight
@@ -1616,8 +1717,25 @@ void launch_alpha(const Geo &g, double2 *X2, int dir, double scale, hipStream_t 
   launch_rows(X2, (int64_t)g.nG * g.nB, g.nA, g.ldA, dir, scale, st);
 }
 
+// real gamma on the two-sequence register FFT (four columns per wavefront)
+static void launch_gamma_real(const Geo &g, const FftDesc &fd, double2 *X2, double *f, int dir, double scale,
+                              hipStream_t st) {
+  const size_t sh = sizeof(double2) * kR32BlueTab + sizeof(double) * kRows32Waves * kR32T;
+  const int64_t S = (int64_t)g.nB * g.nA;
+  auto go = [&](auto kern) {
+    const unsigned grid = reg_grid(kern, 64 * kRows32Waves, sh, (S + 4 * kRows32Waves - 1) / (4 * kRows32Waves));
+    kern<<<grid, 64 * kRows32Waves, sh, st>>>(g, fd, X2, f, scale);
+  };
+  dir > 0 ? go(k_gamma_real_r32<+1>) : go(k_gamma_real_r32<-1>);
+  SO3_LAUNCH_CHECK();
+}
+
 void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, double scale, hipStream_t st) {
   const FftDesc fd = fft_plan(2 * g.N - 1);
+  if (fd.len == 1024 && fd.blue && g.real) {
+    launch_gamma_real(g, fd, (double2 *)X2, (double *)f, +1, scale, st);
+    return;
+  }
   if (fd.len == 1024 && fd.blue) {
     launch_gamma_tile<kGxF>(g, fd, X2, f, scale, st);
     return;
@@ -1635,6 +1753,10 @@ void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, double scale, hi
 
 void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, double scale, hipStream_t st) {
   const FftDesc fd = fft_plan(2 * g.N - 1);
+  if (fd.len == 1024 && fd.blue && g.real) {
+    launch_gamma_real(g, fd, X2, (double *)f, -1, scale, st);
+    return;
+  }
   if (fd.len == 1024 && fd.blue) {
     launch_gamma_tile<kFxG>(g, fd, f, X2, scale, st);
     return;
