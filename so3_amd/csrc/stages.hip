@@ -228,7 +228,9 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
   }
 }
 
-// Gamma stages, Bluestein length 2N-1 with inner length 1024: a group is
+// Gamma between X2 and f of a complex signal whose alpha stage does not run
+// on the register FFT (the other cases: k_gamma_r32, k_gamma_real_r32),
+// Bluestein length 2N-1 with inner length 1024: a group is
 // kRegWaves consecutive columns p = b nA + a (one per wavefront), moved
 // between HBM and the registers through an LDS tile of nG x kRegWaves
 // entries, so that one access instruction covers 16 n slots x 4 columns: 16
@@ -240,17 +242,14 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
 // the uses).
 __device__ __forceinline__ int gtile(int k, int c) { return 4 * k + (c ^ ((k >> 2) & 3)); }
 
-// Gamma tile kernel modes: where the columns come from and go to.
-//   kGxF  : inverse, X2 -> f        kFxG  : forward, f -> X2
-//   kGinv : inverse, in place on X2 kGfwd : forward, in place on X2 (complex)
-enum GammaMode { kGxF = 0, kFxG = 1, kGinv = 2, kGfwd = 3 };
+// Gamma tile kernel modes: kGxF inverse, X2 -> f; kFxG forward, f -> X2
+enum GammaMode { kGxF = 0, kFxG = 1 };
 
 template <int MODE>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gamma_tile(
     Geo g, FftDesc fd, const void *src, void *dst, double scale) {
-  // src and dst are the same array in the in-place modes
   static_assert(kRegWaves == 4, "gamma tile of 4 columns");
-  constexpr int DIR = (MODE == kGxF || MODE == kGinv) ? +1 : -1;
+  constexpr int DIR = MODE == kGxF ? +1 : -1;
   constexpr int kPF = 8;  // tile entries per thread: nG * 4 <= 2048
   extern __shared__ double2 lds[];
   const int wv = threadIdx.x >> 6;
@@ -260,7 +259,6 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))
   const int S = g.nB * g.nA;  // columns; f has S entries per g
   const int64_t groups = (S + kRegWaves - 1) / kRegWaves;
   const int nG = 2 * g.N - 1;
-  const int nout = (MODE == kFxG && g.real) ? g.N : nG;
   const double2 *X2s = (const double2 *)src;
   double2 *X2d = (double2 *)dst;
   // A thread always handles column c = threadIdx.x % 4 of a group and the n
@@ -283,16 +281,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))
       const int k = k0 + 64 * i;
       double2 x = make_double2(0.0, 0.0);
       if (ok && k < nG) {
-        if constexpr (MODE == kFxG) {
-          x = g.real ? make_double2(((const double *)src)[(int64_t)k * S + p], 0.0)
-                     : ((const double2 *)src)[(int64_t)k * S + p];
-        } else if (MODE == kGxF && g.real) {
-          if (k == 0) x = make_double2(X2s[x0].x, 0.0);
-          else if (k < g.N) x = X2s[x0 + k * xs];
-          else x = cconj(X2s[x0 + (nG - k) * xs]);
-        } else {
-          x = X2s[x0 + k * xs];
-        }
+        if constexpr (MODE == kFxG) x = ((const double2 *)src)[(int64_t)k * S + p];
+        else x = X2s[x0 + k * xs];
       }
       P[i] = x;
     }
@@ -322,7 +312,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
     for (int q = 0; q < 8; ++q) {  // nG <= 512
       const int k = lane + 64 * q;
-      if (k < nout) tile[gtile(k, wv)] = v[q];
+      if (k < nG) tile[gtile(k, wv)] = v[q];
     }
     __syncthreads();
     {
@@ -333,14 +323,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
         for (int i = 0; i < kPF; ++i) {
           const int k = k0 + 64 * i;
-          if (k >= nout) continue;
+          if (k >= nG) continue;
           const double2 x = tile[gtile(k, c)];
-          if constexpr (MODE == kGxF) {
-            if (g.real) ((double *)dst)[(int64_t)k * S + p] = x.x * scale;
-            else ((double2 *)dst)[(int64_t)k * S + p] = cscale(x, scale);
-          } else {
-            X2d[x0 + k * xs] = cscale(x, scale);
-          }
+          if constexpr (MODE == kGxF) ((double2 *)dst)[(int64_t)k * S + p] = cscale(x, scale);
+          else X2d[x0 + k * xs] = cscale(x, scale);
         }
       }
     }
@@ -1594,6 +1580,7 @@ static unsigned reg_grid(Kern k, int threads, size_t lds, int64_t groups) {
 template <int MODE>
 static void launch_gamma_tile(const Geo &g, const FftDesc &fd, const void *src, void *dst, double scale,
                               hipStream_t st) {
+  if (g.real) throw HipError("gamma tile: complex signals only");
   const int64_t S = (int64_t)g.nB * g.nA;
   const size_t sh = reg_shared<kRegWaves>(kRegBlueTab, (int64_t)fd.n * 4);
   const unsigned grid = reg_grid(k_gamma_tile<MODE>, kThreads, sh, (S + kRegWaves - 1) / kRegWaves);

@@ -56,10 +56,12 @@ def _fit(coefs, L, N, real):
 
 
 # L = N = 256 and sizes with every Bluestein length between 257 and 512 on the
-# register FFT: alpha and beta of 2L - 1 (MW) or 2L (MW_SS), gamma of 2N - 1
+# register FFT: alpha and beta of 2L - 1 (MW) or 2L (MW_SS), gamma of 2N - 1;
+# L = 300 runs alpha and beta on the workgroup engine and gamma between X2
+# and f (complex: the gamma tile kernel; real: two columns per DFT)
 @pytest.mark.parametrize("L,N,real,ss", [(256, 256, 0, 0), (256, 256, 1, 0), (256, 256, 0, 1), (256, 256, 1, 1),
                                          (150, 140, 0, 0), (170, 131, 0, 1), (129, 129, 0, 0),
-                                         (211, 180, 1, 0)])
+                                         (211, 180, 1, 0), (300, 140, 0, 0), (300, 150, 1, 0)])
 def test_sparse_inverse_and_round_trip(oracle, L, N, real, ss):
     p = P.SO3Parameters(L=L, N=N, reality=real, sampling_scheme=ss, n_order=P.SO3_N_ORDER_ZERO_FIRST)
     rng = np.random.default_rng(7 + real + L)
