@@ -106,7 +106,7 @@ FftDesc build(int n) {
   // a smooth length runs directly when one workgroup holds it in registers
   // (radix 5 and 7 passes hold fewer points per thread), else via Bluestein;
   // every n in [257, 512] takes Bluestein with M = 1024, which runs on the
-  // register-resident FFT (fft_reg.hpp) faster than the workgroup engine
+  // register-resident FFTs (fft_reg.hpp, fft_reg32.hpp) faster than the workgroup engine
   // runs even a smooth n directly
   bool direct = factor(n, rad) && !(n >= 257 && n <= 512);
   if (direct) {

@@ -1,7 +1,11 @@
 // stages.hip -- DFT stages of the Wigner transform (beta, alpha, gamma).
 //
-// Every stage is one launch of LDS-staged batched DFTs (fft_device.hpp); a
-// workgroup owns K sequences.  Reference counterparts:
+// Every stage is one launch of batched DFTs.  Inner length 1024 (n = 1024,
+// or a Bluestein n in [257, 512]) runs register-resident: one sequence per
+// wavefront (fft_reg.hpp) or two (fft_reg32.hpp, the faster, used wherever
+// the stage is written for it); other lengths run on the LDS-staged
+// workgroup engine (fft_device.hpp), a workgroup owning K sequences.
+// Reference counterparts:
 //   inverse beta : phase exp(i m' pi/(2L-1)), m' < 0 by F(-m') = (-1)^{m+n} F(m'),
 //                  DFT over m' -> b (so3_core.c:1148-1206; SSHT inverse)
 //   alpha        : DFT over m <-> a (so3_core.c:1168-1206, 1335-1364)
