@@ -340,9 +340,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))
 
 // Gamma in place on X2 (complex) on the two-sequence register FFT: the
 // columns p = b nA + a are numbered as in k_gamma_tile; wavefront w of group
-// t takes columns 2 (t W + w) and the next one, one per half-wavefront, and
-// loads and stores their nG n slots straight from X2.  The W wavefronts of a
-// workgroup start together on 2 W consecutive columns, whose entries of one
+// t takes columns 2 (t W + u) and the next one, u = quad(w) (a permutation
+// of the wavefronts, set out at the group barrier), one per half-wavefront,
+// and loads and stores their nG n slots straight from X2.  Each half of the
+// workgroup starts together on W consecutive columns, whose entries of one
 // slot share 128-byte lines, so most of each line is used from L1 instead of
 // being staged through an LDS tile (no room beside eight transpose buffers).
 template <int DIR>
@@ -356,12 +357,23 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
   __syncthreads();
   const int S = g.nB * g.nA, nG = g.nG;
   const int64_t groups = (S + 2 * kRows32Waves - 1) / (2 * kRows32Waves);
+  // One barrier per group keeps the wavefronts of a workgroup within part of
+  // a group of each other.  Even wavefronts pass it before their loads, odd
+  // ones after them, so the two wavefronts of a SIMD are apart by the time
+  // the loads take (6% faster than passing it together).  The even
+  // wavefronts take the first half of the group's columns and the odd ones
+  // the second, so that the users of a 128-byte line pass the barrier
+  // together and find the line in L2.  Every wavefront passes it once per
+  // group.
+  const bool late = (wv & 1) != 0;
+  const int quad = (wv & 1) * (kRows32Waves / 2) + (wv >> 1);
   for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
-    // one barrier per group keeps the wavefronts that share 128-byte lines
-    // in step, so a line is still in L2 when its last user reads it
-    __syncthreads();
-    const int p0 = (int)(t * kRows32Waves + wv) * 2;
-    if (p0 >= S) break;  // the whole wavefront, last group
+    if (!late) __syncthreads();
+    const int p0 = (int)(t * kRows32Waves + quad) * 2;
+    if (p0 >= S) {  // the whole wavefront, last group
+      if (late) __syncthreads();
+      break;
+    }
     const int lane = loop_lane(), l = lane & 31;
     // a missing second column of the last group repeats the first one (both
     // halves then write the same values)
@@ -375,6 +387,7 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
       const int k = l + 32 * q;
       v[q] = col[(q < 8 ? k : min(k, nG - 1)) * xs];  // past nG: cleared by the chirp
     }
+    if (late) __syncthreads();
     blue32<DIR, 1>(v, nG, tab, T, lane);  // nG = 2N - 1 is odd
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
@@ -787,8 +800,9 @@ __device__ __forceinline__ int m_slot_of(const Geo &g, int m) { return m >= 0 ? 
 //   forward (r2c): the DFT Y of f_A + i f_B separates into
 //     X_A = (Y(k) + conj Y(-k)) / 2, X_B = (Y(k) - conj Y(-k)) / (2 i),
 //     Y(-k) through the wavefront's transpose buffer.
-// The group barrier keeps the wavefronts that share lines in step (see
-// k_gamma_r32).
+// The group barrier keeps the wavefronts that share lines in step.  (The
+// staggered barrier of k_gamma_r32 measured slower here: the forward
+// transform lost what the inverse gained.)
 template <int DIR>
 __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_gamma_real_r32(
     Geo g, FftDesc fd, double2 *X2, double *f, double scale) {
@@ -904,7 +918,8 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
   const int nBf = g.nB & ~(kX2Blk - 1);  // rows b < nBf lie in full X2 blocks
   for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
     // one barrier per group keeps the wavefronts that share 128-byte lines
-    // in step (their partial-line stores meet in L2)
+    // in step (their partial-line stores meet in L2); the staggered barrier
+    // of k_fwd_beta_r32 measured 14% slower here
     __syncthreads();
     const int lane = loop_lane(), l = lane & 31, h = lane >> 5;
     const int nslot = (int)(t / chunks);
@@ -1010,7 +1025,8 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
 // antisymmetric member break its antisymmetry; they are left out of the
 // pair and added back through their folded responses (pole table).
 // On the two-sequence register FFT, half-wavefront h of wavefront w takes the pair
-// (mA, mA + 1), mA = m0 + 4 w + 2 h.  In layout P the remap to the
+// (mA, mA + 1), mA = m0 + 4 u + 2 h, u = quad(w) (a permutation of the
+// wavefronts, set out at the group barrier).  In layout P the remap to the
 // convolution index keeps m'' >= 0 in the lane's own registers 0..7 and
 // gathers m'' < 0 (registers 24..31) through the transpose buffer; the fold
 // G(m') + s G(-m') takes G(m') from the lane's own registers and G(-m')
@@ -1035,14 +1051,17 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
   const int chunks = (g.nA + C - 1) / C;
   const int64_t groups = (int64_t)g.nG * chunks;
   const int C2 = 2 * g.NS;
-  // One barrier per group keeps the wavefronts that share 128-byte lines
-  // within half a group of each other, so a line is still in L2 when its
-  // last user reads it.  Even wavefronts pass it before their loads, odd
+  // One barrier per group keeps the wavefronts of a workgroup within half a
+  // group of each other.  Even wavefronts pass it before their loads, odd
   // ones after their Bluestein DFT: the two wavefronts of a SIMD are then
   // half a group apart, and one computes while the other waits for its
-  // loads (4.5% faster than passing it together).  Every wavefront passes it
-  // once per group.
+  // loads.  Every wavefront passes it once per group.  Neighbouring quads of
+  // m share 128-byte lines, so the even wavefronts take the first half of
+  // the group's quads and the odd ones the second: a line's users then pass
+  // the barrier together, and the line is still in L2 when the last of
+  // them reads it.
   const bool late = (wv & 1) != 0;
+  const int quad = (wv & 1) * (kRows32Waves / 2) + (wv >> 1);
   for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
     if (!late) __syncthreads();
     const int nslot = (int)(t / chunks);
@@ -1052,7 +1071,7 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
       continue;
     }
     const int lane = loop_lane(), l = lane & 31, h = lane >> 5;
-    const int m0 = mlo + (int)(t % chunks) * C + 4 * wv;
+    const int m0 = mlo + (int)(t % chunks) * C + 4 * quad;
     const int mA = m0 + 2 * h, mB = mA + 1;
     const bool negA = ((m0 + n) & 1) != 0;  // s_A = -1 (then s_B = +1); wave-uniform
     const double sA = negA ? -1.0 : 1.0;
