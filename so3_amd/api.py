@@ -60,7 +60,7 @@ def inverse(flmn, p, method=VIA_SSHT):
     flmn = np.ascontiguousarray(flmn, dtype=np.complex128)
     if flmn.size < flmn_size(p):
         raise ValueError("flmn has %d entries, needs %d" % (flmn.size, flmn_size(p)))
-    q = p if method == VIA_SSHT else p.replace(sampling_scheme=P.SO3_SAMPLING_MW)
+    q = p if method == VIA_SSHT else p.replace(sampling_scheme=P.SO3_SAMPLING_MW, steerable=0)
     if p.reality:
         f = np.zeros(f_size(q), dtype=np.float64)
         fn = lib().so3_core_inverse_via_ssht_real if method == VIA_SSHT else lib().so3_core_inverse_direct_real
@@ -81,7 +81,7 @@ def forward(f, p, method=VIA_SSHT, flmn=None):
     else:
         f = np.ascontiguousarray(f, dtype=np.complex128)
         fn = lib().so3_core_forward_via_ssht if method == VIA_SSHT else lib().so3_core_forward_direct
-    q = p if method == VIA_SSHT else p.replace(sampling_scheme=P.SO3_SAMPLING_MW)
+    q = p if method == VIA_SSHT else p.replace(sampling_scheme=P.SO3_SAMPLING_MW, steerable=0)
     if f.size < f_size(q):
         raise ValueError("f has %d entries, needs %d" % (f.size, f_size(q)))
     fn(_ptr(out), _ptr(f), _s(p))

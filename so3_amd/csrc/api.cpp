@@ -69,7 +69,10 @@ void *grow(void *&p, size_t &cap, size_t bytes) {
 
 size_t f_bytes(const so3_parameters_t *p, Method m) {
   so3_parameters_t q = *p;
-  if (m == so3g::kDirect) q.sampling_scheme = SO3_SAMPLING_MW;  // direct is MW only
+  if (m == so3g::kDirect) {
+    q.sampling_scheme = SO3_SAMPLING_MW;  // direct is MW only
+    q.steerable = 0;                      // and writes all 2N - 1 gamma samples
+  }
   return (size_t)so3_sampling_f_size(&q) * (p->reality ? sizeof(double) : 2 * sizeof(double));
 }
 size_t flmn_bytes(const so3_parameters_t *p) {

@@ -20,6 +20,7 @@ struct Geo {
   int real;        // real signal: n >= 0 only, c2r / r2c over gamma
   int NS;          // n signs handled per |n|: 2 complex, 1 real
   int nmodeL;      // keep only l == |n| (direct transforms with N_MODE_L)
+  int steer;       // steerable (via SSHT, complex): N gamma samples at pi g / N
   int clmode;      // (2l+1)/(8 pi^2) factor: 1 in the inverse contraction
                    // (synthesis), 2 in the forward one (adjoint of the
                    // inverse), 0 nowhere (forward, adjoint of the forward)
@@ -75,6 +76,8 @@ void launch_alpha(const Geo &g, double2 *X2, int dir, double scale, hipStream_t 
 void launch_rows(double2 *X, int64_t nrows, int n, int ld, int dir, double scale, hipStream_t st);
 void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, double scale, hipStream_t st);
 void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, double scale, hipStream_t st);
+// steerable gamma: dir = +1 X2 -> f (N samples kept), dir = -1 f -> X2
+void launch_gamma_steer(const Geo &g, const void *src, void *dst, int dir, double scale, hipStream_t st);
 // adjoint != 0: zero-fill instead of the beta extension and no weights (the
 // beta stage of the adjoint of the inverse).  wr_even (the weight spectrum
 // without the two odd taps, natural order) and pole (the folded responses of

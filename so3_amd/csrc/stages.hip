@@ -529,6 +529,70 @@ __global__ __launch_bounds__(kThreads) void k_fwd_gamma(Geo g, FftDesc fd, int K
   }
 }
 
+// ---------------------------------------------------------------- steerable gamma
+// Steerable signals keep N gamma samples at pi g / N (so3_core.c:115-130,
+// 219-222 inverse; :309-328 forward).
+//   inverse: the n slots, n < 0 moved to n + 2N and entry N zero, go through
+//     a backward DFT of length 2N; samples g < N are kept;
+//   forward: the N samples, zero-extended to 2N, go through a forward DFT;
+//     n of the parity of N - 1 take sc times entry n mod 2N, the other n
+//     are zero (the reference computes only those n).
+// A workgroup owns K columns p = b nA + a, as in k_inv_gamma / k_fwd_gamma.
+__global__ __launch_bounds__(kThreads) void k_inv_gamma_steer(Geo g, FftDesc fd, int K, int ld,
+                                                              const double2 *__restrict__ X2,
+                                                              double2 *__restrict__ f, double scale) {
+  extern __shared__ double2 lds[];
+  double2 *A = lds;
+  const double2 *tw = fft_load_twiddles(lds + K * ld, fd);
+  const int64_t S = (int64_t)g.nB * g.nA;
+  const int64_t p0 = (int64_t)blockIdx.x * K;
+  const int N = g.N, M = 2 * g.N;
+  for (int idx = threadIdx.x; idx < K * M; idx += blockDim.x) {
+    const int j = idx / K, s = idx - j * K;
+    const int64_t p = p0 + s;
+    double2 v = make_double2(0.0, 0.0);
+    if (p < S && j != N) {
+      const int k = j < N ? j : j - 1;  // n = j - 2N sits at slot n + 2N - 1
+      const int b = (int)(p / g.nA), a = (int)(p % g.nA);
+      v = X2[x2_row(g, k, b) * g.ldA + a];
+    }
+    A[s * ld + j] = v;
+  }
+  __syncthreads();
+  fft_run<+1>(A, K, ld, fd, tw);
+  for (int idx = threadIdx.x; idx < K * N; idx += blockDim.x) {
+    const int k = idx / K, s = idx - k * K;
+    const int64_t p = p0 + s;
+    if (p < S) f[k * S + p] = cscale(A[s * ld + k], scale);
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_fwd_gamma_steer(Geo g, FftDesc fd, int K, int ld,
+                                                              const double2 *__restrict__ f,
+                                                              double2 *__restrict__ X2, double sc) {
+  extern __shared__ double2 lds[];
+  double2 *A = lds;
+  const double2 *tw = fft_load_twiddles(lds + K * ld, fd);
+  const int64_t S = (int64_t)g.nB * g.nA;
+  const int64_t p0 = (int64_t)blockIdx.x * K;
+  const int N = g.N, M = 2 * g.N, nG = g.nG;
+  for (int idx = threadIdx.x; idx < K * M; idx += blockDim.x) {
+    const int j = idx / K, s = idx - j * K;
+    const int64_t p = p0 + s;
+    A[s * ld + j] = (p < S && j < N) ? f[j * S + p] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  fft_run<-1>(A, K, ld, fd, tw);
+  for (int idx = threadIdx.x; idx < K * nG; idx += blockDim.x) {
+    const int k = idx / K, s = idx - k * K;
+    const int64_t p = p0 + s;
+    if (p >= S) continue;
+    const int n = k < N ? k : k - nG;
+    const double2 v = ((n + N - 1) & 1) ? make_double2(0.0, 0.0) : cscale(A[s * ld + (n < 0 ? n + M : n)], sc);
+    X2[x2_row(g, k, (int)(p / g.nA)) * g.ldA + p % g.nA] = v;
+  }
+}
+
 // ---------------------------------------------------------------- forward beta
 __global__ __launch_bounds__(kThreads) void k_fwd_beta(Geo g, FftDesc fb, FftDesc fw, int K, int ld,
                                                        const double2 *__restrict__ X2,
@@ -1789,6 +1853,26 @@ void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, double scale, hi
   const int64_t S = (int64_t)g.nB * g.nA;
   k_fwd_gamma<<<(unsigned)((S + K - 1) / K), kThreads, sh, st>>>(
       g, fd, K, ld, g.real ? nullptr : (const double2 *)f, g.real ? (const double *)f : nullptr, X2, scale);
+  SO3_LAUNCH_CHECK();
+}
+
+// steerable gamma on the LDS engine (length 2N), complex only
+void launch_gamma_steer(const Geo &g, const void *src, void *dst, int dir, double scale, hipStream_t st) {
+  if (!g.steer || g.real) throw HipError("steerable gamma needs a complex steerable transform");
+  const FftDesc fd = fft_plan(2 * g.N);
+  const int ld = fft_ld(fd);
+  const int tb = tw_count(fd) * (int)sizeof(double2);
+  const int K = choose_k(ld, fft_capacity(fd), fd.len, tb, 64);
+  const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
+  const int64_t S = (int64_t)g.nB * g.nA;
+  const unsigned grid = (unsigned)((S + K - 1) / K);
+  if (dir > 0) {
+    set_lds(k_inv_gamma_steer, sh);
+    k_inv_gamma_steer<<<grid, kThreads, sh, st>>>(g, fd, K, ld, (const double2 *)src, (double2 *)dst, scale);
+  } else {
+    set_lds(k_fwd_gamma_steer, sh);
+    k_fwd_gamma_steer<<<grid, kThreads, sh, st>>>(g, fd, K, ld, (const double2 *)src, (double2 *)dst, scale);
+  }
   SO3_LAUNCH_CHECK();
 }
 

@@ -83,16 +83,19 @@ Geo make_geo(const so3_parameters_t *p, Method method) {
   g.Pb = g.ss ? 2 * g.L : 2 * g.L - 1;
   g.NS = g.real ? 1 : 2;
   g.nmodeL = (method == kDirect && p->n_mode == SO3_N_MODE_L) ? 1 : 0;
+  // the direct transforms ignore steerable, as the reference's do
+  g.steer = (method == kViaSsht && p->steerable) ? 1 : 0;
   g.clmode = 1;
   return g;
 }
 
-using Key = std::tuple<int, int, int, int, int, int, int, int, int, int>;
+using Key = std::tuple<int, int, int, int, int, int, int, int, int, int, int>;
 
 Key key_of(const so3_parameters_t *p, Method method, int dev) {
   const int ss = (method == kViaSsht && p->sampling_scheme == SO3_SAMPLING_MW_SS);
+  const int steer = (method == kViaSsht && p->steerable) ? 1 : 0;
   return Key(dev, p->L, p->N, p->L0, ss, p->reality ? 1 : 0, (int)p->storage, (int)p->n_order,
-             (int)p->n_mode, (int)method);
+             (int)p->n_mode, (int)method, steer);
 }
 
 Plan build_plan(const so3_parameters_t *p, Method method) {
@@ -339,8 +342,10 @@ void validate(const so3_parameters_t *p, Method method) {
   if ((int)p->dl_method != 0 && (int)p->dl_method != 1) throw HipError("Invalid dl method");
   if (p->storage == SO3_STORAGE_COMPACT && p->N > p->L)
     throw HipError("compact storage needs N <= L");
-  if (method == kViaSsht && p->steerable)
-    throw HipError("steerable signals are not supported by this build");
+  // the reference's real via-SSHT transforms have their steerable branches
+  // commented out (so3_core.c:526-548, 743-813)
+  if (method == kViaSsht && p->steerable && p->reality)
+    throw HipError("steerable real signals are not supported");
   if ((size_t)(2 * p->N - 1) * (2 * p->L) * (p->L + 1) > (size_t)1 << 40)
     throw HipError("transform too large");
 }
@@ -374,7 +379,10 @@ void enqueue_inverse_locked(const Plan &P, const so3_parameters_t *p, const void
   NBase nb{P.nbase};
   timed(0, st, [&] { launch_contract_inv(g, (const double2 *)flmn, nb, dl, X1, st); });
   timed(1, st, [&] { launch_inv_beta(g, X1, X2, nb, P.phase_inv, st); });
-  if (reg_order(g)) {
+  if (g.steer) {
+    timed(2, st, [&] { launch_alpha(g, X2, +1, 1.0, st); });
+    timed(3, st, [&] { launch_gamma_steer(g, X2, f, +1, 1.0, st); });
+  } else if (reg_order(g)) {
     timed(3, st, [&] { launch_gamma_x2(g, X2, +1, 1.0, st); });
     timed(2, st, [&] { launch_alpha_x2f(g, X2, (double2 *)f, +1, 1.0, st); });
   } else {
@@ -389,7 +397,10 @@ void enqueue_forward_locked(const Plan &P, const so3_parameters_t *p, Method met
   double2 *Gp = ws, *X2 = ws + P.x1;
   const Geo &g = P.g;
   NBase nb{P.nbase};
-  if (reg_order(g)) {
+  if (g.steer) {
+    timed(4, st, [&] { launch_gamma_steer(g, f, X2, -1, 2 * M_PI / g.N, st); });
+    timed(5, st, [&] { launch_alpha(g, X2, -1, 1.0 / g.nA, st); });
+  } else if (reg_order(g)) {
     timed(5, st, [&] { launch_alpha_fx2(g, (const double2 *)f, X2, -1, 1.0 / g.nA, st); });
     timed(4, st, [&] { launch_gamma_x2(g, X2, -1, 2 * M_PI / (2.0 * g.N - 1.0), st); });
   } else {
