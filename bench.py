@@ -270,6 +270,33 @@ def variants_bench(A, P, torch, steps=5):
     return out
 
 
+def host_abi_bench(A, p, flmn_h, reps=3):
+    """End-to-end fwd+inv through the drop-in host-pointer C ABI
+    (so3_core_inverse_via_ssht / so3_core_forward_via_ssht on pageable numpy
+    arrays): each call copies its input to HBM, transforms and copies the
+    result back, synchronously.  Median wall milliseconds per pair over
+    `reps` pairs after one warm-up pair."""
+    import ctypes
+    lib = A.lib()
+    s = ctypes.byref(p.struct())
+    f = np.empty(A.f_size(p), dtype=np.float64 if p.reality else np.complex128)
+    back = np.zeros(A.flmn_size(p), dtype=np.complex128)
+    inv = lib.so3_core_inverse_via_ssht_real if p.reality else lib.so3_core_inverse_via_ssht
+    fwd = lib.so3_core_forward_via_ssht_real if p.reality else lib.so3_core_forward_via_ssht
+    times = []
+    for k in range(reps + 1):
+        t0 = time.perf_counter()
+        inv(A._ptr(f), A._ptr(flmn_h), s)
+        fwd(A._ptr(back), A._ptr(f), s)
+        times.append(time.perf_counter() - t0)
+    ms = 1e3 * float(np.median(times[1:]))
+    moved = 2 * (f.nbytes + flmn_h.nbytes)
+    err = float(np.abs(back - flmn_h).max() / np.abs(flmn_h).max())
+    return {"ms_per_pair": ms, "pairs_per_s": 1000.0 / ms, "pcie_bytes_per_pair": moved,
+            "pcie_gb_s_if_copy_bound": moved / ms / 1e6, "roundtrip_rel_err": err,
+            "note": "pageable host arrays; inputs copied to HBM and outputs back inside each call"}
+
+
 def cpu_baseline_mt(p, per_thread=3):
     """The same oracle work spread over the host's cores: one thread per core
     (ctypes releases the GIL; every oracle call has its own workspace), each
@@ -436,6 +463,7 @@ def main():
     }
     if not args.no_variants:
         out["variants"] = variants_bench(A, P, torch)
+        out["host_abi"] = host_abi_bench(A, p, flmn_h)
     if not args.no_batch:
         out["batched"] = batched_bench(A, P, torch)
     if not args.no_serve:
