@@ -7,6 +7,7 @@
 #include <cmath>
 #include <complex>
 #include <map>
+#include <tuple>
 #include <mutex>
 #include <vector>
 
@@ -93,7 +94,9 @@ double2 *upload(const std::vector<cld> &v) {
   return d;
 }
 
-FftDesc build(int n) {
+// reg1024: lengths in [257, 512] take Bluestein with M = 1024 (for the
+// register-resident FFTs); otherwise they follow the general rule
+FftDesc build(int n, bool reg1024) {
   FftDesc d{};
   d.n = n;
   std::vector<int> rad;
@@ -105,10 +108,11 @@ FftDesc build(int n) {
   }
   // a smooth length runs directly when one workgroup holds it in registers
   // (radix 5 and 7 passes hold fewer points per thread), else via Bluestein;
-  // every n in [257, 512] takes Bluestein with M = 1024, which runs on the
-  // register-resident FFTs (fft_reg.hpp, fft_reg32.hpp) faster than the workgroup engine
-  // runs even a smooth n directly
-  bool direct = factor(n, rad) && !(n >= 257 && n <= 512);
+  // with reg1024 every n in [257, 512] takes Bluestein with M = 1024, which
+  // runs on the register-resident FFTs (fft_reg.hpp, fft_reg32.hpp) faster
+  // than the workgroup engine runs even a smooth n directly
+  const bool r1024 = reg1024 && n >= 257 && n <= 512;
+  bool direct = factor(n, rad) && !r1024;
   if (direct) {
     int cap = kFftThreads * kFftElems * 4;
     for (int R : rad) cap = std::min(cap, kFftThreads * fft_nb(R) * R);
@@ -122,7 +126,7 @@ FftDesc build(int n) {
     while (M < 2 * n - 1) M *= 2;
     // 3 * 2^k when that suffices and one workgroup holds it (fewer points
     // than the next power of two); n in [257, 512] keeps M = 1024
-    if (!(n >= 257 && n <= 512) && M >= 4 && 3 * (M / 4) >= 2 * n - 1 && 3 * (M / 4) <= 1536) M = 3 * (M / 4);
+    if (!r1024 && M >= 4 && 3 * (M / 4) >= 2 * n - 1 && 3 * (M / 4) <= 1536) M = 3 * (M / 4);
     d.blue = 1;
     d.len = M;
     factor(M, rad);
@@ -182,19 +186,23 @@ int fft_capacity(const FftDesc &d) {
   return cap;
 }
 
-FftDesc fft_plan(int n) {
+static FftDesc plan_cached(int n, bool reg1024) {
   static std::mutex mu;
-  static std::map<std::pair<int, int>, FftDesc> cache;
+  static std::map<std::tuple<int, int, bool>, FftDesc> cache;
   int dev = 0;
   SO3_HIP(hipGetDevice(&dev));
   std::lock_guard<std::mutex> g(mu);
-  auto key = std::make_pair(dev, n);
+  auto key = std::make_tuple(dev, n, reg1024);
   auto it = cache.find(key);
   if (it != cache.end()) return it->second;
-  FftDesc d = build(n);
+  FftDesc d = build(n, reg1024);
   cache[key] = d;
   return d;
 }
+
+FftDesc fft_plan(int n) { return plan_cached(n, true); }
+
+FftDesc fft_plan_lds(int n) { return n >= 257 && n <= 512 ? plan_cached(n, false) : plan_cached(n, true); }
 
 }  // namespace so3g
 

@@ -9,6 +9,11 @@ namespace so3g {
 // device for the lifetime of the process.  Thread-safe.
 FftDesc fft_plan(int n);
 
+// The plan for the workgroup (LDS) engine: as fft_plan, except that a smooth
+// n in [257, 512] runs directly instead of through Bluestein with M = 1024
+// (which only the register-resident FFTs run faster).
+FftDesc fft_plan_lds(int n);
+
 // Largest S * len one workgroup can run through every pass of d (each
 // thread holds fft_nb(R) butterflies of radix R).
 int fft_capacity(const FftDesc &d);

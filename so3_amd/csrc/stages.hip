@@ -1859,7 +1859,7 @@ void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, double scale, hi
 // steerable gamma on the LDS engine (length 2N), complex only
 void launch_gamma_steer(const Geo &g, const void *src, void *dst, int dir, double scale, hipStream_t st) {
   if (!g.steer || g.real) throw HipError("steerable gamma needs a complex steerable transform");
-  const FftDesc fd = fft_plan(2 * g.N);
+  const FftDesc fd = fft_plan_lds(2 * g.N);
   const int ld = fft_ld(fd);
   const int tb = tw_count(fd) * (int)sizeof(double2);
   const int K = choose_k(ld, fft_capacity(fd), fd.len, tb, 64);
@@ -1903,8 +1903,9 @@ void launch_fwd_beta(const Geo &g, const double2 *X2, double2 *Gp, NBase nb, con
     SO3_LAUNCH_CHECK();
     return;
   }
-  const FftDesc fb = fft_plan(g.Pb);
-  const FftDesc fw = fft_plan(Pw);
+  // workgroup engine: smooth lengths run directly
+  const FftDesc fb = fft_plan_lds(g.Pb);
+  const FftDesc fw = fft_plan_lds(Pw);
   const int ld = std::max(fft_ld(fb), fft_ld(fw));
   const int tb = (tw_count(fb) + (fw.len == fb.len ? 0 : tw_count(fw))) * (int)sizeof(double2);
   if (!adjoint && wr_even && pole) {

@@ -140,6 +140,9 @@ Plan build_plan(const so3_parameters_t *p, Method method) {
   // weights w(p) = int_0^pi exp(-i p b) sin b db (so3_sampling_weight) at p mod Pw,
   // backward DFT of length Pw >= 4L-3 (any such length is exact)
   P.Pw = conv_length(4 * L - 3);
+  // the register-resident forward beta (beta DFTs on Bluestein M = 1024, so
+  // L = 129..256) convolves at 1024 points; any Pw >= 4L - 3 is exact
+  if (L <= 256 && fft_plan(g.Pb).blue && fft_plan(g.Pb).len == 1024) P.Pw = 1024;
   {
     const int Pw = P.Pw;
     std::vector<cld> w(Pw, cld(0, 0));

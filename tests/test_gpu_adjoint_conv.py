@@ -61,7 +61,9 @@ def _dot(a, b):
     (8, 8, 0, 0, 0, 0, 0), (9, 4, 1, 0, 1, 1, 0), (8, 5, 0, 1, 0, 0, 0), (10, 6, 0, 0, 0, 1, 2),
     (12, 12, 0, 0, 1, 0, 0), (7, 3, 0, 1, 1, 0, 3), (16, 16, 0, 0, 0, 0, 0),
     # full size: the register-resident beta and DFT kernels (adjoint branch)
-    (256, 256, 0, 0, 0, 0, 0), (256, 256, 0, 1, 0, 0, 0), (200, 150, 3, 0, 0, 1, 0)])
+    (256, 256, 0, 0, 0, 0, 0), (256, 256, 0, 1, 0, 0, 0), (200, 150, 3, 0, 0, 1, 0),
+    # L = 129..192: 1024-point convolution on the one-sequence register kernel
+    (150, 40, 0, 0, 0, 0, 0), (170, 12, 0, 1, 0, 0, 0)])
 def test_adjoint_dot_products(L, N, L0, real, storage, order, mode):
     p = P.SO3Parameters(L=L, N=N, L0=L0, reality=real, storage=storage, n_order=order, n_mode=mode)
     rng = np.random.default_rng(L + 100 * N)

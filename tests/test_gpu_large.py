@@ -100,12 +100,16 @@ def test_sparse_inverse_and_round_trip(oracle, L, N, real, ss):
     assert np.abs(back - flmn).max() < 1e-11
 
 
-# Forward transforms of arbitrary (not band-limited) samples at 193 <= L <=
+# Forward transforms of arbitrary (not band-limited) samples at 100 <= L <=
 # 256, where the beta stages pair sequences of opposite symmetry and the
 # pole samples of the antisymmetric member take a separate path: against the
 # oracle's per-sequence algorithm (N small keeps the oracle fast).
 @pytest.mark.parametrize("L,N,real,ss", [(256, 2, 0, 0), (256, 2, 1, 0), (256, 2, 0, 1), (256, 2, 1, 1),
-                                         (200, 3, 0, 0), (193, 2, 0, 1)])
+                                         (200, 3, 0, 0), (193, 2, 0, 1),
+                                         # 1024-point convolution on the register kernel
+                                         (150, 2, 0, 0), (129, 3, 1, 1),
+                                         # workgroup engine, 512-point direct convolution
+                                         (120, 3, 0, 0), (100, 2, 1, 1)])
 def test_forward_arbitrary_samples_paired_beta(oracle, L, N, real, ss):
     p = P.SO3Parameters(L=L, N=N, reality=real, sampling_scheme=ss, n_order=P.SO3_N_ORDER_ZERO_FIRST)
     rng = np.random.default_rng(L + 10 * real + 100 * ss)
