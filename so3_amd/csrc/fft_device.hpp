@@ -37,6 +37,8 @@ struct FftDesc {
   const double2 *bhat_b; // FFT_len of the backward chirp kernel
   const double2 *tws;    // len == 1024: twiddles of the register FFT
                          // (fft_reg.hpp), else null
+  int twg;               // 1: the workgroup engine reads tw from global
+                         // memory (set per launch; see tw_count)
 };
 
 // floor(x / d) for 0 <= x < 2^16, 1 <= d <= 2^16 with m = floor(2^32/d) + 1;
@@ -267,14 +269,17 @@ __device__ void fft_inner(double2 *X, int S, int ld, const FftDesc &d, const dou
 
 // Twiddle tables of lengths from kTwGlobal up are read from global memory,
 // where they stay in L1/L2: in LDS (32 KB and more) they left room for one
-// workgroup per CU.  tw_count: the table's LDS entries.
+// workgroup per CU.  A launcher sets twg for shorter tables when a
+// workgroup transforms too few sequences to repay staging the table.
+// tw_count: the table's LDS entries.
 constexpr int kTwGlobal = 2048;
-__host__ __device__ inline int tw_count(const FftDesc &d) { return d.len >= kTwGlobal ? 0 : d.len; }
+__host__ __device__ inline bool tw_global(const FftDesc &d) { return d.len >= kTwGlobal || d.twg; }
+__host__ __device__ inline int tw_count(const FftDesc &d) { return tw_global(d) ? 0 : d.len; }
 
 // Copy the twiddle table of d into LDS, or not (tw_count); returns the table
 // fft_run reads.  Call before the first fft_run; the caller synchronises.
 __device__ inline const double2 *fft_load_twiddles(double2 *tw_lds, const FftDesc &d) {
-  if (d.len >= kTwGlobal) return d.tw;
+  if (tw_global(d)) return d.tw;
   for (int k = threadIdx.x; k < d.len; k += kFftThreads) tw_lds[k] = d.tw[k];
   return tw_lds;
 }

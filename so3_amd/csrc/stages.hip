@@ -1904,13 +1904,23 @@ void launch_fwd_beta(const Geo &g, const double2 *X2, double2 *Gp, NBase nb, con
     return;
   }
   // workgroup engine: smooth lengths run directly
-  const FftDesc fb = fft_plan_lds(g.Pb);
-  const FftDesc fw = fft_plan_lds(Pw);
+  FftDesc fb = fft_plan_lds(g.Pb);
+  FftDesc fw = fft_plan_lds(Pw);
   const int ld = std::max(fft_ld(fb), fft_ld(fw));
-  const int tb = (tw_count(fb) + (fw.len == fb.len ? 0 : tw_count(fw))) * (int)sizeof(double2);
+  int tb = (tw_count(fb) + (fw.len == fb.len ? 0 : tw_count(fw))) * (int)sizeof(double2);
   if (!adjoint && wr_even && pole) {
     // K pairs per workgroup, F~(+-1) of each after the tables
-    const int K = choose_k2(ld, fb, fw, tb + 16 * (int)sizeof(double2), 8);
+    int K = choose_k2(ld, fb, fw, tb + 16 * (int)sizeof(double2), 8);
+    if (K == 1) {
+      // one pair per workgroup (L > 256): tables of 1024 points and more
+      // are read from global memory rather than staged for a single pair
+      // (20% faster at L = 288 MW_SS, 368 MW and 384 MW_SS; 6% slower at
+      // L = 320 MW)
+      fb.twg = fb.len >= 1024;
+      fw.twg = fw.len >= 1024;
+      tb = (tw_count(fb) + (fw.len == fb.len ? 0 : tw_count(fw))) * (int)sizeof(double2);
+      K = choose_k2(ld, fb, fw, tb + 16 * (int)sizeof(double2), 8);
+    }
     const size_t sh = (size_t)K * ld * sizeof(double2) + tb + 2 * K * sizeof(double2);
     set_lds(k_fwd_beta_gp, sh);
     const int chunks = (g.nA + 2 * K - 1) / (2 * K);
