@@ -322,102 +322,3 @@ __device__ void fft_run(double2 *X, int S, int ld, const FftDesc &d, const doubl
 }
 
 }  // namespace so3g
-
-namespace so3g {
-
-// ---------------------------------------------------------------------------
-// Wave-level engine: one wavefront transforms one sequence of length
-// len <= kWaveMax held contiguously in LDS.  Within a pass a wavefront
-// reads all its inputs before writing, and LDS operations of a wavefront
-// complete in order, so no workgroup barrier is needed anywhere; only a
-// compiler scheduling fence separates the read and write phases.
-constexpr int kWaveElems = 16;  // complex values a lane holds per pass
-constexpr int kWaveMax = 64 * kWaveElems;
-__host__ __device__ constexpr int wave_nb(int R) { return kWaveElems / R > 1 ? kWaveElems / R : 1; }
-
-template <int R, int DIR>
-__device__ __attribute__((noinline)) void wave_pass(double2 *X, int len, int Ns, const double2 *tw, int lane) {
-  constexpr int NB = wave_nb(R);
-  const int nb = len / R;
-  const int twstep = len / (Ns * R);
-  const unsigned mns = fdiv_magic(Ns);
-  double2 v[NB][R];
-  int ob[NB];
-#pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    const int t = lane + 64 * b;
-    if (t < nb) {
-      const int tq = fdiv(t, mns), j = t - tq * Ns;
-#pragma unroll
-      for (int q = 0; q < R; ++q) v[b][q] = X[t + q * nb];
-      if (Ns > 1) {
-#pragma unroll
-        for (int q = 1; q < R; ++q) v[b][q] = cmul(v[b][q], twl<DIR>(tw, q * j * twstep));
-      }
-      dft_small<R, DIR>(v[b]);
-      ob[b] = tq * Ns * R + j;
-    }
-  }
-  __builtin_amdgcn_wave_barrier();
-#pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    const int t = lane + 64 * b;
-    if (t < nb) {
-#pragma unroll
-      for (int q = 0; q < R; ++q) X[ob[b] + q * Ns] = v[b][fft_out_slot<R>(q)];
-    }
-  }
-  __builtin_amdgcn_wave_barrier();
-}
-
-template <int DIR>
-__device__ void wave_inner(double2 *X, const FftDesc &d, const double2 *tw, int lane) {
-  int Ns = 1;
-  unsigned long long code = d.radcode;
-  for (int p = 0; p < d.npass; ++p, code >>= 4) {
-    switch ((int)(code & 15)) {
-      case 0: wave_pass<2, DIR>(X, d.len, Ns, tw, lane); Ns *= 2; break;
-      case 1: wave_pass<3, DIR>(X, d.len, Ns, tw, lane); Ns *= 3; break;
-      case 2: wave_pass<4, DIR>(X, d.len, Ns, tw, lane); Ns *= 4; break;
-      case 3: wave_pass<5, DIR>(X, d.len, Ns, tw, lane); Ns *= 5; break;
-      case 4: wave_pass<7, DIR>(X, d.len, Ns, tw, lane); Ns *= 7; break;
-      case 5: wave_pass<8, DIR>(X, d.len, Ns, tw, lane); Ns *= 8; break;
-      default: wave_pass<16, DIR>(X, d.len, Ns, tw, lane); Ns *= 16; break;
-    }
-  }
-}
-
-// Full DFT of length d.n in place on one wavefront's sequence X (n values,
-// room for d.len).  Bluestein's chirp and kernel multiplies are per lane.
-template <int DIR>
-__device__ void wave_fft(double2 *X, const FftDesc &d, const double2 *tw, int lane) {
-  if (!d.blue) {
-    wave_inner<DIR>(X, d, tw, lane);
-    return;
-  }
-  const int n = d.n, M = d.len;
-  for (int j = lane; j < M; j += 64) {
-    if (j < n) {
-      double2 c = d.chirp[j];
-      if (DIR > 0) c.y = -c.y;
-      X[j] = cmul(X[j], c);
-    } else {
-      X[j] = make_double2(0.0, 0.0);
-    }
-  }
-  __builtin_amdgcn_wave_barrier();
-  wave_inner<-1>(X, d, tw, lane);
-  const double2 *bh = DIR < 0 ? d.bhat_f : d.bhat_b;
-  for (int j = lane; j < M; j += 64) X[j] = cmul(X[j], bh[j]);
-  __builtin_amdgcn_wave_barrier();
-  wave_inner<+1>(X, d, tw, lane);
-  const double inv = 1.0 / (double)M;
-  for (int j = lane; j < n; j += 64) {
-    double2 c = d.chirp[j];
-    if (DIR > 0) c.y = -c.y;
-    X[j] = cmul(X[j], cscale(c, inv));
-  }
-  __builtin_amdgcn_wave_barrier();
-}
-
-}  // namespace so3g

@@ -205,15 +205,3 @@ FftDesc fft_plan(int n) { return plan_cached(n, true); }
 FftDesc fft_plan_lds(int n) { return n >= 257 && n <= 512 ? plan_cached(n, false) : plan_cached(n, true); }
 
 }  // namespace so3g
-
-namespace so3g {
-bool wave_fits(const FftDesc &d) {
-  if (d.len > kWaveMax) return false;
-  unsigned long long code = d.radcode;
-  for (int p = 0; p < d.npass; ++p, code >>= 4) {
-    const int R = kRadixOf[code & 15];
-    if (d.len / R > 64 * wave_nb(R)) return false;
-  }
-  return true;
-}
-}  // namespace so3g

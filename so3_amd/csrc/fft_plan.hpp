@@ -22,9 +22,3 @@ int fft_capacity(const FftDesc &d);
 inline int fft_ld(const FftDesc &d) { return d.len + (d.len % 32 == 0 ? 1 : 0); }
 
 }  // namespace so3g
-
-namespace so3g {
-// true when one wavefront can run d (every pass fits wave_nb(R) butterflies
-// per lane)
-bool wave_fits(const FftDesc &d);
-}  // namespace so3g
