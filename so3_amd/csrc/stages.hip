@@ -397,65 +397,6 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
   }
 }
 
-// inverse beta: sequence (n slot, m slot) per wavefront, W consecutive m
-// slots of one n slot per group; the b < nB outputs leave through the LDS
-// tile so that HBM sees runs of W entries.  Pb is a Bluestein length with
-// inner length 1024.
-template <int W>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_inv_beta_reg(
-    Geo g, FftDesc fd, const double2 *__restrict__ X1, double2 *__restrict__ X2,
-    const int64_t *__restrict__ nbase, const double2 *__restrict__ phase) {
-  extern __shared__ double2 lds[];
-  const int wv = threadIdx.x >> 6;
-  double2 *tab = lds, *tile = lds + kRegBlueTab;
-  double *T = (double *)(lds + kRegBlueTab) + wv * kRegT;
-  reg_stage_blue<+1>(tab, fd);
-  __syncthreads();
-  const int chunks = (g.nA + W - 1) / W;
-  const int64_t groups = (int64_t)g.nG * chunks;
-  const int Pb = g.Pb, L = g.L;
-  for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
-    const int lane = loop_lane();
-    const int nslot = (int)(t / chunks);
-    const int ms0 = (int)(t % chunks) * W;
-    const int n = n_of_slot(g, nslot);
-    const bool active = nbase[n + g.N - 1] != kNoBlock;
-    const int mslot = ms0 + wv;
-    const bool mok = active && mslot < g.nA;
-    const int m = m_of_slot(g, mslot);
-    const double2 *row = X1 + ((int64_t)nslot * g.nA + mslot) * L;
-    const bool flip = ((m + n) & 1) != 0;
-    double2 v[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int k = lane + 64 * q;
-      double2 x = make_double2(0.0, 0.0);
-      if (mok && k < Pb && m >= -(L - 1)) {
-        const int mp = k <= L - 1 ? k : k - Pb;
-        if (mp >= -(L - 1)) {
-          x = row[mp < 0 ? -mp : mp];
-          if (mp < 0 && flip) x = make_double2(-x.x, -x.y);
-          x = cmul(x, phase[k]);
-        }
-      }
-      v[q] = x;
-    }
-    blue_reg<+1>(v, Pb, tab, T, lane);
-    __syncthreads();  // every wavefront is done with its transpose buffer
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int b = lane + 64 * q;
-      if (b < g.nB) tile[b * tile_ld<W>() + wv] = v[q];
-    }
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < g.nB * W; idx += (64 * W)) {
-      const int b = idx / W, c = idx - b * W;
-      if (ms0 + c < g.nA) X2[x2_row(g, nslot, b) * g.ldA + ms0 + c] = tile[b * tile_ld<W>() + c];
-    }
-    __syncthreads();  // the tile is read out before the next group's transposes
-  }
-}
-
 // ---------------------------------------------------------------- inverse gamma
 __global__ __launch_bounds__(kThreads) void k_inv_gamma(Geo g, FftDesc fd, int K, int ld,
                                                         const double2 *__restrict__ X2,
@@ -1697,15 +1638,6 @@ void launch_inv_beta(const Geo &g, const double2 *X1, double2 *X2, NBase nb, con
       kern<<<grid, 64 * kRows32Waves, sh, st>>>(g, fd, X1, X2, nb.base, phase);
     };
     (g.Pb & 1) ? go(k_inv_beta_r32<1>) : go(k_inv_beta_r32<0>);
-    SO3_LAUNCH_CHECK();
-    return;
-  }
-  if (fd.len == 1024 && fd.blue && g.nB <= 1024) {
-    constexpr int W = kBetaW;
-    const size_t sh = reg_shared<W>(kRegBlueTab, (int64_t)g.nB * tile_ld<W>());
-    const int chunks = (g.nA + W - 1) / W;
-    const unsigned grid = reg_grid(k_inv_beta_reg<W>, 64 * W, sh, (int64_t)g.nG * chunks);
-    k_inv_beta_reg<W><<<grid, 64 * W, sh, st>>>(g, fd, X1, X2, nb.base, phase);
     SO3_LAUNCH_CHECK();
     return;
   }
