@@ -27,7 +27,10 @@ enum { SO3X_VIA_SSHT = 0, SO3X_DIRECT = 1 };
 const char *so3x_last_error(void);
 
 /* Inverse transform flmn -> f on device memory (so3_core_inverse_via_ssht /
- * _direct semantics, selected by `method`). */
+ * _direct semantics, selected by `method`).  Single transforms share one
+ * workspace per device; calls on different streams are ordered on it by
+ * events (each waits for the previous call, whatever its stream), so they
+ * are safe but do not overlap.  Use a batch (so3x_batch) for overlap. */
 int so3x_inverse(void *f, const void *flmn, const so3_parameters_t *parameters, int method,
                  void *stream);
 /* Forward transform f -> flmn on device memory. */
@@ -101,7 +104,10 @@ int so3x_batch_tune(const so3x_item_t *items, int count, void *stream);
 
 /* Capture a batch into a HIP graph (the items are prepared first, outside
  * the capture); launch it on any stream as often as wanted; destroy it.
- * The graph holds the device pointers of the items. */
+ * The graph holds the device pointers of the items, which must stay valid
+ * until it is destroyed.  It owns its workspace and keeps its Delta tables
+ * resident (so3x_cache_limit never evicts them); launches of one graph run
+ * one after another, whatever streams they are made on. */
 int so3x_graph_create(const so3x_item_t *items, int count, int lanes, long long *graph);
 int so3x_graph_launch(long long graph, void *stream);
 int so3x_graph_destroy(long long graph);

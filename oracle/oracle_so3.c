@@ -786,3 +786,15 @@ void oracle_time_sample(const oracle_params *p, const int *ns, int count, double
   ssht_ws_destroy(w);
   free(dl);
 }
+
+/* ------------------------------------------------------------------ */
+/* helpers shared with oracle_adjoint.c (declared in oracle_int.h)      */
+/* ------------------------------------------------------------------ */
+int oracle_h_ind(int el, int m, int n, const oracle_params *p, int real) { return o_ind(el, m, n, p, real); }
+int oracle_h_n_active(int n, int N, int mode) { return o_n_active(n, N, mode); }
+double *oracle_h_dl(const oracle_params *p) { return dl_for(p); }
+void oracle_h_check(const oracle_params *p) { check_params(p); }
+int oracle_h_l_nset(int el, const oracle_params *p, int real, int *ns) { return direct_l_nset(el, p, real, ns); }
+double complex oracle_h_weight(int p) { return o_weight(p); }
+double oracle_h_dlv(const double *dl, int l, int mp, int m) { return dlv(dl, l, mp, m); }
+double complex oracle_h_ipow(int k) { return ipow(k); }

@@ -37,8 +37,7 @@ def lib():
                      "oracle_forward_direct", "oracle_forward_direct_real",
                      "oracle_adjoint_inverse_direct", "oracle_adjoint_forward_direct",
                      "oracle_adjoint_inverse_direct_real", "oracle_adjoint_forward_direct_real"):
-            if hasattr(_lib, name):
-                getattr(_lib, name).argtypes = [P, P, ctypes.POINTER(oracle_params)]
+            getattr(_lib, name).argtypes = [P, P, ctypes.POINTER(oracle_params)]
         _lib.oracle_dl_quarter.argtypes = [ctypes.c_int, ctypes.c_int, P]
         _lib.oracle_dl_offset.argtypes = [ctypes.c_int]
         _lib.oracle_dl_offset.restype = ctypes.c_long

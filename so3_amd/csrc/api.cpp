@@ -167,7 +167,10 @@ extern "C" {
 
 void so3_core_inverse_via_ssht(std::complex<double> *f, const std::complex<double> *flmn,
                                const so3_parameters_t *parameters) {
-  host_inverse(__func__, f, flmn, parameters, so3g::kViaSsht);
+  // the complex transform ignores `reality`, as the reference's does
+  so3_parameters_t q = *parameters;
+  q.reality = 0;
+  host_inverse(__func__, f, flmn, &q, so3g::kViaSsht);
 }
 void so3_core_inverse_via_ssht_real(double *f, const std::complex<double> *flmn,
                                     const so3_parameters_t *parameters) {
@@ -315,6 +318,9 @@ void so3_conv_convolution(std::complex<double> *h, const so3_parameters_t *hp,
                           const std::complex<double> *g, const so3_parameters_t *gp) {
   // so3_conv.c:154-184: forward_direct(f), forward_direct(g), harmonic
   // product, inverse_direct(h), all on the GPU (temporaries are freed).
+  // f, g and h are complex signals (the reference calls the complex direct
+  // transforms), so every step uses the complex parameter blocks and the
+  // complex coefficient layout whatever `reality` says.
   try {
     so3_parameters_t fq = *fp, gq = *gp, hq = *hp;
     fq.reality = gq.reality = hq.reality = 0;
@@ -323,15 +329,15 @@ void so3_conv_convolution(std::complex<double> *h, const so3_parameters_t *hp,
     so3g::validate(&hq, so3g::kDirect);
     const size_t ff = f_bytes(&fq, so3g::kDirect), fg = f_bytes(&gq, so3g::kDirect),
                  fh = f_bytes(&hq, so3g::kDirect);
+    const size_t lf = flmn_bytes(&fq), lg = flmn_bytes(&gq), lh = flmn_bytes(&hq);
     DevBuf df(f, ff, true), dg(g, fg, true), dh(h, fh, false);
-    DevBuf fl(nullptr, flmn_bytes(fp), false), gl(nullptr, flmn_bytes(gp), false),
-        hl(nullptr, flmn_bytes(hp), false);
-    SO3_HIP(hipMemset(fl.d, 0, flmn_bytes(fp)));
-    SO3_HIP(hipMemset(gl.d, 0, flmn_bytes(gp)));
-    SO3_HIP(hipMemset(hl.d, 0, flmn_bytes(hp)));
+    DevBuf fl(nullptr, lf, false), gl(nullptr, lg, false), hl(nullptr, lh, false);
+    SO3_HIP(hipMemset(fl.d, 0, lf));
+    SO3_HIP(hipMemset(gl.d, 0, lg));
+    SO3_HIP(hipMemset(hl.d, 0, lh));
     so3g::run_forward(&fq, so3g::kDirect, df.d, fl.d, nullptr);
     so3g::run_forward(&gq, so3g::kDirect, dg.d, gl.d, nullptr);
-    so3g::harmonic_convolution(hp, fp, gp, fl.d, gl.d, hl.d, nullptr);
+    so3g::harmonic_convolution(&hq, &fq, &gq, fl.d, gl.d, hl.d, nullptr);
     so3g::run_inverse(&hq, so3g::kDirect, hl.d, dh.d, nullptr);
     SO3_HIP(hipDeviceSynchronize());
     dh.copy_out(h, fh);

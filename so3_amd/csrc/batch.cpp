@@ -9,6 +9,12 @@
 // another.  Everything a batch needs (plans, Delta tables, DFT plans, index
 // tables, lane slabs) is built before the first launch, so a batch can be
 // captured into a HIP graph and replayed with no host work per item.
+//
+// A captured graph owns what its kernels address besides the caller's
+// buffers: its own lane slabs (freed when the graph is destroyed, never
+// shared with eager batches or other graphs) and a pin on every Delta table
+// it reads (pinned tables are never evicted).  Replays of one graph are
+// ordered: each launch waits for the previous one, on whatever stream.
 #include "batch.hpp"
 
 #include <algorithm>
@@ -45,6 +51,10 @@ std::map<std::string, int> g_tuned;  // batch shape -> lane count
 struct GraphRec {
   hipGraph_t g = nullptr;
   hipGraphExec_t x = nullptr;
+  std::vector<void *> slabs;                // lane workspaces the graph's kernels address
+  std::vector<std::pair<int, int>> pins;    // (L, dl_method) of the pinned Delta tables
+  hipEvent_t done = nullptr;                // end of the last replay
+  bool launched = false;
 };
 std::map<long long, GraphRec> g_graphs;
 std::map<long long, hipEvent_t> g_tickets;
@@ -125,54 +135,89 @@ void run_item(const so3x_item_t &it, void *ws, hipStream_t s) {
   }
 }
 
-// the batch on `lanes` lanes, caller holds g_mu
-void enqueue_locked(const so3x_item_t *items, int count, int lanes, hipStream_t st) {
-  if (count <= 0) return;
-  if (!items) throw HipError("items pointer is NULL");
-  const std::vector<size_t> ws = prepare_items(items, count);
+// Lane assignment of a batch: longest processing time first onto the
+// least-loaded lane, and the workspace each lane needs.
+struct LanePlan {
+  int lanes = 1;
+  std::vector<int> order, lane_of;
+  std::vector<size_t> need;  // double2 elements per lane
+};
+
+LanePlan plan_lanes(const so3x_item_t *items, int count, int lanes, const std::vector<size_t> &ws) {
+  LanePlan P;
   if (lanes <= 0) {
     auto it = g_tuned.find(shape_key(items, count));
     lanes = it != g_tuned.end() ? it->second : 4;
   }
-  lanes = std::max(1, std::min({lanes, kMaxLanes, count}));
-  // longest processing time first onto the least-loaded lane
-  std::vector<int> order(count);
-  std::iota(order.begin(), order.end(), 0);
+  P.lanes = std::max(1, std::min({lanes, kMaxLanes, count}));
+  P.order.resize(count);
+  std::iota(P.order.begin(), P.order.end(), 0);
   std::vector<double> cost(count);
   for (int i = 0; i < count; ++i) cost[i] = item_cost(items[i]);
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-  std::vector<double> load(lanes, 0.0);
-  std::vector<int> lane_of(count);
-  std::vector<size_t> need(lanes, 0);
-  for (int i : order) {
+  std::stable_sort(P.order.begin(), P.order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+  std::vector<double> load(P.lanes, 0.0);
+  P.lane_of.resize(count);
+  P.need.assign(P.lanes, 0);
+  for (int i : P.order) {
     const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-    lane_of[i] = l;
+    P.lane_of[i] = l;
     load[l] += cost[i];
-    need[l] = std::max(need[l], ws[i]);
+    P.need[l] = std::max(P.need[l], ws[i]);
   }
+  return P;
+}
+
+// the batch on `lanes` lanes, caller holds g_mu.  slabs == nullptr: the
+// device's shared lane slabs (grown as needed); otherwise one caller-owned
+// slab per lane of at least the planned size (a graph's own).
+void enqueue_locked(const so3x_item_t *items, int count, int lanes, hipStream_t st,
+                    const std::vector<void *> *slabs = nullptr) {
+  if (count <= 0) return;
+  if (!items) throw HipError("items pointer is NULL");
+  const std::vector<size_t> ws = prepare_items(items, count);
+  const LanePlan P = plan_lanes(items, count, lanes, ws);
   DevLanes &d = dev_lanes();
-  for (int l = 0; l < lanes; ++l) {
+  std::vector<void *> use(P.lanes, nullptr);
+  for (int l = 0; l < P.lanes; ++l) {
     Lane &ln = d.lane[l];
     if (!ln.s) {
       SO3_HIP(hipStreamCreateWithFlags(&ln.s, hipStreamNonBlocking));
       SO3_HIP(hipEventCreateWithFlags(&ln.done, hipEventDisableTiming));
     }
-    if (ln.cap < need[l]) {
+    if (slabs) {
+      if ((int)slabs->size() < P.lanes) throw HipError("graph workspace does not match its batch");
+      use[l] = (*slabs)[l];
+      continue;
+    }
+    if (ln.cap < P.need[l]) {
       SO3_HIP(hipStreamSynchronize(ln.s));
       if (ln.ws) SO3_HIP(hipFree(ln.ws));
       ln.ws = nullptr;
       ln.cap = 0;
-      SO3_HIP(hipMalloc(&ln.ws, sizeof(double2) * need[l]));
-      ln.cap = need[l];
+      SO3_HIP(hipMalloc(&ln.ws, sizeof(double2) * P.need[l]));
+      ln.cap = P.need[l];
     }
+    use[l] = ln.ws;
   }
   SO3_HIP(hipEventRecord(d.fork, st));
-  for (int l = 0; l < lanes; ++l) SO3_HIP(hipStreamWaitEvent(d.lane[l].s, d.fork, 0));
-  for (int i : order) run_item(items[i], d.lane[lane_of[i]].ws, d.lane[lane_of[i]].s);
-  for (int l = 0; l < lanes; ++l) {
+  for (int l = 0; l < P.lanes; ++l) SO3_HIP(hipStreamWaitEvent(d.lane[l].s, d.fork, 0));
+  for (int i : P.order) run_item(items[i], use[P.lane_of[i]], d.lane[P.lane_of[i]].s);
+  for (int l = 0; l < P.lanes; ++l) {
     SO3_HIP(hipEventRecord(d.lane[l].done, d.lane[l].s));
     SO3_HIP(hipStreamWaitEvent(st, d.lane[l].done, 0));
   }
+}
+
+void free_graph(GraphRec &rec) {
+  if (rec.done) {
+    (void)hipEventSynchronize(rec.done);
+    (void)hipEventDestroy(rec.done);
+  }
+  if (rec.x) (void)hipGraphExecDestroy(rec.x);
+  if (rec.g) (void)hipGraphDestroy(rec.g);
+  for (void *p : rec.slabs) (void)hipFree(p);
+  for (auto &pn : rec.pins) dl_tables_pin(pn.first, pn.second, -1);
+  rec = GraphRec{};
 }
 
 }  // namespace
@@ -209,17 +254,36 @@ int batch_tune(const so3x_item_t *items, int count, hipStream_t st) {
 
 long long graph_create(const so3x_item_t *items, int count, int lanes) {
   std::lock_guard<std::mutex> lk(g_mu);
+  if (count <= 0 || !items) throw HipError("a graph needs at least one item");
   hipStream_t cap = nullptr;
   SO3_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
   GraphRec rec;
   try {
-    // one eager run builds every plan, table and lane slab and sets the
+    // one eager run builds every plan, table and DFT plan and sets the
     // kernels' LDS attributes, so the capture records launches only
     enqueue_locked(items, count, lanes, cap);
     SO3_HIP(hipStreamSynchronize(cap));
+    // the graph's own lane slabs and pins on its Delta tables
+    const std::vector<size_t> ws = prepare_items(items, count);
+    const LanePlan P = plan_lanes(items, count, lanes, ws);
+    for (int l = 0; l < P.lanes; ++l) {
+      void *p = nullptr;
+      SO3_HIP(hipMalloc(&p, sizeof(double2) * std::max<size_t>(P.need[l], 1)));
+      rec.slabs.push_back(p);
+    }
+    for (int i = 0; i < count; ++i) {
+      const std::pair<int, int> k(items[i].parameters.L, (int)items[i].parameters.dl_method);
+      if (std::find(rec.pins.begin(), rec.pins.end(), k) != rec.pins.end()) continue;
+      dl_tables_pin(k.first, k.second, +1);
+      rec.pins.push_back(k);
+    }
+    // every table is resident now (built by the eager run, pinned since)
+    for (auto &pn : rec.pins)
+      if (!dl_tables_find(pn.first, pn.second)) throw HipError("Delta table evicted before capture");
+    SO3_HIP(hipEventCreateWithFlags(&rec.done, hipEventDisableTiming));
     SO3_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed));
     try {
-      enqueue_locked(items, count, lanes, cap);
+      enqueue_locked(items, count, P.lanes, cap, &rec.slabs);
     } catch (...) {
       hipGraph_t dummy = nullptr;
       (void)hipStreamEndCapture(cap, &dummy);
@@ -229,6 +293,7 @@ long long graph_create(const so3x_item_t *items, int count, int lanes) {
     SO3_HIP(hipStreamEndCapture(cap, &rec.g));
     SO3_HIP(hipGraphInstantiate(&rec.x, rec.g, nullptr, nullptr, 0));
   } catch (...) {
+    free_graph(rec);
     (void)hipStreamDestroy(cap);
     throw;
   }
@@ -242,15 +307,19 @@ void graph_launch(long long graph, hipStream_t st) {
   std::lock_guard<std::mutex> lk(g_mu);
   auto it = g_graphs.find(graph);
   if (it == g_graphs.end()) throw HipError("unknown graph handle");
-  SO3_HIP(hipGraphLaunch(it->second.x, st));
+  GraphRec &rec = it->second;
+  // replays share the graph's slabs: each waits for the previous one
+  if (rec.launched) SO3_HIP(hipStreamWaitEvent(st, rec.done, 0));
+  SO3_HIP(hipGraphLaunch(rec.x, st));
+  SO3_HIP(hipEventRecord(rec.done, st));
+  rec.launched = true;
 }
 
 void graph_destroy(long long graph) {
   std::lock_guard<std::mutex> lk(g_mu);
   auto it = g_graphs.find(graph);
   if (it == g_graphs.end()) throw HipError("unknown graph handle");
-  SO3_HIP(hipGraphExecDestroy(it->second.x));
-  SO3_HIP(hipGraphDestroy(it->second.g));
+  free_graph(it->second);  // waits for the last replay first
   g_graphs.erase(it);
 }
 
@@ -295,10 +364,7 @@ void ticket_wait(long long ticket) {
 
 void release_batch() {
   std::lock_guard<std::mutex> lk(g_mu);
-  for (auto &kv : g_graphs) {
-    (void)hipGraphExecDestroy(kv.second.x);
-    (void)hipGraphDestroy(kv.second.g);
-  }
+  for (auto &kv : g_graphs) free_graph(kv.second);
   g_graphs.clear();
   for (auto &kv : g_tickets) (void)hipEventDestroy(kv.second);
   g_tickets.clear();

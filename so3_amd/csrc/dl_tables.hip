@@ -182,6 +182,7 @@ DlTables build(int L, int method) {
 std::mutex g_mu;
 std::map<std::tuple<int, int, int>, DlTables> g_cache;
 std::map<std::tuple<int, int, int>, unsigned long long> g_used;  // last use, per table
+std::map<std::tuple<int, int, int>, int> g_pins;                 // holders that forbid eviction
 unsigned long long g_clock = 0;
 size_t g_limit = (size_t)32 << 30;  // bytes of tables kept per process
 std::map<int, double *> g_zero;
@@ -205,7 +206,7 @@ void make_room(size_t incoming, const std::tuple<int, int, int> &keep) {
     auto victim = g_cache.end();
     unsigned long long oldest = ~0ull;
     for (auto it = g_cache.begin(); it != g_cache.end(); ++it) {
-      if (it->first == keep) continue;
+      if (it->first == keep || g_pins.count(it->first)) continue;
       const unsigned long long u = g_used[it->first];
       if (u < oldest) oldest = u, victim = it;
     }
@@ -241,6 +242,27 @@ const DlTables &dl_tables(int L, int method) {
   return g_cache.emplace(key, build(L, method)).first->second;
 }
 
+const DlTables *dl_tables_find(int L, int method) {
+  int dev = 0;
+  SO3_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> g(g_mu);
+  auto key = std::make_tuple(dev, L, method);
+  auto it = g_cache.find(key);
+  if (it == g_cache.end()) return nullptr;
+  g_used[key] = ++g_clock;
+  return &it->second;
+}
+
+void dl_tables_pin(int L, int method, int delta) {
+  int dev = 0;
+  SO3_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> g(g_mu);
+  auto key = std::make_tuple(dev, L, method);
+  int &c = g_pins[key];
+  c += delta;
+  if (c <= 0) g_pins.erase(key);
+}
+
 void dl_tables_limit(size_t bytes) {
   std::lock_guard<std::mutex> g(g_mu);
   g_limit = bytes;
@@ -266,6 +288,7 @@ void dl_tables_clear() {
   for (auto &kv : g_cache) free_tables(kv.second);
   g_cache.clear();
   g_used.clear();
+  g_pins.clear();
 }
 
 }  // namespace so3g

@@ -37,6 +37,11 @@ struct DlTables {
 // the least recently used tables.
 const DlTables &dl_tables(int L, int method);
 void dl_tables_limit(size_t bytes);
+// The cached table or nullptr (never builds, never allocates).
+const DlTables *dl_tables_find(int L, int method);
+// Pin (delta = +1) or unpin (-1) a table: pinned tables are never evicted
+// (captured graphs hold their addresses).
+void dl_tables_pin(int L, int method, int delta);
 
 // Drop every cached table and zero page (frees device memory).
 void dl_tables_clear();

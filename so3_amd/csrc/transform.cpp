@@ -6,8 +6,11 @@
 //   forward: gamma DFT (g) -> alpha DFT (a) -> beta extension, DFT and weights
 //            -> contraction (m')
 // The Delta tables and DFT plans are cached per device; one workspace arena
-// per device serves every transform (calls on the plain path are
-// serialised).
+// per device serves every single transform.  Its use is stream-ordered: each
+// call records an event on its stream after its last stage, and the next
+// call, on whatever stream, waits for that event before its first stage, so
+// calls on different streams take turns on the arena instead of overwriting
+// each other's intermediates.
 #include "transform.hpp"
 
 #include <algorithm>
@@ -300,7 +303,9 @@ void resolve_pending() {
 
 struct Arena {
   double2 *buf = nullptr;
-  size_t cap = 0;  // double2 elements
+  size_t cap = 0;              // double2 elements
+  hipEvent_t done = nullptr;   // end of the last call that used buf
+  bool recorded = false;
 };
 std::map<int, Arena> g_arena;
 
@@ -313,11 +318,22 @@ const Plan &get_plan(const so3_parameters_t *p, Method method) {
   return g_plans.emplace(k, build_plan(p, method)).first->second;
 }
 
-double2 *arena(size_t elems) {
+bool capturing(hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  SO3_HIP(hipStreamIsCapturing(st, &cs));
+  return cs != hipStreamCaptureStatusNone;
+}
+
+// The device's arena, at least `elems` double2 long, ready for work
+// enqueued next on `st`: ordered after every earlier user.  Pair with
+// arena_release(st) after the call's last launch.
+double2 *arena_acquire(size_t elems, hipStream_t st) {
   int dev = 0;
   SO3_HIP(hipGetDevice(&dev));
   Arena &a = g_arena[dev];
+  if (!a.done) SO3_HIP(hipEventCreateWithFlags(&a.done, hipEventDisableTiming));
   if (a.cap < elems) {
+    if (capturing(st)) throw HipError("the workspace must be allocated before graph capture (call once eagerly)");
     if (a.buf) {
       SO3_HIP(hipDeviceSynchronize());
       SO3_HIP(hipFree(a.buf));
@@ -327,7 +343,19 @@ double2 *arena(size_t elems) {
     SO3_HIP(hipMalloc(&a.buf, sizeof(double2) * elems));
     a.cap = elems;
   }
+  // a captured call is ordered by its graph; an event recorded outside the
+  // capture cannot be waited on inside it
+  if (a.recorded && !capturing(st)) SO3_HIP(hipStreamWaitEvent(st, a.done, 0));
   return a.buf;
+}
+
+void arena_release(hipStream_t st) {
+  int dev = 0;
+  SO3_HIP(hipGetDevice(&dev));
+  Arena &a = g_arena[dev];
+  if (capturing(st)) return;
+  SO3_HIP(hipEventRecord(a.done, st));
+  a.recorded = true;
 }
 
 }  // namespace
@@ -427,14 +455,18 @@ void run_inverse(const so3_parameters_t *p, Method method, const void *flmn, voi
   validate(p, method);
   std::lock_guard<std::mutex> lk(g_mu);
   const Plan &P = get_plan(p, method);
-  enqueue_inverse_locked(P, p, flmn, f, arena(P.x1 + P.x2), st);
+  dl_tables(p->L, (int)p->dl_method);  // builds (synchronously) before the arena is taken
+  enqueue_inverse_locked(P, p, flmn, f, arena_acquire(P.x1 + P.x2, st), st);
+  arena_release(st);
 }
 
 void run_forward(const so3_parameters_t *p, Method method, const void *f, void *flmn, hipStream_t st) {
   validate(p, method);
   std::lock_guard<std::mutex> lk(g_mu);
   const Plan &P = get_plan(p, method);
-  enqueue_forward_locked(P, p, method, f, flmn, arena(P.x1 + P.x2), st);
+  dl_tables(p->L, (int)p->dl_method);
+  enqueue_forward_locked(P, p, method, f, flmn, arena_acquire(P.x1 + P.x2, st), st);
+  arena_release(st);
 }
 
 size_t workspace_elems(const so3_parameters_t *p, Method method) {
@@ -486,7 +518,7 @@ void run_adjoint_inverse(const so3_parameters_t *p, const void *f, void *flmn, h
   std::lock_guard<std::mutex> lk(g_mu);
   const Plan &P = get_plan(p, kDirect);
   const DlTables &dl = dl_tables(p->L, (int)p->dl_method);
-  double2 *ws = arena(P.x1 + P.x2);
+  double2 *ws = arena_acquire(P.x1 + P.x2, st);
   double2 *Gp = ws, *X2 = ws + P.x1;
   Geo g = P.g;
   NBase nb{P.nbase};
@@ -501,6 +533,7 @@ void run_adjoint_inverse(const so3_parameters_t *p, const void *f, void *flmn, h
   launch_fwd_zero(g, (double2 *)flmn, nb, 1, 0, (int)p->storage, (int)p->n_order, st);
   g.clmode = 2;
   launch_contract_fwd(g, Gp, nb, dl, (double2 *)flmn, st);
+  arena_release(st);
 }
 
 // adjoint of so3_core_forward_direct: flmn -> f (so3_adjoint.c:402-937)
@@ -509,7 +542,7 @@ void run_adjoint_forward(const so3_parameters_t *p, const void *flmn, void *f, h
   std::lock_guard<std::mutex> lk(g_mu);
   const Plan &P = get_plan(p, kDirect);
   const DlTables &dl = dl_tables(p->L, (int)p->dl_method);
-  double2 *ws = arena(P.x1 + P.x2);
+  double2 *ws = arena_acquire(P.x1 + P.x2, st);
   double2 *X1 = ws, *X2 = ws + P.x1;
   Geo g = P.g;
   g.clmode = 0;
@@ -523,6 +556,7 @@ void run_adjoint_forward(const so3_parameters_t *p, const void *flmn, void *f, h
     launch_alpha(g, X2, +1, 1.0 / g.nA, st);
     launch_inv_gamma(g, X2, f, 1.0 / (2.0 * g.N - 1.0), st);
   }
+  arena_release(st);
 }
 
 void release_all() {
@@ -539,7 +573,10 @@ void release_all() {
     (void)hipFree(kv.second.nbase);
   }
   g_plans.clear();
-  for (auto &kv : g_arena) (void)hipFree(kv.second.buf);
+  for (auto &kv : g_arena) {
+    (void)hipFree(kv.second.buf);
+    if (kv.second.done) (void)hipEventDestroy(kv.second.done);
+  }
   g_arena.clear();
   dl_tables_clear();
 }

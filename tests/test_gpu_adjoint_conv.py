@@ -1,12 +1,16 @@
 """Adjoint transforms and convolutions on the GPU.
 
-Adjoints are pinned by the dot-product identity <A x, y> = <x, A^H y>
-(the reference's own check, src/matlab/so3_adjoint_forward_test.m:41-65 and
-so3_adjoint_inverse_test.m:45-64), with A the GPU direct transform that
-tests/test_gpu_transforms.py pins against the oracle.  The S^2 -> SO(3)
-convolution is compared with the oracle, the SO(3) harmonic convolution
-with the brute-force sum of tests/test_convolution.c:134-177 (tolerance
-1e-12 as there).
+Adjoints are compared with the oracle's restatement of src/c/so3_adjoint.c
+(itself pinned against brute-force matrices in tests/test_oracle.py) on
+identical inputs, and at full size by the dot-product identity
+<A x, y> = <x, A^H y> (the reference's own check,
+src/matlab/so3_adjoint_forward_test.m:41-65 and so3_adjoint_inverse_test.m:45-64),
+with A the GPU direct transform that tests/test_gpu_transforms.py pins against
+the oracle.  The S^2 -> SO(3) convolution is compared with the oracle, the
+SO(3) harmonic convolution with the brute-force sum of
+tests/test_convolution.c:134-177 (tolerance 1e-12 as there), and the
+real-space convolution with the oracle's direct forward -> harmonic product
+-> direct inverse chain (src/c/so3_conv.c:154-184).
 """
 import ctypes
 
@@ -134,3 +138,76 @@ def test_harmonic_convolution_brute_force(N, real):
             if lib().so3_sampling_is_elmn_non_zero(el, n, k, s):
                 want += f[A.elmn2ind(el, m, k, p)] * np.conj(g[A.elmn2ind(el, n, k, p)])
         assert abs(h[i] - want) < 1e-12
+
+
+@pytest.mark.parametrize("L,N,L0,real,storage,order,mode", [
+    (8, 8, 0, 0, 0, 0, 0), (9, 4, 1, 0, 1, 1, 0), (8, 5, 0, 1, 0, 0, 0), (10, 6, 0, 0, 0, 1, 2),
+    (12, 12, 0, 1, 1, 0, 1), (7, 3, 0, 0, 1, 0, 3), (6, 4, 0, 0, 0, 0, 4), (16, 16, 2, 0, 0, 0, 0),
+    (33, 9, 0, 1, 0, 1, 0)])
+def test_adjoints_vs_oracle(oracle, L, N, L0, real, storage, order, mode):
+    """GPU so3_adjoint_* against the oracle's so3_adjoint.c restatement on the
+    same inputs (every stored coefficient, every sample), 1e-12 relative."""
+    p = P.SO3Parameters(L=L, N=N, L0=L0, reality=real, storage=storage, n_order=order, n_mode=mode)
+    rng = np.random.default_rng(L + 10 * N + mode)
+    y = _rand_f(p, rng)
+    got = adjoint_inverse(y, p)
+    ref = oracle.adjoint_inverse(y, p)
+    assert np.abs(got - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+    x = synth.gen_flmn(p, 4)
+    got = adjoint_forward(x, p)
+    ref = oracle.adjoint_forward(x, p)
+    assert np.abs(got - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+
+
+def _harmonic_conv_ref(f, fp, g, gp, hp):
+    """so3_conv_harmonic_convolution (so3_conv.c:44-90) as a brute-force sum"""
+    h = np.zeros(A.flmn_size(hp), complex)
+    ns, ne, ni = P.n_loop_values(fp)
+    s = ctypes.byref(gp.struct())
+    for i in range(A.flmn_size(hp)):
+        el, m, n = A.ind2elmn(i, hp)
+        if abs(m) > el or abs(n) > el or el < hp.L0:
+            continue
+        for k in range(ns, ne + 1, ni):
+            if lib().so3_sampling_is_elmn_non_zero(el, n, k, s):
+                h[i] += f[A.elmn2ind(el, m, k, fp)] * np.conj(g[A.elmn2ind(el, n, k, gp)])
+    return h
+
+
+@pytest.mark.parametrize("real_flag", [0, 1])
+def test_convolution_vs_oracle_chain(oracle, real_flag):
+    """so3_conv_convolution(h, f, g): direct forward of f and g, harmonic
+    product, direct inverse of h (so3_conv.c:154-184), against the same chain
+    on the oracle.  f, g and h are complex signals whatever `reality` says
+    (the reference calls the complex direct transforms)."""
+    fp = P.SO3Parameters(L=8, N=4, L0=1, reality=real_flag, n_order=0)
+    gp = P.SO3Parameters(L=9, N=5, reality=real_flag, n_order=0)
+    lib().so3_conv_get_parameters_of_convolved_lmn.restype = P.so3_parameters_t
+    lib().so3_conv_get_parameters_of_convolved_lmn.argtypes = [ctypes.POINTER(P.so3_parameters_t)] * 2
+    hs = lib().so3_conv_get_parameters_of_convolved_lmn(_s(fp), _s(gp))
+    hp = P.SO3Parameters(**{k: getattr(hs, k) for k, _ in P.so3_parameters_t._fields_})
+    assert (hp.L, hp.L0, hp.N) == (8, 1, 8)
+    fc, gc, hc = fp.replace(reality=0), gp.replace(reality=0), hp.replace(reality=0)
+    rng = np.random.default_rng(12)
+    f = rng.uniform(-1, 1, A.f_size(fc)) + 1j * rng.uniform(-1, 1, A.f_size(fc))
+    g = rng.uniform(-1, 1, A.f_size(gc)) + 1j * rng.uniform(-1, 1, A.f_size(gc))
+    h = np.zeros(A.f_size(hc), complex)
+    lib().so3_conv_convolution.argtypes = [ctypes.c_void_p, ctypes.POINTER(P.so3_parameters_t)] * 3
+    lib().so3_conv_convolution(_ptr(h), _s(hp), _ptr(f), _s(fp), _ptr(g), _s(gp))
+    fl = oracle.forward(f, fc, "direct")
+    gl = oracle.forward(g, gc, "direct")
+    hl = _harmonic_conv_ref(fl, fc, gl, gc, hc)
+    ref = oracle.inverse(hl, hc, "direct")
+    assert np.abs(h - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+
+
+def test_complex_inverse_ignores_reality(oracle):
+    """so3_core_inverse_via_ssht is the complex transform whatever `reality`
+    says (the reference's complex path never reads it)."""
+    p = P.SO3Parameters(L=8, N=5, n_order=0)
+    flmn = synth.gen_flmn(p, 6)
+    f = np.zeros(A.f_size(p), complex)
+    pr = p.replace(reality=1)
+    lib().so3_core_inverse_via_ssht(_ptr(f), _ptr(flmn), _s(pr))
+    ref = oracle.inverse(flmn, p)
+    assert np.abs(f - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())

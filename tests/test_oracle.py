@@ -12,7 +12,14 @@ sign or index, a transposed operand) fails it:
     the exact integral of their MW / MW_SS trigonometric interpolant
     (pins extension sign, quadrature weights and the half-sample phase);
   - round trips and the agreement of the two formulations (via SSHT vs
-    direct), as the reference's tests/test_so3.c:33-188 require.
+    direct), as the reference's tests/test_so3.c:33-188 require;
+  - the adjoint of the direct inverse against the conjugate transpose of
+    the brute-force synthesis matrix, the adjoint of the direct forward
+    against the transpose of the (quadrature-pinned) forward matrix, and the
+    dot-product identities of the reference's MATLAB adjoint tests;
+  - s2toso3 against the defining directional correlation
+    h(rho) = int f(w) conj(g(R_rho^{-1} w)) dw on S^2, by quadrature with
+    explicit rotation matrices and scipy's spherical harmonics.
 """
 import math
 from fractions import Fraction
@@ -244,3 +251,174 @@ def test_real_signal_is_real(oracle):
     b = oracle.inverse(fc, pc)
     assert np.abs(b.imag).max() < 1e-13
     assert np.abs(a - b.real).max() < 1e-13
+
+
+# ------------------------------------------------------------------ adjoints
+def _brute_matrix(p):
+    """Columns of the synthesis map restricted to the transform's
+    coefficient set, from the defining sum with explicit d-functions; a real
+    signal uses the complex synthesis of the same (n >= 0) coefficients."""
+    pc = p.replace(reality=0)
+    l, m, n = synth.nonzero_lmn(p)
+    cols = []
+    for ll, mm, nn in zip(l, m, n):
+        x = np.zeros(P.flmn_size(pc), complex)
+        x[P.elmn2ind_vec(ll, mm, nn, pc)] = 1.0
+        cols.append(brute_inverse(x, pc))
+    return np.array(cols).T, (l, m, n)
+
+
+ADJ_CASES = [(4, 3, 0, 0, 0, 0, 0), (5, 3, 1, 0, 0, 1, 1), (4, 3, 0, 1, 0, 0, 0), (5, 4, 0, 0, 2, 0, 0),
+             (5, 4, 0, 1, 1, 1, 0), (4, 3, 0, 0, 3, 0, 1), (4, 3, 0, 0, 4, 0, 0), (4, 2, 0, 1, 2, 0, 0)]
+
+
+@pytest.mark.parametrize("L,N,L0,real,mode,storage,order", ADJ_CASES)
+def test_adjoint_inverse_is_brute_force_transpose(oracle, L, N, L0, real, mode, storage, order):
+    """so3_adjoint_inverse_direct(f) = A^H f with A the synthesis matrix built
+    from explicit d-functions (real signals: the n >= 0 half of the complex
+    adjoint, the r2c transform of so3_adjoint.c:1044-1080)."""
+    p = P.SO3Parameters(L=L, N=N, L0=L0, reality=real, n_mode=mode, storage=storage, n_order=order)
+    A, (l, m, n) = _brute_matrix(p)
+    rng = np.random.default_rng(L + 7 * N + mode)
+    nf = oracle.f_size(p)
+    y = rng.uniform(-1, 1, nf) if real else rng.uniform(-1, 1, nf) + 1j * rng.uniform(-1, 1, nf)
+    got = oracle.adjoint_inverse(y, p)
+    idx = P.elmn2ind_vec(l, m, n, p)
+    assert np.abs(got[idx] - A.conj().T @ y).max() < 1e-13
+    # every other stored coefficient with l >= |n| is zero (written, not left)
+    rest = np.setdiff1d(np.arange(got.size), idx)
+    assert np.abs(got[rest]).max(initial=0.0) == 0.0
+
+
+@pytest.mark.parametrize("L,N,L0,real,mode,storage,order", ADJ_CASES)
+def test_adjoint_forward_is_forward_transpose(oracle, L, N, L0, real, mode, storage, order):
+    """so3_adjoint_forward_direct(x) = B^H x with B the matrix of the oracle's
+    direct forward transform (pinned by the exact-quadrature test above);
+    real signals: c2r semantics, Re(B^H x') with the n > 0 entries of x'
+    doubled (so3_adjoint.c:1791-1825)."""
+    p = P.SO3Parameters(L=L, N=N, L0=L0, reality=real, n_mode=mode, storage=storage, n_order=order)
+    l, m, n = synth.nonzero_lmn(p)
+    idx = P.elmn2ind_vec(l, m, n, p)
+    nf = oracle.f_size(p)
+    cols = []
+    for j in range(nf):
+        e = np.zeros(nf, float if real else complex)
+        e[j] = 1.0
+        cols.append(oracle.forward(e, p, "direct")[idx])
+    B = np.array(cols).T
+    x = synth.gen_flmn(p, 3)
+    xv = x[idx]
+    if real:
+        ref = (B.conj().T @ (xv * np.where(n > 0, 2.0, 1.0))).real
+    else:
+        ref = B.conj().T @ xv
+    assert np.abs(oracle.adjoint_forward(x, p) - ref).max() < 1e-13
+
+
+@pytest.mark.parametrize("real", [0, 1])
+def test_adjoint_dot_products_oracle(oracle, real):
+    """<A x, y> = <x, A^H y> and <B y, x> = <y, B^H x> at L = N = 12 on the
+    oracle's own direct pair (the reference's MATLAB checks,
+    src/matlab/so3_adjoint_inverse_test.m:45-64, so3_adjoint_forward_test.m:41-65)."""
+    p = P.SO3Parameters(L=12, N=12, L0=2, reality=real)
+    rng = np.random.default_rng(5)
+    x = synth.gen_flmn(p, 9)
+    nf = oracle.f_size(p)
+    y = rng.uniform(-1, 1, nf) if real else rng.uniform(-1, 1, nf) + 1j * rng.uniform(-1, 1, nf)
+    l, m, n = synth.nonzero_lmn(p)
+    idx = P.elmn2ind_vec(l, m, n, p)
+    w = np.where(n > 0, 2.0, 1.0) if real else 1.0  # real: n > 0 coefficients count twice
+    part = (lambda z: z.real) if real else (lambda z: z)  # real signals: real inner products
+    lhs = np.vdot(oracle.inverse(x, p, "direct"), y)
+    rhs = part(np.vdot(x[idx] * w, oracle.adjoint_inverse(y, p)[idx]))
+    assert abs(lhs - rhs) < 1e-11 * abs(lhs)
+    lhs = part(np.vdot(oracle.forward(y, p, "direct")[idx] * w, x[idx]))
+    rhs = np.vdot(y, oracle.adjoint_forward(x, p))
+    assert abs(lhs - rhs) < 1e-11 * abs(lhs)
+
+
+# --------------------------------------------------------- S^2 -> SO(3) conv
+def _rot_z(t):
+    c, s = np.cos(t), np.sin(t)
+    return np.array([[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]])
+
+
+def _rot_y(t):
+    c, s = np.cos(t), np.sin(t)
+    return np.array([[c, 0.0, s], [0.0, 1.0, 0.0], [-s, 0.0, c]])
+
+
+def _s2_signal(coef, L, th, ph):
+    from scipy.special import sph_harm_y
+    out = np.zeros(th.shape, complex)
+    for l in range(L):
+        for m in range(-l, l + 1):
+            if coef[l * l + l + m] != 0:
+                out += coef[l * l + l + m] * sph_harm_y(l, m, th, ph)
+    return out
+
+
+@pytest.mark.parametrize("real", [0, 1])
+def test_s2toso3_is_the_directional_correlation(oracle, real):
+    """h(rho) = int_{S^2} f(w) conj(g(R_rho^{-1} w)) dw, R_rho = Rz(alpha)
+    Ry(beta) Rz(gamma), evaluated by Gauss-Legendre x trapezoid quadrature
+    with scipy's spherical harmonics and explicit rotation matrices, against
+    the oracle's inverse of s2toso3(flm, glm) on the grid (pins the conj
+    placement, the g index and the 8 pi^2/(2l+1) factor of so3_conv.c:203-208).
+    g is band-limited to |m| < N, which the harmonic form assumes."""
+    L, N = 6, 3
+    rng = np.random.default_rng(4)
+    flm = rng.uniform(-1, 1, L * L) + 1j * rng.uniform(-1, 1, L * L)
+    glm = rng.uniform(-1, 1, L * L) + 1j * rng.uniform(-1, 1, L * L)
+    for l in range(L):
+        for m in range(-l, l + 1):
+            if abs(m) >= N:
+                glm[l * l + l + m] = 0
+    if real:  # real signals on the sphere: f_{l,-m} = (-1)^m conj(f_lm)
+        for c in (flm, glm):
+            for l in range(L):
+                c[l * l + l] = c[l * l + l].real
+                for m in range(1, l + 1):
+                    c[l * l + l - m] = (-1) ** m * np.conj(c[l * l + l + m])
+    hp = P.SO3Parameters(L=L, N=N, reality=real, n_order=P.SO3_N_ORDER_ZERO_FIRST)
+    h_grid = oracle.inverse(oracle.s2toso3(flm, glm, hp), hp)
+    na, nb, ng = 2 * L - 1, L, 2 * N - 1
+    x, w = np.polynomial.legendre.leggauss(2 * L + 2)
+    nph = 4 * L
+    TH, PH = np.meshgrid(np.arccos(x), 2 * np.pi * np.arange(nph) / nph, indexing="ij")
+    W = np.repeat(w[:, None], nph, 1) * 2 * np.pi / nph
+    v = np.stack([np.sin(TH) * np.cos(PH), np.sin(TH) * np.sin(PH), np.cos(TH)], -1)
+    fv = _s2_signal(flm, L, TH, PH)
+    for a, b, g in [(0, 0, 0), (3, 2, 1), (10, 5, 4), (7, 1, 2), (1, 4, 3), (9, 3, 0)]:
+        R = _rot_z(2 * np.pi * a / na) @ _rot_y((2 * b + 1) * np.pi / (2 * L - 1)) @ _rot_z(2 * np.pi * g / ng)
+        u = v @ R  # rows R^T v = R^{-1} v
+        gv = _s2_signal(glm, L, np.arccos(np.clip(u[..., 2], -1, 1)), np.arctan2(u[..., 1], u[..., 0]))
+        val = np.sum(W * fv * np.conj(gv))
+        assert abs(h_grid[(g * nb + b) * na + a] - val) < 1e-13, (a, b, g)
+
+
+def test_delta_tables_match_mpmath_fixtures(oracle):
+    """The oracle's L = 256 Risbo and Trapani tables reproduce, through
+    d^l_mn(b) = i^{n-m} sum_m' D_m'm D_m'n e^{i m' b}, the mpmath values of
+    Wigner's explicit sum at the sampled beta of both grids
+    (tests/golden/wigner_d_large.json, written by make_wigner_fixtures.py)."""
+    import json
+    import os
+    fix = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "wigner_d_large.json")))
+    for method in (0, 1):
+        T = oracle.dl_quarter(256, method)
+        for key in ("mw256", "mwss256"):
+            c = fix[key]
+            L = c["L"]
+            for k, vals in c["d"].items():
+                l, m, n = map(int, k.split(","))
+                D = _full(T[l], l)
+                mp = np.arange(-l, l + 1)
+                for j, v in zip(c["b"], vals):
+                    be = np.pi * j / L if c["ss"] else (2 * j + 1) * np.pi / (2 * L - 1)
+                    d = (1j ** ((n - m) % 4)) * np.sum(D[:, m + l] * D[:, n + l] * np.exp(1j * mp * be))
+                    assert abs(d - v) < 5e-14, (method, key, l, m, n, j)
+    # the small-l values agree with the double-precision explicit sum
+    c = fix["mw1024"]
+    for j, v in zip(c["b"], c["d"]["3,3,3"]):
+        assert abs(d_exact(3, 3, 3, (2 * j + 1) * np.pi / (2 * 1024 - 1)) - v) < 1e-15

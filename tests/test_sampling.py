@@ -164,3 +164,23 @@ def test_is_elmn_non_zero():
                 assert lib().so3_sampling_is_elmn_non_zero_return_int(el, m, n, s) == int(want)
     assert lib().so3_sampling_is_i_in_loop_range(5, 1, 9, 2)
     assert not lib().so3_sampling_is_i_in_loop_range(4, 1, 9, 2)
+
+
+def test_convolved_parameters_known_answer():
+    """so3_conv_get_parameters_of_convolved_lmn on the reference's case
+    (tests/test_convolution.c:191-213): f {L0=1, L=3, N=2}, g {L0=0, L=4,
+    N=4} -> h {L=3, L0=1, N=3} (h.N = h.L, so3_conv.c:118-120)."""
+    kat = GOLD["convolved_parameters"]
+    fp = P.SO3Parameters(n_order=0, **kat["f"])
+    gp = P.SO3Parameters(n_order=0, **kat["g"])
+    fn = lib().so3_conv_get_parameters_of_convolved_lmn
+    fn.restype = P.so3_parameters_t
+    fn.argtypes = [ctypes.POINTER(P.so3_parameters_t)] * 2
+    h = fn(ctypes.byref(fp.struct()), ctypes.byref(gp.struct()))
+    assert (h.L, h.L0, h.N) == (kat["h"]["L"], kat["h"]["L0"], kat["h"]["N"])
+    # the _void variant fills the same block
+    out = P.so3_parameters_t()
+    fv = lib().so3_conv_get_parameters_of_convolved_lmn_void
+    fv.argtypes = [ctypes.POINTER(P.so3_parameters_t)] * 3
+    fv(ctypes.byref(out), ctypes.byref(fp.struct()), ctypes.byref(gp.struct()))
+    assert (out.L, out.L0, out.N) == (3, 1, 3)
