@@ -37,6 +37,20 @@ int so3x_inverse(void *f, const void *flmn, const so3_parameters_t *parameters, 
 int so3x_forward(void *flmn, const void *f, const so3_parameters_t *parameters, int method,
                  void *stream);
 
+/* Allocation-free inverse.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same result as
+ * so3x_inverse, but it calls no hipMalloc, hipFree or other allocating
+ * routine: its workspace is a static device array of 64 MiB per GPU (held
+ * until the process ends), and the parameters must have been prepared on
+ * this device before (so3x_prepare, or one so3x_inverse call with them).
+ * Fails with a message when the transform's workspace exceeds 64 MiB (about
+ * L = N = 80 complex), or when the plan or Delta table is not resident.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (different streams without ordering between them) race. */
+int so3x_inverse_noalloc(void *f, const void *flmn, const so3_parameters_t *parameters, int method,
+                         void *stream);
+
 /* Build (or fetch) the Delta tables and DFT plans for these parameters. */
 int so3x_prepare(const so3_parameters_t *parameters);
 

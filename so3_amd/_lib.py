@@ -54,6 +54,7 @@ SIGNATURES = {
     "so3x_last_error": (ctypes.c_char_p, []),
     "so3x_inverse": (_I, [_P, _P, _PP, _I, _P]),
     "so3x_forward": (_I, [_P, _P, _PP, _I, _P]),
+    "so3x_inverse_noalloc": (_I, [_P, _P, _PP, _I, _P]),
     "so3x_prepare": (_I, [_PP]),
     "so3x_workspace_bytes": (ctypes.c_longlong, [_PP]),
     "so3x_release_all": (_I, []),

@@ -370,6 +370,17 @@ int so3x_forward(void *flmn, const void *f, const so3_parameters_t *parameters, 
   }
 }
 
+int so3x_inverse_noalloc(void *f, const void *flmn, const so3_parameters_t *parameters, int method,
+                         void *stream) {
+  try {
+    so3g::run_inverse_noalloc(parameters, method == SO3X_DIRECT ? so3g::kDirect : so3g::kViaSsht, flmn, f,
+                              (hipStream_t)stream);
+    return 0;
+  } catch (const std::exception &e) {
+    return status(e);
+  }
+}
+
 int so3x_prepare(const so3_parameters_t *parameters) {
   try {
     so3g::prepare(parameters, so3g::kViaSsht);

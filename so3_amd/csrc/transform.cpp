@@ -253,6 +253,9 @@ Plan build_plan(const so3_parameters_t *p, Method method) {
   fft_plan(g.nA);
   fft_plan(2 * N - 1);
   fft_plan(P.Pw);
+  fft_plan_lds(g.Pb);
+  fft_plan_lds(P.Pw);
+  fft_plan_lds(2 * N);
   return P;
 }
 
@@ -404,9 +407,8 @@ void prepare(const so3_parameters_t *p, Method method) {
 
 namespace {
 
-void enqueue_inverse_locked(const Plan &P, const so3_parameters_t *p, const void *flmn, void *f, double2 *ws,
+void enqueue_inverse_locked(const Plan &P, const DlTables &dl, const void *flmn, void *f, double2 *ws,
                             hipStream_t st) {
-  const DlTables &dl = dl_tables(p->L, (int)p->dl_method);
   double2 *X1 = ws, *X2 = ws + P.x1;
   const Geo &g = P.g;
   NBase nb{P.nbase};
@@ -455,8 +457,8 @@ void run_inverse(const so3_parameters_t *p, Method method, const void *flmn, voi
   validate(p, method);
   std::lock_guard<std::mutex> lk(g_mu);
   const Plan &P = get_plan(p, method);
-  dl_tables(p->L, (int)p->dl_method);  // builds (synchronously) before the arena is taken
-  enqueue_inverse_locked(P, p, flmn, f, arena_acquire(P.x1 + P.x2, st), st);
+  const DlTables &dl = dl_tables(p->L, (int)p->dl_method);  // built (synchronously) before the arena is taken
+  enqueue_inverse_locked(P, dl, flmn, f, arena_acquire(P.x1 + P.x2, st), st);
   arena_release(st);
 }
 
@@ -480,13 +482,70 @@ size_t workspace_elems(const so3_parameters_t *p, Method method) {
 void enqueue_inverse(const so3_parameters_t *p, Method method, const void *flmn, void *f, void *ws,
                      hipStream_t st) {
   std::lock_guard<std::mutex> lk(g_mu);
-  enqueue_inverse_locked(get_plan(p, method), p, flmn, f, (double2 *)ws, st);
+  enqueue_inverse_locked(get_plan(p, method), dl_tables(p->L, (int)p->dl_method), flmn, f, (double2 *)ws, st);
 }
 
 void enqueue_forward(const so3_parameters_t *p, Method method, const void *f, void *flmn, void *ws,
                      hipStream_t st) {
   std::lock_guard<std::mutex> lk(g_mu);
   enqueue_forward_locked(get_plan(p, method), p, method, f, flmn, (double2 *)ws, st);
+}
+
+// ---- the allocation-free inverse (so3x_inverse_noalloc)
+//
+// Its workspace is this file-scope device array, one copy per GPU that the
+// library runs on, held until the process ends.  In host code `scratch` is
+// only a stand-in: the device address of each GPU's copy comes from
+// hipGetSymbolAddress, taken once per device (with that device current) on
+// the entry point's first call there, and only that address is passed to
+// kernels.  The transform's kernels write every workspace entry they read,
+// so nothing is cleared between calls.  All calls on one GPU share the
+// array: calls that overlap on one GPU (different streams, no ordering)
+// race.
+constexpr size_t kScratchBytes = (size_t)64 << 20;  // 64 MiB
+constexpr size_t kScratchElems = kScratchBytes / sizeof(double2);
+static __device__ __align__(256) double2 scratch[kScratchElems];
+std::map<int, double2 *> g_scratch;  // device -> address of its copy of scratch
+
+double2 *scratch_address() {
+  int dev = 0;
+  SO3_HIP(hipGetDevice(&dev));
+  auto it = g_scratch.find(dev);
+  if (it != g_scratch.end()) return it->second;
+  void *ptr = nullptr;
+  SO3_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) throw HipError("hipGetSymbolAddress returned no address for the static workspace");
+  g_scratch[dev] = (double2 *)ptr;
+  return (double2 *)ptr;
+}
+
+size_t scratch_limit_bytes() { return kScratchBytes; }
+
+void run_inverse_noalloc(const so3_parameters_t *p, Method method, const void *flmn, void *f, hipStream_t st) {
+  validate(p, method);
+  std::lock_guard<std::mutex> lk(g_mu);
+  int dev = 0;
+  SO3_HIP(hipGetDevice(&dev));
+  auto it = g_plans.find(key_of(p, method, dev));
+  if (it == g_plans.end())
+    throw HipError("parameters not prepared on this device: call so3x_prepare (or so3x_inverse) with them first");
+  const Plan &P = it->second;
+  const DlTables *dl = dl_tables_find(p->L, (int)p->dl_method);
+  if (!dl) throw HipError("Delta table not resident: call so3x_prepare with these parameters first");
+  const size_t need = sizeof(double2) * (P.x1 + P.x2);
+  if (need > kScratchBytes)
+    throw HipError("workspace of " + std::to_string(need) + " bytes exceeds the " +
+                   std::to_string(kScratchBytes) + "-byte static workspace of the allocation-free entry point");
+  double2 *ws = scratch_address();
+  const bool timing = g_timing;  // the stage profiler creates events: off for this call
+  g_timing = false;
+  try {
+    enqueue_inverse_locked(P, *dl, flmn, f, ws, st);
+  } catch (...) {
+    g_timing = timing;
+    throw;
+  }
+  g_timing = timing;
 }
 
 void set_stage_timing(bool on) {

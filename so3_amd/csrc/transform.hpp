@@ -18,6 +18,11 @@ void run_forward(const so3_parameters_t *p, Method method, const void *f_dev, vo
 void run_adjoint_inverse(const so3_parameters_t *p, const void *f_dev, void *flmn_dev, hipStream_t st);
 void run_adjoint_forward(const so3_parameters_t *p, const void *flmn_dev, void *f_dev, hipStream_t st);
 void prepare(const so3_parameters_t *p, Method method);
+// The inverse on the library's static workspace (no allocation of any
+// kind): needs a prepared plan and a resident Delta table, and a workspace
+// of at most scratch_limit_bytes().
+void run_inverse_noalloc(const so3_parameters_t *p, Method method, const void *flmn, void *f, hipStream_t st);
+size_t scratch_limit_bytes();
 
 // Batched execution: plan, tables and the workspace size (double2 elements)
 // of one transform (builds what is missing, so call it outside graph
