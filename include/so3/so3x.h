@@ -54,6 +54,29 @@ int so3x_inverse_noalloc(void *f, const void *flmn, const so3_parameters_t *para
 /* Build (or fetch) the Delta tables and DFT plans for these parameters. */
 int so3x_prepare(const so3_parameters_t *parameters);
 
+/* Autotuner keyed on (L, N, reality, sampling).  so3x_tune times, on
+ * `stream`, the choices the library can make for these parameters -- the
+ * workgroup order of the inverse and forward contractions, and the length Pw
+ * of the forward transform's weight convolution (every 2^a or 3 * 2^a from
+ * 4L - 3 up to twice the default, which decides the FFT engine and plan that
+ * run it) -- and keeps the fastest for every later transform with the same
+ * (L, N, reality, sampling), single or batched.  It allocates and
+ * synchronises; call it outside timed regions.  choice (3 ints, may be NULL)
+ * receives (inverse order, forward order, Pw), ms (3 doubles, may be NULL)
+ * the best inverse-contraction, forward-contraction and forward-beta times.
+ * The contraction order never changes a result; another Pw changes only the
+ * rounding (the convolution is exact for every candidate).
+ * so3x_tune_get reports the choices in force; so3x_tune_save writes the
+ * table as text ("L N reality mw_ss corder_inv corder_fwd pw" per line) and
+ * so3x_tune_load reads one back (returns the number of entries, merged into
+ * the table in force). */
+int so3x_tune(const so3_parameters_t *parameters, int method, void *stream, int *choice, double *ms);
+int so3x_tune_get(const so3_parameters_t *parameters, int method, int *choice);
+int so3x_tune_save(const char *path);
+int so3x_tune_load(const char *path);
+/* Forget every tuned choice (back to the default rules). */
+int so3x_tune_clear(void);
+
 /* Device bytes of workspace one transform with these parameters needs. */
 long long so3x_workspace_bytes(const so3_parameters_t *parameters);
 

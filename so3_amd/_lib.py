@@ -50,12 +50,20 @@ SIGNATURES = {
     "so3_core_forward_direct_real": (None, [_P, _P, _PP]),
     # convolution (so3_conv.h)
     "so3_conv_s2toso3_harmonic_convolution": (None, [_P, _PP, _P, _P]),
+    "so3_conv_harmonic_convolution": (None, [_P, _PP, _P, _PP, _P, _PP]),
+    "so3_conv_convolution": (None, [_P, _PP, _P, _PP, _P, _PP]),
+    "so3_conv_get_parameters_of_convolved_lmn_void": (None, [_PP, _PP, _PP]),
     # extensions (so3x.h)
     "so3x_last_error": (ctypes.c_char_p, []),
     "so3x_inverse": (_I, [_P, _P, _PP, _I, _P]),
     "so3x_forward": (_I, [_P, _P, _PP, _I, _P]),
     "so3x_inverse_noalloc": (_I, [_P, _P, _PP, _I, _P]),
     "so3x_prepare": (_I, [_PP]),
+    "so3x_tune": (_I, [_PP, _I, _P, _PI, ctypes.POINTER(ctypes.c_double)]),
+    "so3x_tune_get": (_I, [_PP, _I, _PI]),
+    "so3x_tune_save": (_I, [ctypes.c_char_p]),
+    "so3x_tune_load": (_I, [ctypes.c_char_p]),
+    "so3x_tune_clear": (_I, []),
     "so3x_workspace_bytes": (ctypes.c_longlong, [_PP]),
     "so3x_release_all": (_I, []),
     "so3x_cache_limit": (_I, [ctypes.c_longlong]),

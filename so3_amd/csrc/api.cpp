@@ -381,6 +381,49 @@ int so3x_inverse_noalloc(void *f, const void *flmn, const so3_parameters_t *para
   }
 }
 
+int so3x_tune(const so3_parameters_t *parameters, int method, void *stream, int *choice, double *ms) {
+  try {
+    so3g::tune(parameters, method == SO3X_DIRECT ? so3g::kDirect : so3g::kViaSsht, (hipStream_t)stream, choice, ms);
+    return 0;
+  } catch (const std::exception &e) {
+    return status(e);
+  }
+}
+
+int so3x_tune_get(const so3_parameters_t *parameters, int method, int *choice) {
+  try {
+    if (!choice) throw HipError("choice pointer is NULL");
+    so3g::tune_get(parameters, method == SO3X_DIRECT ? so3g::kDirect : so3g::kViaSsht, choice);
+    return 0;
+  } catch (const std::exception &e) {
+    return status(e);
+  }
+}
+
+int so3x_tune_save(const char *path) {
+  try {
+    if (!path) throw HipError("path is NULL");
+    so3g::tune_save(path);
+    return 0;
+  } catch (const std::exception &e) {
+    return status(e);
+  }
+}
+
+int so3x_tune_clear(void) {
+  so3g::tune_clear();
+  return 0;
+}
+
+int so3x_tune_load(const char *path) {
+  try {
+    if (!path) throw HipError("path is NULL");
+    return so3g::tune_load(path);
+  } catch (const std::exception &e) {
+    return status(e);
+  }
+}
+
 int so3x_prepare(const so3_parameters_t *parameters) {
   try {
     so3g::prepare(parameters, so3g::kViaSsht);

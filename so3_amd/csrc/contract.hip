@@ -57,6 +57,11 @@ __host__ __device__ inline int64_t tri(int64_t k) { return k * (k + 1) / 2; }
 // T % 8.  Only tiles with x >= y hold work (xg >= yg); they are numbered row
 // by row (yt, then xt >= yt), lane block (z) outermost, so neighbouring tiles
 // of one row, which carry similar work, land on different XCDs.
+// Order 1 (chosen per (L, N) by the tuner) keeps the lane block innermost on
+// each XCD instead: the XCD runs the nz lane blocks of one tile back to
+// back, so the tile's coefficient rows (read once per lane block) are still
+// in its L2 for the later ones; the slots of tile q are q % 8 + 8 (nz (q / 8)
+// + z).  Both orders compute every output with the same arithmetic.
 struct CBlock {
   int xg, yg, z;
   bool ok;
@@ -67,18 +72,29 @@ __host__ __device__ inline int cblock_tiles(int nxg, int nyg) {  // per lane blo
   for (int yt = 0; yt < ty; ++yt) n += tx > yt ? tx - yt : 0;
   return n;
 }
-__host__ __device__ inline int64_t cblock_grid(int nxg, int nyg, int nz) {
-  const int64_t nt = (int64_t)nz * cblock_tiles(nxg, nyg);
+__host__ __device__ inline int64_t cblock_grid(int nxg, int nyg, int nz, int order) {
+  const int64_t per = cblock_tiles(nxg, nyg);
+  if (order == 1) return 64 * 8 * ((per + 7) / 8) * nz;
+  const int64_t nt = (int64_t)nz * per;
   return 64 * ((nt + 7) / 8 * 8);
 }
-__device__ __forceinline__ CBlock cblock(int nxg, int nyg, int nz) {
+__device__ __forceinline__ CBlock cblock(int nxg, int nyg, int nz, int order) {
   const unsigned p = blockIdx.x, k = p >> 3;
   const int T = (int)((p & 7) + 8 * (k >> 6)), w = (int)(k & 63);
   const int tx = (nxg + 7) / 8, ty = (nyg + 7) / 8;
   const int per = cblock_tiles(nxg, nyg);
   CBlock b;
-  b.z = T / per;
-  int r = T - b.z * per, yt = 0;
+  int r;
+  if (order == 1) {
+    const int j = T >> 3;  // slot on this XCD
+    b.z = j % nz;
+    r = (T & 7) + 8 * (j / nz);
+    if (r >= per) r = per, b.z = nz;  // past the last tile: no work
+  } else {
+    b.z = T / per;
+    r = T - b.z * per;
+  }
+  int yt = 0;
   while (yt < ty && r >= tx - yt) {  // row yt holds tiles xt = yt .. tx - 1
     r -= tx - yt;
     ++yt;
@@ -168,7 +184,7 @@ __global__ __launch_bounds__(256) void k_contract_inv(Geo g, const double2 *__re
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nmb = (g.L + 63) >> 6;
   const int Np = min(g.N, g.L);
-  const CBlock cb = cblock((g.L + kXG - 1) / kXG, (Np + 3) / 4, nmb);
+  const CBlock cb = cblock((g.L + kXG - 1) / kXG, (Np + 3) / 4, nmb, g.corder);
   const int mb = cb.z;
   const int x0 = cb.xg * kXG, y0 = cb.yg * 4;
   if (!cb.ok || y0 >= Np || x0 + kXG - 1 < y0 || x0 >= g.L) return;  // whole workgroup
@@ -353,7 +369,7 @@ __global__ __launch_bounds__(256) void k_contract_fwd(Geo g, const double *__res
   // grows with l, so the longest workgroups start first and the tail is short
   const int nlb = (g.L + 63) >> 6;
   const int Np = min(g.N, g.L);
-  const CBlock cb = cblock((g.L + kXG - 1) / kXG, (Np + 3) / 4, nlb);
+  const CBlock cb = cblock((g.L + kXG - 1) / kXG, (Np + 3) / 4, nlb, g.corder);
   const int lb = nlb - 1 - cb.z;
   const int x0 = cb.xg * kXG, y0 = cb.yg * 4;
   const int lo = lb * 64;
@@ -488,7 +504,7 @@ void launch_contract_inv(const Geo &g, const double2 *flmn, NBase nb, const DlTa
                          hipStream_t st) {
   const int Np = std::min(g.N, g.L);
   const int nmb = (g.L + 63) / 64;
-  const unsigned grid = (unsigned)cblock_grid((g.L + kXG - 1) / kXG, (Np + 3) / 4, nmb);
+  const unsigned grid = (unsigned)cblock_grid((g.L + kXG - 1) / kXG, (Np + 3) / 4, nmb, g.corder);
   if (g.NS == 2)
     k_contract_inv<2><<<grid, 256, 0, st>>>(g, flmn, nb.base, dl.ti, dl.offi, dl.nmt, zero_page(), X1);
   else
@@ -504,7 +520,7 @@ void launch_contract_fwd(const Geo &g, const double2 *Gp, NBase nb, const DlTabl
                          double2 *flmn, hipStream_t st) {
   const int Np = std::min(g.N, g.L);
   const int nlb = (g.L + 63) / 64;
-  const unsigned grid = (unsigned)cblock_grid((g.L + kXG - 1) / kXG, (Np + 3) / 4, nlb);
+  const unsigned grid = (unsigned)cblock_grid((g.L + kXG - 1) / kXG, (Np + 3) / 4, nlb, g.corder);
   if (g.NS == 2)
     k_contract_fwd<2><<<grid, 256, 0, st>>>(g, (const double *)Gp, nb.base, dl.tf, dl.offf, zero_page(), flmn);
   else

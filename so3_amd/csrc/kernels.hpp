@@ -24,6 +24,7 @@ struct Geo {
   int clmode;      // (2l+1)/(8 pi^2) factor: 1 in the inverse contraction
                    // (synthesis), 2 in the forward one (adjoint of the
                    // inverse), 0 nowhere (forward, adjoint of the forward)
+  int corder;      // contraction workgroup order (0 or 1, see contract.hip)
 };
 
 // Per-n data: base[n + N - 1] = storage index of (l=0, m=0, n) in the flmn

@@ -94,13 +94,31 @@ Geo make_geo(const so3_parameters_t *p, Method method) {
   return g;
 }
 
-using Key = std::tuple<int, int, int, int, int, int, int, int, int, int, int>;
+// ---- tuned choices per (L, N, reality, MW_SS): see tune() below
+struct TuneVal {
+  int corder_inv = 0, corder_fwd = 0;  // contraction workgroup orders
+  int pw = 0;                          // weight-convolution length (0: the default rule)
+};
+using TuneKey = std::tuple<int, int, int, int>;
+std::map<TuneKey, TuneVal> g_tune;
+
+TuneKey tune_key(const so3_parameters_t *p, Method method) {
+  const int ss = (method == kViaSsht && p->sampling_scheme == SO3_SAMPLING_MW_SS);
+  return TuneKey(p->L, p->N, p->reality ? 1 : 0, ss);
+}
+
+TuneVal tuned(const so3_parameters_t *p, Method method) {
+  auto it = g_tune.find(tune_key(p, method));
+  return it == g_tune.end() ? TuneVal{} : it->second;
+}
+
+using Key = std::tuple<int, int, int, int, int, int, int, int, int, int, int, int>;
 
 Key key_of(const so3_parameters_t *p, Method method, int dev) {
   const int ss = (method == kViaSsht && p->sampling_scheme == SO3_SAMPLING_MW_SS);
   const int steer = (method == kViaSsht && p->steerable) ? 1 : 0;
   return Key(dev, p->L, p->N, p->L0, ss, p->reality ? 1 : 0, (int)p->storage, (int)p->n_order,
-             (int)p->n_mode, (int)method, steer);
+             (int)p->n_mode, (int)method, steer, tuned(p, method).pw);
 }
 
 Plan build_plan(const so3_parameters_t *p, Method method) {
@@ -148,6 +166,7 @@ Plan build_plan(const so3_parameters_t *p, Method method) {
   // the register-resident forward beta (beta DFTs on Bluestein M = 1024, so
   // L = 129..256) convolves at 1024 points; any Pw >= 4L - 3 is exact
   if (L <= 256 && fft_plan(g.Pb).blue && fft_plan(g.Pb).len == 1024) P.Pw = 1024;
+  if (const int pw = tuned(p, method).pw) P.Pw = pw;  // the tuner's choice (>= 4L - 3)
   {
     const int Pw = P.Pw;
     std::vector<cld> w(Pw, cld(0, 0));
@@ -408,9 +427,10 @@ void prepare(const so3_parameters_t *p, Method method) {
 namespace {
 
 void enqueue_inverse_locked(const Plan &P, const DlTables &dl, const void *flmn, void *f, double2 *ws,
-                            hipStream_t st) {
+                            hipStream_t st, int corder) {
   double2 *X1 = ws, *X2 = ws + P.x1;
-  const Geo &g = P.g;
+  Geo g = P.g;
+  g.corder = corder;
   NBase nb{P.nbase};
   timed(0, st, [&] { launch_contract_inv(g, (const double2 *)flmn, nb, dl, X1, st); });
   timed(1, st, [&] { launch_inv_beta(g, X1, X2, nb, P.phase_inv, st); });
@@ -430,7 +450,8 @@ void enqueue_forward_locked(const Plan &P, const so3_parameters_t *p, Method met
                             double2 *ws, hipStream_t st) {
   const DlTables &dl = dl_tables(p->L, (int)p->dl_method);
   double2 *Gp = ws, *X2 = ws + P.x1;
-  const Geo &g = P.g;
+  Geo g = P.g;
+  g.corder = tuned(p, method).corder_fwd;
   NBase nb{P.nbase};
   if (g.steer) {
     timed(4, st, [&] { launch_gamma_steer(g, f, X2, -1, 2 * M_PI / g.N, st); });
@@ -458,7 +479,7 @@ void run_inverse(const so3_parameters_t *p, Method method, const void *flmn, voi
   std::lock_guard<std::mutex> lk(g_mu);
   const Plan &P = get_plan(p, method);
   const DlTables &dl = dl_tables(p->L, (int)p->dl_method);  // built (synchronously) before the arena is taken
-  enqueue_inverse_locked(P, dl, flmn, f, arena_acquire(P.x1 + P.x2, st), st);
+  enqueue_inverse_locked(P, dl, flmn, f, arena_acquire(P.x1 + P.x2, st), st, tuned(p, method).corder_inv);
   arena_release(st);
 }
 
@@ -482,7 +503,8 @@ size_t workspace_elems(const so3_parameters_t *p, Method method) {
 void enqueue_inverse(const so3_parameters_t *p, Method method, const void *flmn, void *f, void *ws,
                      hipStream_t st) {
   std::lock_guard<std::mutex> lk(g_mu);
-  enqueue_inverse_locked(get_plan(p, method), dl_tables(p->L, (int)p->dl_method), flmn, f, (double2 *)ws, st);
+  enqueue_inverse_locked(get_plan(p, method), dl_tables(p->L, (int)p->dl_method), flmn, f, (double2 *)ws, st,
+                         tuned(p, method).corder_inv);
 }
 
 void enqueue_forward(const so3_parameters_t *p, Method method, const void *f, void *flmn, void *ws,
@@ -540,12 +562,178 @@ void run_inverse_noalloc(const so3_parameters_t *p, Method method, const void *f
   const bool timing = g_timing;  // the stage profiler creates events: off for this call
   g_timing = false;
   try {
-    enqueue_inverse_locked(P, *dl, flmn, f, ws, st);
+    enqueue_inverse_locked(P, *dl, flmn, f, ws, st, tuned(p, method).corder_inv);
   } catch (...) {
     g_timing = timing;
     throw;
   }
   g_timing = timing;
+}
+
+// ---- autotuner
+//
+// For one (L, N, reality, sampling) the tuner times, on zero-filled buffers
+// of the real size (kernel times do not depend on the values):
+//   * the inverse and the forward contraction in both workgroup orders;
+//   * the forward beta stage with each candidate weight-convolution length
+//     Pw: the default rule and every other 2^a or 3 * 2^a in [4L - 3, 4096]
+//     up to twice the default that the workgroup FFT engine runs directly
+//     without Bluestein (any Pw >= 4L - 3 gives the exact linear
+//     convolution; the length decides which FFT engine and plan run it).
+// The fastest of each is kept and applies to every later transform with
+// these parameters, alone or batched (so batched results stay bit-identical
+// to single ones).  The workgroup order never changes a result; another Pw
+// changes the rounding only.
+namespace {
+
+// the default, then the other lengths the workgroup engine runs directly
+std::vector<int> pw_candidates(int L, int def) {
+  std::vector<int> c{def};
+  const int lo = 4 * L - 3, hi = std::min(4096, 2 * def);
+  for (int b = 1; b <= hi; b *= 2)
+    for (int m : {b, 3 * b})
+      if (m >= lo && m <= hi && std::find(c.begin(), c.end(), m) == c.end() && !fft_plan_lds(m).blue)
+        c.push_back(m);
+  std::sort(c.begin() + 1, c.end());
+  return c;
+}
+
+template <typename F>
+float best_ms(hipStream_t st, hipEvent_t a, hipEvent_t b, F &&fn) {
+  fn();  // warm-up (sets launch attributes)
+  float best = 1e30f;
+  for (int r = 0; r < 4; ++r) {
+    SO3_HIP(hipEventRecord(a, st));
+    fn();
+    SO3_HIP(hipEventRecord(b, st));
+    SO3_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    SO3_HIP(hipEventElapsedTime(&ms, a, b));
+    best = std::min(best, ms);
+  }
+  return best;
+}
+
+}  // namespace
+
+void tune(const so3_parameters_t *p, Method method, hipStream_t st, int *out, double *ms_out) {
+  validate(p, method);
+  std::lock_guard<std::mutex> lk(g_mu);
+  const TuneKey tk = tune_key(p, method);
+  g_tune.erase(tk);  // the default rule first
+  const Plan &P0 = get_plan(p, method);
+  const DlTables &dl = dl_tables(p->L, (int)p->dl_method);
+  const size_t nl = (size_t)so3_sampling_flmn_size(p) + 64;
+  double2 *flmn = nullptr, *ws = nullptr;
+  hipEvent_t a = nullptr, b = nullptr;
+  TuneVal best;
+  double ms[4] = {0, 0, 0, 0};
+  try {
+    SO3_HIP(hipMalloc(&flmn, sizeof(double2) * nl));
+    SO3_HIP(hipMalloc(&ws, sizeof(double2) * (P0.x1 + P0.x2)));
+    SO3_HIP(hipMemsetAsync(flmn, 0, sizeof(double2) * nl, st));
+    SO3_HIP(hipMemsetAsync(ws, 0, sizeof(double2) * (P0.x1 + P0.x2), st));
+    SO3_HIP(hipEventCreate(&a));
+    SO3_HIP(hipEventCreate(&b));
+    NBase nb{P0.nbase};
+    double2 *X1 = ws, *X2 = ws + P0.x1;
+    float bi = 1e30f, bf = 1e30f;
+    for (int order = 0; order < 2; ++order) {
+      Geo g = P0.g;
+      g.corder = order;
+      const float ti = best_ms(st, a, b, [&] { launch_contract_inv(g, flmn, nb, dl, X1, st); });
+      const float tf = best_ms(st, a, b, [&] { launch_contract_fwd(g, X1, nb, dl, flmn, st); });
+      if (ti < bi) bi = ti, best.corder_inv = order;
+      if (tf < bf) bf = tf, best.corder_fwd = order;
+    }
+    ms[0] = bi;
+    ms[1] = bf;
+    float bw = 1e30f;
+    for (int pw : pw_candidates(p->L, P0.Pw)) {
+      TuneVal t;
+      t.pw = pw == P0.Pw ? 0 : pw;
+      g_tune[tk] = t;
+      const Plan &Pc = get_plan(p, method);
+      const float tw = best_ms(st, a, b, [&] {
+        launch_fwd_beta(Pc.g, X2, X1, nb, Pc.phase_fwd, Pc.wr, Pc.wr_even, Pc.pole, Pc.Pw, 0, st);
+      });
+      if (tw < bw) bw = tw, best.pw = t.pw, ms[3] = pw;
+    }
+    ms[2] = bw;
+  } catch (...) {
+    g_tune.erase(tk);
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(flmn);
+    (void)hipFree(ws);
+    throw;
+  }
+  g_tune[tk] = best;
+  SO3_HIP(hipStreamSynchronize(st));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  SO3_HIP(hipFree(flmn));
+  SO3_HIP(hipFree(ws));
+  if (out) {
+    out[0] = best.corder_inv;
+    out[1] = best.corder_fwd;
+    out[2] = best.pw ? best.pw : P0.Pw;
+  }
+  if (ms_out)
+    for (int i = 0; i < 3; ++i) ms_out[i] = ms[i];
+}
+
+void tune_clear() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_tune.clear();
+}
+
+void tune_get(const so3_parameters_t *p, Method method, int *out) {
+  validate(p, method);
+  std::lock_guard<std::mutex> lk(g_mu);
+  const TuneVal t = tuned(p, method);
+  out[0] = t.corder_inv;
+  out[1] = t.corder_fwd;
+  out[2] = t.pw ? t.pw : get_plan(p, method).Pw;
+}
+
+// one line per entry: "L N reality mw_ss corder_inv corder_fwd pw" (pw 0: default)
+void tune_save(const char *path) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  FILE *fp = fopen(path, "w");
+  if (!fp) throw HipError(std::string("cannot write ") + path);
+  fprintf(fp, "# so3 MI355X tuning table: L N reality mw_ss corder_inv corder_fwd pw\n");
+  for (auto &kv : g_tune)
+    fprintf(fp, "%d %d %d %d %d %d %d\n", std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first),
+            std::get<3>(kv.first), kv.second.corder_inv, kv.second.corder_fwd, kv.second.pw);
+  if (fclose(fp) != 0) throw HipError(std::string("cannot write ") + path);
+}
+
+int tune_load(const char *path) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  FILE *fp = fopen(path, "r");
+  if (!fp) throw HipError(std::string("cannot read ") + path);
+  char line[256];
+  int n = 0;
+  while (fgets(line, sizeof line, fp)) {
+    if (line[0] == '#') continue;
+    int L, N, r, ss, ci, cf, pw;
+    if (sscanf(line, "%d %d %d %d %d %d %d", &L, &N, &r, &ss, &ci, &cf, &pw) != 7) continue;
+    if (L < 1 || N < 1 || (ci != 0 && ci != 1) || (cf != 0 && cf != 1) ||
+        (pw != 0 && (pw < 4 * L - 3 || pw > 4096 || (pw & (pw - 1) && (pw % 3 || (pw / 3) & (pw / 3 - 1)))))) {
+      fclose(fp);
+      throw HipError(std::string("invalid tuning entry in ") + path + ": " + line);
+    }
+    TuneVal t;
+    t.corder_inv = ci;
+    t.corder_fwd = cf;
+    t.pw = pw;
+    g_tune[TuneKey(L, N, r, ss)] = t;
+    ++n;
+  }
+  fclose(fp);
+  return n;
 }
 
 void set_stage_timing(bool on) {

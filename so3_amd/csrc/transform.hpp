@@ -23,6 +23,14 @@ void prepare(const so3_parameters_t *p, Method method);
 // of at most scratch_limit_bytes().
 void run_inverse_noalloc(const so3_parameters_t *p, Method method, const void *flmn, void *f, hipStream_t st);
 size_t scratch_limit_bytes();
+// Autotuner (transform.cpp): times the tuned choices for these parameters on
+// `st` and keeps the fastest; out = (inverse order, forward order, Pw),
+// ms_out = (inverse contraction, forward contraction, forward beta) times.
+void tune(const so3_parameters_t *p, Method method, hipStream_t st, int *out, double *ms_out);
+void tune_get(const so3_parameters_t *p, Method method, int *out);
+void tune_save(const char *path);
+void tune_clear();
+int tune_load(const char *path);
 
 // Batched execution: plan, tables and the workspace size (double2 elements)
 // of one transform (builds what is missing, so call it outside graph

@@ -37,6 +37,14 @@ class SO3Parameters:
         self.storage, self.n_mode = int(storage), int(n_mode)
         self.dl_method, self.steerable = int(dl_method), int(steerable)
 
+    def from_dict(self, d):
+        """Fill the fields from a dict (or a so3_parameters_t) with the same keys
+        (bindings.pyx:118-130)."""
+        get = (lambda k: d[k]) if isinstance(d, dict) else (lambda k: getattr(d, k))
+        for name, _ in so3_parameters_t._fields_:
+            setattr(self, name, int(get(name)))
+        return self
+
     def struct(self):
         s = so3_parameters_t()
         for name, _ in so3_parameters_t._fields_:
