@@ -51,6 +51,17 @@ int so3x_forward(void *flmn, const void *f, const so3_parameters_t *parameters, 
 int so3x_inverse_noalloc(void *f, const void *flmn, const so3_parameters_t *parameters, int method,
                          void *stream);
 
+/* The MATLAB sample layout of the reference's mex interface
+ * (src/matlab/so3_inverse_mex.c:310-339, so3_forward_mex.c:78-97):
+ * f_m[a nG nB + b nG + g] (gamma fastest) with real and imaginary parts in
+ * separate arrays, against the library's f[g nA nB + b nA + a].  Device
+ * memory, asynchronous on `stream`; im is NULL for real signals
+ * (parameters->reality = 1); from_mex takes im = NULL as zero imaginary
+ * parts.  nA, nB, nG are so3_sampling_nalpha / nbeta / ngamma. */
+int so3x_to_mex_layout(double *re, double *im, const void *f, const so3_parameters_t *parameters, void *stream);
+int so3x_from_mex_layout(void *f, const double *re, const double *im, const so3_parameters_t *parameters,
+                         void *stream);
+
 /* Build (or fetch) the Delta tables and DFT plans for these parameters. */
 int so3x_prepare(const so3_parameters_t *parameters);
 

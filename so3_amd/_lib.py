@@ -64,6 +64,8 @@ SIGNATURES = {
     "so3x_tune_save": (_I, [ctypes.c_char_p]),
     "so3x_tune_load": (_I, [ctypes.c_char_p]),
     "so3x_tune_clear": (_I, []),
+    "so3x_to_mex_layout": (_I, [_P, _P, _P, _PP, _P]),
+    "so3x_from_mex_layout": (_I, [_P, _P, _P, _PP, _P]),
     "so3x_workspace_bytes": (ctypes.c_longlong, [_PP]),
     "so3x_release_all": (_I, []),
     "so3x_cache_limit": (_I, [ctypes.c_longlong]),

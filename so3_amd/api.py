@@ -240,6 +240,18 @@ def forward_device(flmn_ptr, f_ptr, p, method=VIA_SSHT, stream=None):
                              ctypes.c_void_p(stream or 0)))
 
 
+def to_mex_layout_device(re_ptr, im_ptr, f_ptr, p, stream=None):
+    """f (library layout) -> MATLAB layout re / im on the device (im 0 for real p)."""
+    check(lib().so3x_to_mex_layout(ctypes.c_void_p(re_ptr), ctypes.c_void_p(im_ptr or 0), ctypes.c_void_p(f_ptr),
+                                   _s(p), ctypes.c_void_p(stream or 0)))
+
+
+def from_mex_layout_device(f_ptr, re_ptr, im_ptr, p, stream=None):
+    """MATLAB layout re / im -> f (library layout) on the device (im 0: zero imaginary parts)."""
+    check(lib().so3x_from_mex_layout(ctypes.c_void_p(f_ptr), ctypes.c_void_p(re_ptr), ctypes.c_void_p(im_ptr or 0),
+                                     _s(p), ctypes.c_void_p(stream or 0)))
+
+
 def prepare(p):
     check(lib().so3x_prepare(_s(p)))
 

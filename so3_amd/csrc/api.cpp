@@ -424,6 +424,26 @@ int so3x_tune_load(const char *path) {
   }
 }
 
+int so3x_to_mex_layout(double *re, double *im, const void *f, const so3_parameters_t *parameters, void *stream) {
+  try {
+    so3g::launch_mex_layout(true, const_cast<void *>(f), re, im, parameters, (hipStream_t)stream);
+    return 0;
+  } catch (const std::exception &e) {
+    return status(e);
+  }
+}
+
+int so3x_from_mex_layout(void *f, const double *re, const double *im, const so3_parameters_t *parameters,
+                         void *stream) {
+  try {
+    so3g::launch_mex_layout(false, f, const_cast<double *>(re), const_cast<double *>(im), parameters,
+                            (hipStream_t)stream);
+    return 0;
+  } catch (const std::exception &e) {
+    return status(e);
+  }
+}
+
 int so3x_prepare(const so3_parameters_t *parameters) {
   try {
     so3g::prepare(parameters, so3g::kViaSsht);

@@ -216,7 +216,7 @@ void free_graph(GraphRec &rec) {
   if (rec.x) (void)hipGraphExecDestroy(rec.x);
   if (rec.g) (void)hipGraphDestroy(rec.g);
   for (void *p : rec.slabs) (void)hipFree(p);
-  for (auto &pn : rec.pins) dl_tables_pin(pn.first, pn.second, -1);
+  for (auto &pn : rec.pins) dl_tables_unpin(pn.first, pn.second);
   rec = GraphRec{};
 }
 
@@ -271,15 +271,15 @@ long long graph_create(const so3x_item_t *items, int count, int lanes) {
       SO3_HIP(hipMalloc(&p, sizeof(double2) * std::max<size_t>(P.need[l], 1)));
       rec.slabs.push_back(p);
     }
+    // pin the table that serves each item (built by the eager run); the
+    // capture then records exactly these tables
     for (int i = 0; i < count; ++i) {
-      const std::pair<int, int> k(items[i].parameters.L, (int)items[i].parameters.dl_method);
+      const int L = items[i].parameters.L, m = (int)items[i].parameters.dl_method;
+      const std::pair<int, int> k(dl_tables(L, m).L, m);
       if (std::find(rec.pins.begin(), rec.pins.end(), k) != rec.pins.end()) continue;
-      dl_tables_pin(k.first, k.second, +1);
+      if (dl_tables_pin(L, m) != k.first) throw HipError("Delta table changed while pinning");
       rec.pins.push_back(k);
     }
-    // every table is resident now (built by the eager run, pinned since)
-    for (auto &pn : rec.pins)
-      if (!dl_tables_find(pn.first, pn.second)) throw HipError("Delta table evicted before capture");
     SO3_HIP(hipEventCreateWithFlags(&rec.done, hipEventDisableTiming));
     SO3_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed));
     try {

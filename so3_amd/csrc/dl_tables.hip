@@ -230,14 +230,34 @@ size_t table_bytes(int L) {
 
 }  // namespace
 
+// The smallest cached table of (dev, method) that covers L (its own L' >= L),
+// or end().  A table built for L' serves every L <= L': Delta^l does not
+// depend on L, each column's tiles for l < L are a prefix of its tiles for
+// l < L', and the tile entries with L <= l < L' meet only zero coefficients
+// (inverse) or outputs l >= L that are never stored (forward), so every
+// transform's result is bit-identical with either table.
+using TKey = std::tuple<int, int, int>;
+std::map<TKey, DlTables>::iterator covering(int dev, int L, int method) {
+  auto best = g_cache.end();
+  for (auto it = g_cache.begin(); it != g_cache.end(); ++it) {
+    const int d = std::get<0>(it->first), l = std::get<1>(it->first), m = std::get<2>(it->first);
+    if (d != dev || m != method || l < L) continue;
+    if (best == g_cache.end() || l < std::get<1>(best->first)) best = it;
+  }
+  return best;
+}
+
 const DlTables &dl_tables(int L, int method) {
   int dev = 0;
   SO3_HIP(hipGetDevice(&dev));
   std::lock_guard<std::mutex> g(g_mu);
+  auto it = covering(dev, L, method);
+  if (it != g_cache.end()) {
+    g_used[it->first] = ++g_clock;
+    return it->second;
+  }
   auto key = std::make_tuple(dev, L, method);
   g_used[key] = ++g_clock;
-  auto it = g_cache.find(key);
-  if (it != g_cache.end()) return it->second;
   make_room(table_bytes(L), key);
   return g_cache.emplace(key, build(L, method)).first->second;
 }
@@ -246,21 +266,30 @@ const DlTables *dl_tables_find(int L, int method) {
   int dev = 0;
   SO3_HIP(hipGetDevice(&dev));
   std::lock_guard<std::mutex> g(g_mu);
-  auto key = std::make_tuple(dev, L, method);
-  auto it = g_cache.find(key);
+  auto it = covering(dev, L, method);
   if (it == g_cache.end()) return nullptr;
-  g_used[key] = ++g_clock;
+  g_used[it->first] = ++g_clock;
   return &it->second;
 }
 
-void dl_tables_pin(int L, int method, int delta) {
+int dl_tables_pin(int L, int method) {
   int dev = 0;
   SO3_HIP(hipGetDevice(&dev));
   std::lock_guard<std::mutex> g(g_mu);
-  auto key = std::make_tuple(dev, L, method);
-  int &c = g_pins[key];
-  c += delta;
-  if (c <= 0) g_pins.erase(key);
+  auto it = covering(dev, L, method);
+  if (it == g_cache.end()) throw HipError("no Delta table to pin");
+  ++g_pins[it->first];
+  return std::get<1>(it->first);
+}
+
+void dl_tables_unpin(int table_L, int method) {
+  int dev = 0;
+  SO3_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> g(g_mu);
+  auto key = std::make_tuple(dev, table_L, method);
+  auto it = g_pins.find(key);
+  if (it == g_pins.end()) return;
+  if (--it->second <= 0) g_pins.erase(it);
 }
 
 void dl_tables_limit(size_t bytes) {

@@ -31,17 +31,20 @@ struct DlTables {
   float build_ms = 0.f;
 };
 
-// Cached per (device, L, method); builds on first use (synchronously).  The
-// cache keeps at most dl_tables_limit bytes of tables (32 GiB by default):
-// building a table that would exceed it first waits for the device and drops
-// the least recently used tables.
+// Cached per (device, table L, method).  A table built for L' serves every
+// L <= L' with bit-identical results (the smallest cached L' >= L is used),
+// so a mix of sizes needs one table per method, at its largest L.  Builds on
+// first use (synchronously).  The cache keeps at most dl_tables_limit bytes
+// of tables (32 GiB by default): building a table that would exceed it first
+// waits for the device and drops the least recently used unpinned tables.
 const DlTables &dl_tables(int L, int method);
 void dl_tables_limit(size_t bytes);
-// The cached table or nullptr (never builds, never allocates).
+// The table that serves L, or nullptr (never builds, never allocates).
 const DlTables *dl_tables_find(int L, int method);
-// Pin (delta = +1) or unpin (-1) a table: pinned tables are never evicted
-// (captured graphs hold their addresses).
-void dl_tables_pin(int L, int method, int delta);
+// Pin the table that serves L (returns its table L) / unpin it: pinned
+// tables are never evicted (captured graphs hold their addresses).
+int dl_tables_pin(int L, int method);
+void dl_tables_unpin(int table_L, int method);
 
 // Drop every cached table and zero page (frees device memory).
 void dl_tables_clear();

@@ -29,6 +29,9 @@ size_t scratch_limit_bytes();
 void tune(const so3_parameters_t *p, Method method, hipStream_t st, int *out, double *ms_out);
 void tune_get(const so3_parameters_t *p, Method method, int *out);
 void tune_save(const char *path);
+// MATLAB sample layout (layout.hip): to_mex f -> (re, im), else (re, im) -> f;
+// im is null for real signals (and may be null for a real-valued complex input)
+void launch_mex_layout(bool to_mex, void *f, double *re, double *im, const so3_parameters_t *p, hipStream_t st);
 void tune_clear();
 int tune_load(const char *path);
 

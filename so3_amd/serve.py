@@ -14,7 +14,8 @@ followed by the inverse transform) arrive over time.  The loop
   arrival) and, on request, its outputs.
 
 The Delta tables, DFT plans and index tables stay resident in the library
-between requests (the table cache is bounded; see so3x_cache_limit), so a
+between requests (the table cache is bounded; see so3x_cache_limit; one
+table per recursion method, at the largest L, serves every smaller L), so a
 request of a shape seen before allocates only its own buffers.
 """
 import collections
@@ -109,9 +110,11 @@ class Server:
         return spec, bufs
 
     def prepare(self, requests):
-        """Build the resident tables and plans of every shape in `requests`."""
+        """Build the resident tables and plans of every shape in `requests`.
+        Largest L first: the Delta table of the largest L serves every smaller
+        L, so one table per method is built."""
         seen = set()
-        for r in requests:
+        for r in sorted(requests, key=lambda r: -r.p.L):
             key = (r.p.L, r.p.N, r.p.reality, r.p.sampling_scheme, r.p.dl_method)
             if key not in seen:
                 seen.add(key)

@@ -62,7 +62,11 @@ def _items(specs, seed):
 
 
 def test_graph_survives_workspace_growth_and_eviction():
+    from so3_amd._lib import lib
     sid = torch.cuda.current_stream().cuda_stream
+    # start from an empty table cache, so that the L = 130 build below is the
+    # largest table and evicts the graph's tables unless they are pinned
+    assert lib().so3x_release_all() == 0
     items, keep = _items([(20, 8, 0), (33, 5, 1), (17, 17, 0)], 50)
     g = A.graph_create(items, lanes=2)
     try:
@@ -74,8 +78,9 @@ def test_graph_survives_workspace_growth_and_eviction():
         A.batch(big, lanes=2, stream=sid)
         torch.cuda.synchronize()
         # a table build under a zero limit evicts every table not pinned
+        # (L = 130 is larger than every cached table, so it is built)
         A.cache_limit(0)
-        A.prepare(P.SO3Parameters(L=41, N=3))
+        A.prepare(P.SO3Parameters(L=130, N=2))
         for rep in range(3):
             for _, f, b in keep:
                 f.zero_()
