@@ -144,10 +144,26 @@ typedef struct {
 
 int so3x_batch(const so3x_item_t *items, int count, int lanes, void *stream);
 
-/* Time this batch (already-prepared items) with 1, 2, 4 and 8 lanes on
- * `stream` and remember the fastest for batches of the same shape (the
- * multiset of item kinds and parameter blocks).  Runs the batch several
- * times, so outputs are overwritten.  Returns the chosen lane count. */
+/* Batch modes.  LANES: every item runs whole on one of the lanes (above).
+ * PERSISTENT: the batch runs in three phases -- the inverse contractions of
+ * all items in one persistent launch per signal type (complex / real), whose
+ * workgroups pull (item, block) work from device queues (one per XCD, with
+ * stealing), then each item's DFT stages on the lanes, then the forward
+ * contractions of all items in one such launch.  Each item then has its own
+ * workspace (summed over the items; batches beyond 64 GiB of workspaces run
+ * in consecutive groups).  Results are bit-identical in both modes.  AUTO:
+ * the mode and lane count so3x_batch_tune chose for this batch's shape when
+ * lanes <= 0 (LANES with 4 lanes when never tuned), LANES when lanes > 0. */
+enum { SO3X_BATCH_AUTO = 0, SO3X_BATCH_LANES = 1, SO3X_BATCH_PERSISTENT = 2 };
+int so3x_batch_persistent(const so3x_item_t *items, int count, int lanes, void *stream);
+/* The mode and lane count in force for this batch's shape (tuned or default). */
+int so3x_batch_choice(const so3x_item_t *items, int count, int *mode, int *lanes);
+
+/* Time this batch (already-prepared items) in both modes with 1, 2, 4 and 8
+ * lanes on `stream` and remember the fastest mode and lane count for batches
+ * of the same shape (the multiset of item kinds and parameter blocks).  Runs
+ * the batch several times, so outputs are overwritten.  Returns the chosen
+ * lane count (so3x_batch_choice reports the mode). */
 int so3x_batch_tune(const so3x_item_t *items, int count, void *stream);
 
 /* Capture a batch into a HIP graph (the items are prepared first, outside
@@ -157,6 +173,8 @@ int so3x_batch_tune(const so3x_item_t *items, int count, void *stream);
  * resident (so3x_cache_limit never evicts them); launches of one graph run
  * one after another, whatever streams they are made on. */
 int so3x_graph_create(const so3x_item_t *items, int count, int lanes, long long *graph);
+/* As so3x_graph_create, in the given batch mode (SO3X_BATCH_*). */
+int so3x_graph_create_mode(const so3x_item_t *items, int count, int lanes, int mode, long long *graph);
 int so3x_graph_launch(long long graph, void *stream);
 int so3x_graph_destroy(long long graph);
 

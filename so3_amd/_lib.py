@@ -90,6 +90,9 @@ SIGNATURES.update({
     "so3x_batch": (_I, [_PITEM, _I, _I, _P]),
     "so3x_batch_tune": (_I, [_PITEM, _I, _P]),
     "so3x_graph_create": (_I, [_PITEM, _I, _I, _PLL]),
+    "so3x_graph_create_mode": (_I, [_PITEM, _I, _I, _I, _PLL]),
+    "so3x_batch_persistent": (_I, [_PITEM, _I, _I, _P]),
+    "so3x_batch_choice": (_I, [_PITEM, _I, _PI, _PI]),
     "so3x_graph_launch": (_I, [ctypes.c_longlong, _P]),
     "so3x_graph_destroy": (_I, [ctypes.c_longlong]),
     "so3x_ticket": (_I, [_P, _PLL]),

@@ -347,6 +347,21 @@ def batch(items, lanes=0, stream=None):
     check(lib().so3x_batch(items, len(items), lanes, ctypes.c_void_p(stream or 0)))
 
 
+BATCH_AUTO, BATCH_LANES, BATCH_PERSISTENT = 0, 1, 2
+
+
+def batch_persistent(items, lanes=0, stream=None):
+    """Enqueue a batch in the persistent mode (grouped, queue-fed contractions)."""
+    check(lib().so3x_batch_persistent(items, len(items), lanes, ctypes.c_void_p(stream or 0)))
+
+
+def batch_choice(items):
+    """(mode, lanes) in force for this batch's shape."""
+    m, l = ctypes.c_int(), ctypes.c_int()
+    check(lib().so3x_batch_choice(items, len(items), ctypes.byref(m), ctypes.byref(l)))
+    return m.value, l.value
+
+
 def batch_tune(items, stream=None):
     r = lib().so3x_batch_tune(items, len(items), ctypes.c_void_p(stream or 0))
     if r < 0:
@@ -354,9 +369,9 @@ def batch_tune(items, stream=None):
     return r
 
 
-def graph_create(items, lanes=0):
+def graph_create(items, lanes=0, mode=BATCH_AUTO):
     h = ctypes.c_longlong()
-    check(lib().so3x_graph_create(items, len(items), lanes, ctypes.byref(h)))
+    check(lib().so3x_graph_create_mode(items, len(items), lanes, mode, ctypes.byref(h)))
     return h.value
 
 

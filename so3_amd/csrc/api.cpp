@@ -522,7 +522,25 @@ int so3x_device_info(char *buf, int len) {
 // ---------------------------------------------------------------- batches
 int so3x_batch(const so3x_item_t *items, int count, int lanes, void *stream) {
   try {
-    so3g::batch_enqueue(items, count, lanes, (hipStream_t)stream);
+    so3g::batch_enqueue(items, count, lanes, SO3X_BATCH_AUTO, (hipStream_t)stream);
+    return 0;
+  } catch (const std::exception &e) {
+    return status(e);
+  }
+}
+
+int so3x_batch_persistent(const so3x_item_t *items, int count, int lanes, void *stream) {
+  try {
+    so3g::batch_enqueue(items, count, lanes, SO3X_BATCH_PERSISTENT, (hipStream_t)stream);
+    return 0;
+  } catch (const std::exception &e) {
+    return status(e);
+  }
+}
+
+int so3x_batch_choice(const so3x_item_t *items, int count, int *mode, int *lanes) {
+  try {
+    so3g::batch_choice(items, count, mode, lanes);
     return 0;
   } catch (const std::exception &e) {
     return status(e);
@@ -540,7 +558,19 @@ int so3x_batch_tune(const so3x_item_t *items, int count, void *stream) {
 int so3x_graph_create(const so3x_item_t *items, int count, int lanes, long long *graph) {
   try {
     if (!graph) throw HipError("graph pointer is NULL");
-    *graph = so3g::graph_create(items, count, lanes);
+    *graph = so3g::graph_create(items, count, lanes, SO3X_BATCH_AUTO);
+    return 0;
+  } catch (const std::exception &e) {
+    return status(e);
+  }
+}
+
+int so3x_graph_create_mode(const so3x_item_t *items, int count, int lanes, int mode, long long *graph) {
+  try {
+    if (!graph) throw HipError("graph pointer is NULL");
+    if (mode != SO3X_BATCH_AUTO && mode != SO3X_BATCH_LANES && mode != SO3X_BATCH_PERSISTENT)
+      throw HipError("invalid batch mode");
+    *graph = so3g::graph_create(items, count, lanes, mode);
     return 0;
   } catch (const std::exception &e) {
     return status(e);

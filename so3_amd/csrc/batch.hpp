@@ -6,9 +6,13 @@
 
 namespace so3g {
 
-void batch_enqueue(const so3x_item_t *items, int count, int lanes, hipStream_t st);
+// mode: SO3X_BATCH_AUTO (the tuned mode and lane count for this batch's
+// shape when lanes <= 0, else lanes), SO3X_BATCH_LANES or
+// SO3X_BATCH_PERSISTENT
+void batch_enqueue(const so3x_item_t *items, int count, int lanes, int mode, hipStream_t st);
 int batch_tune(const so3x_item_t *items, int count, hipStream_t st);
-long long graph_create(const so3x_item_t *items, int count, int lanes);
+void batch_choice(const so3x_item_t *items, int count, int *mode, int *lanes);
+long long graph_create(const so3x_item_t *items, int count, int lanes, int mode);
 void graph_launch(long long graph, hipStream_t st);
 void graph_destroy(long long graph);
 long long ticket_new(hipStream_t st);

@@ -78,8 +78,8 @@ __host__ __device__ inline int64_t cblock_grid(int nxg, int nyg, int nz, int ord
   const int64_t nt = (int64_t)nz * per;
   return 64 * ((nt + 7) / 8 * 8);
 }
-__device__ __forceinline__ CBlock cblock(int nxg, int nyg, int nz, int order) {
-  const unsigned p = blockIdx.x, k = p >> 3;
+__device__ __forceinline__ CBlock cblock(unsigned p, int nxg, int nyg, int nz, int order) {
+  const unsigned k = p >> 3;
   const int T = (int)((p & 7) + 8 * (k >> 6)), w = (int)(k & 63);
   const int tx = (nxg + 7) / 8, ty = (nyg + 7) / 8;
   const int per = cblock_tiles(nxg, nyg);
@@ -170,21 +170,22 @@ __device__ __forceinline__ void wait_ring(int ahead, int n) {
 // ------------------------------------------------------------------ inverse
 // Workgroup: x group x0 .. x0+3, y0 + w for wavefront w, m' block mpb ..
 // mpb+63 (four 16-wide tiles); reduction over l in chunks of 4.
+// Block `blk` of the transform's grid; sm is the workgroup's LDS ring
+// (kRing * kBufD doubles).  Returns early for blocks without work.
 template <int NS>
-__global__ __launch_bounds__(256) void k_contract_inv(Geo g, const double2 *__restrict__ flmn,
-                                                      const int64_t *__restrict__ nbase,
-                                                      const double *__restrict__ ti,
-                                                      const int64_t *__restrict__ offi, int nmt,
-                                                      const double *__restrict__ zero,
-                                                      double2 *__restrict__ X1) {
+__device__ __forceinline__ void contract_inv_block(unsigned blk, const Geo &g, const double2 *__restrict__ flmn,
+                                                   const int64_t *__restrict__ nbase,
+                                                   const double *__restrict__ ti,
+                                                   const int64_t *__restrict__ offi, int nmt,
+                                                   const double *__restrict__ zero,
+                                                   double2 *__restrict__ X1, double *sm) {
   constexpr int C = 2 * NS;
   using St = Stage<C>;
-  __shared__ __attribute__((aligned(16))) double sm[kRing * kBufD];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nmb = (g.L + 63) >> 6;
   const int Np = min(g.N, g.L);
-  const CBlock cb = cblock((g.L + kXG - 1) / kXG, (Np + 3) / 4, nmb, g.corder);
+  const CBlock cb = cblock(blk, (g.L + kXG - 1) / kXG, (Np + 3) / 4, nmb, g.corder);
   const int mb = cb.z;
   const int x0 = cb.xg * kXG, y0 = cb.yg * 4;
   if (!cb.ok || y0 >= Np || x0 + kXG - 1 < y0 || x0 >= g.L) return;  // whole workgroup
@@ -336,6 +337,17 @@ __global__ __launch_bounds__(256) void k_contract_inv(Geo g, const double2 *__re
   }
 }
 
+template <int NS>
+__global__ __launch_bounds__(256) void k_contract_inv(Geo g, const double2 *__restrict__ flmn,
+                                                      const int64_t *__restrict__ nbase,
+                                                      const double *__restrict__ ti,
+                                                      const int64_t *__restrict__ offi, int nmt,
+                                                      const double *__restrict__ zero,
+                                                      double2 *__restrict__ X1) {
+  __shared__ __attribute__((aligned(16))) double sm[kRing * kBufD];
+  contract_inv_block<NS>(blockIdx.x, g, flmn, nbase, ti, offi, nmt, zero, X1, sm);
+}
+
 // X1 rows of |n| in [L, N): no l reaches them
 __global__ void k_zero_rows(Geo g, double2 *__restrict__ X1) {
   const int an = g.L + blockIdx.y;
@@ -354,22 +366,21 @@ __global__ void k_zero_rows(Geo g, double2 *__restrict__ X1) {
 // Workgroup: x group x0 .. x0+3, y0 + w for wavefront w, l block lo .. lo+63
 // (four 16-wide tiles); reduction over m' in chunks of 4.
 template <int NS>
-__global__ __launch_bounds__(256) void k_contract_fwd(Geo g, const double *__restrict__ Gp,
-                                                      const int64_t *__restrict__ nbase,
-                                                      const double *__restrict__ tf,
-                                                      const int64_t *__restrict__ offf,
-                                                      const double *__restrict__ zero,
-                                                      double2 *__restrict__ flmn) {
+__device__ __forceinline__ void contract_fwd_block(unsigned blk, const Geo &g, const double *__restrict__ Gp,
+                                                   const int64_t *__restrict__ nbase,
+                                                   const double *__restrict__ tf,
+                                                   const int64_t *__restrict__ offf,
+                                                   const double *__restrict__ zero,
+                                                   double2 *__restrict__ flmn, double *sm) {
   constexpr int C = 2 * NS;
   using St = Stage<C>;
-  __shared__ __attribute__((aligned(16))) double sm[kRing * kBufD];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   // l blocks are the outermost tile dimension, highest first: their work
   // grows with l, so the longest workgroups start first and the tail is short
   const int nlb = (g.L + 63) >> 6;
   const int Np = min(g.N, g.L);
-  const CBlock cb = cblock((g.L + kXG - 1) / kXG, (Np + 3) / 4, nlb, g.corder);
+  const CBlock cb = cblock(blk, (g.L + kXG - 1) / kXG, (Np + 3) / 4, nlb, g.corder);
   const int lb = nlb - 1 - cb.z;
   const int x0 = cb.xg * kXG, y0 = cb.yg * 4;
   const int lo = lb * 64;
@@ -498,6 +509,99 @@ __global__ __launch_bounds__(256) void k_contract_fwd(Geo g, const double *__res
   }
 }
 
+template <int NS>
+__global__ __launch_bounds__(256) void k_contract_fwd(Geo g, const double *__restrict__ Gp,
+                                                      const int64_t *__restrict__ nbase,
+                                                      const double *__restrict__ tf,
+                                                      const int64_t *__restrict__ offf,
+                                                      const double *__restrict__ zero,
+                                                      double2 *__restrict__ flmn) {
+  __shared__ __attribute__((aligned(16))) double sm[kRing * kBufD];
+  contract_fwd_block<NS>(blockIdx.x, g, Gp, nbase, tf, offf, zero, flmn, sm);
+}
+
+// ------------------------------------------------------------------ grouped
+// One persistent launch runs the contraction blocks of many transforms (the
+// items of a batch).  Item i owns virtual blocks off_i .. off_i + nblk_i - 1,
+// in the order of its own grid (nblk_i and off_i are multiples of 8, so a
+// virtual block v keeps the XCD v % 8 that the item's block order was laid
+// out for).  The work is handed out by eight device queues, one per XCD:
+// queue x holds the virtual blocks v = x + 8 k, k = 0, 1, ...; a workgroup
+// takes the next k of its own XCD's queue with an atomic add, and once that
+// queue is empty it takes from the other seven in turn, so every block runs
+// exactly once whatever XCD a workgroup reports, and every workgroup exits
+// after seeing all eight queues empty.  The queues are zeroed before each
+// launch (k_queue_zero).  Each block runs the per-transform body above, so
+// results are bit-identical to the transform's own launch.
+__device__ __forceinline__ unsigned xcc_id() {
+  unsigned x;
+  asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(x));
+  return x & 7u;
+}
+
+// thread 0 takes the next virtual block (or ~0u when all queues are empty)
+// and finds its item; both are broadcast through LDS
+__device__ __forceinline__ void grouped_next(const CItem *__restrict__ items, int nitems, unsigned per_queue,
+                                             unsigned *__restrict__ queues, unsigned xcc, int &tried,
+                                             unsigned *s_v, int *s_item) {
+  if (threadIdx.x == 0) {
+    unsigned v = ~0u;
+    while (tried < 8) {
+      const unsigned x = (xcc + (unsigned)tried) & 7u;
+      const unsigned k = atomicAdd(&queues[x], 1u);
+      if (k < per_queue) {
+        v = x + 8u * k;
+        break;
+      }
+      ++tried;
+    }
+    int it = 0;
+    if (v != ~0u) {
+      int lo = 0, hi = nitems - 1;  // last item with off <= v
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].off <= v) lo = mid;
+        else hi = mid - 1;
+      }
+      it = lo;
+    }
+    *s_v = v;
+    *s_item = it;
+  }
+  __syncthreads();
+}
+
+template <int NS, int DIR>
+__global__ __launch_bounds__(256) void k_contract_grouped(const CItem *__restrict__ items, int nitems,
+                                                          unsigned per_queue, unsigned *__restrict__ queues,
+                                                          const double *__restrict__ zero) {
+  __shared__ __attribute__((aligned(16))) double sm[kRing * kBufD];
+  __shared__ unsigned s_v;
+  __shared__ int s_item;
+  const unsigned xcc = xcc_id();
+  int tried = 0;
+  for (;;) {
+    grouped_next(items, nitems, per_queue, queues, xcc, tried, &s_v, &s_item);
+    const unsigned v = __builtin_amdgcn_readfirstlane(s_v);
+    const int i = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();  // every wave has read s_v / s_item before thread 0 writes them again
+    if (v == ~0u) break;
+    const CItem &it = items[i];
+    const Geo g = it.g;
+    if (DIR > 0)
+      contract_inv_block<NS>(v - it.off, g, (const double2 *)it.in, it.nbase, it.tab, it.off_tab, it.nmt, zero,
+                             it.out, sm);
+    else
+      contract_fwd_block<NS>(v - it.off, g, (const double *)it.in, it.nbase, it.tab, it.off_tab, zero, it.out, sm);
+    // the block's last LDS reads are done before the next block's copies land
+    __syncthreads();
+  }
+}
+
+__global__ void k_queue_zero(unsigned *q, int n) {
+  if ((int)threadIdx.x < n) q[threadIdx.x] = 0u;
+}
+
 }  // namespace
 
 void launch_contract_inv(const Geo &g, const double2 *flmn, NBase nb, const DlTables &dl, double2 *X1,
@@ -526,6 +630,46 @@ void launch_contract_fwd(const Geo &g, const double2 *Gp, NBase nb, const DlTabl
   else
     k_contract_fwd<1><<<grid, 256, 0, st>>>(g, (const double *)Gp, nb.base, dl.tf, dl.offf, zero_page(), flmn);
   SO3_LAUNCH_CHECK();
+}
+
+unsigned contract_blocks(const Geo &g) {
+  const int Np = std::min(g.N, g.L);
+  const int nz = (g.L + 63) / 64;
+  return (unsigned)cblock_grid((g.L + kXG - 1) / kXG, (Np + 3) / 4, nz, g.corder);
+}
+
+void launch_contract_grouped(const CItem *items_dev, int nitems, unsigned total_blocks, int ns, int dir,
+                             unsigned *queues, hipStream_t st) {
+  if (nitems <= 0 || total_blocks == 0) return;
+  if (total_blocks % 8) throw HipError("grouped contraction: block count not a multiple of 8");
+  k_queue_zero<<<1, 64, 0, st>>>(queues, 8);
+  SO3_LAUNCH_CHECK();
+  // persistent grid: the workgroups that fit at once (two per CU), at most
+  // one per block
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    SO3_HIP(hipGetDevice(&dev));
+    SO3_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  }
+  const unsigned grid = std::min<unsigned>(total_blocks, (unsigned)(2 * cus));
+  const unsigned per_queue = total_blocks / 8;
+  const double *z = zero_page();
+  if (dir > 0) {
+    if (ns == 2) k_contract_grouped<2, 1><<<grid, 256, 0, st>>>(items_dev, nitems, per_queue, queues, z);
+    else k_contract_grouped<1, 1><<<grid, 256, 0, st>>>(items_dev, nitems, per_queue, queues, z);
+  } else {
+    if (ns == 2) k_contract_grouped<2, -1><<<grid, 256, 0, st>>>(items_dev, nitems, per_queue, queues, z);
+    else k_contract_grouped<1, -1><<<grid, 256, 0, st>>>(items_dev, nitems, per_queue, queues, z);
+  }
+  SO3_LAUNCH_CHECK();
+}
+
+void launch_zero_rows(const Geo &g, double2 *X1, hipStream_t st) {
+  if (g.N > g.L) {
+    k_zero_rows<<<dim3(64, g.N - g.L), 256, 0, st>>>(g, X1);
+    SO3_LAUNCH_CHECK();
+  }
 }
 
 }  // namespace so3g

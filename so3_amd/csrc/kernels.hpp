@@ -58,6 +58,30 @@ __host__ __device__ inline int64_t x2_row(const Geo &g, int nslot, int b) {
 }
 void launch_contract_inv(const Geo &g, const double2 *flmn, NBase nb, const DlTables &dl, double2 *X1,
                          hipStream_t st);
+
+// One item of a grouped (persistent, queue-fed) contraction launch: the
+// transform's geometry, its input (flmn inverse, G' forward) and output (X1
+// inverse, flmn forward), n bases, Delta table (ti / offi / nmt inverse,
+// tf / offf forward), its block count (contract_blocks) and first virtual
+// block (items sorted by off, both multiples of 8).
+struct CItem {
+  Geo g;
+  const void *in;
+  double2 *out;
+  const int64_t *nbase;
+  const double *tab;
+  const int64_t *off_tab;
+  int nmt;
+  unsigned nblk, off;
+};
+unsigned contract_blocks(const Geo &g);
+// items_dev: device array of nitems CItem; queues: 8 device counters
+// (zeroed by the launch itself); dir +1 inverse, -1 forward; ns 2 complex,
+// 1 real.  The inverse's zero rows (|n| >= L) are not written: see
+// launch_zero_rows.
+void launch_contract_grouped(const CItem *items_dev, int nitems, unsigned total_blocks, int ns, int dir,
+                             unsigned *queues, hipStream_t st);
+void launch_zero_rows(const Geo &g, double2 *X1, hipStream_t st);
 void launch_contract_fwd(const Geo &g, const double2 *Gp, NBase nb, const DlTables &dl,
                          double2 *flmn, hipStream_t st);
 

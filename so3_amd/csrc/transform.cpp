@@ -426,13 +426,23 @@ void prepare(const so3_parameters_t *p, Method method) {
 
 namespace {
 
+void inverse_post_locked(const Plan &P, void *f, double2 *ws, hipStream_t st);
+
 void enqueue_inverse_locked(const Plan &P, const DlTables &dl, const void *flmn, void *f, double2 *ws,
                             hipStream_t st, int corder) {
-  double2 *X1 = ws, *X2 = ws + P.x1;
+  double2 *X1 = ws;
   Geo g = P.g;
   g.corder = corder;
   NBase nb{P.nbase};
   timed(0, st, [&] { launch_contract_inv(g, (const double2 *)flmn, nb, dl, X1, st); });
+  inverse_post_locked(P, f, ws, st);
+}
+
+// the inverse's stages after the contraction (X1 at ws)
+void inverse_post_locked(const Plan &P, void *f, double2 *ws, hipStream_t st) {
+  double2 *X1 = ws, *X2 = ws + P.x1;
+  const Geo &g = P.g;
+  NBase nb{P.nbase};
   timed(1, st, [&] { launch_inv_beta(g, X1, X2, nb, P.phase_inv, st); });
   if (g.steer) {
     timed(2, st, [&] { launch_alpha(g, X2, +1, 1.0, st); });
@@ -446,12 +456,11 @@ void enqueue_inverse_locked(const Plan &P, const DlTables &dl, const void *flmn,
   }
 }
 
-void enqueue_forward_locked(const Plan &P, const so3_parameters_t *p, Method method, const void *f, void *flmn,
-                            double2 *ws, hipStream_t st) {
-  const DlTables &dl = dl_tables(p->L, (int)p->dl_method);
+// the forward's stages before the contraction (G' left at ws)
+void forward_pre_locked(const Plan &P, const so3_parameters_t *p, Method method, const void *f, void *flmn,
+                        double2 *ws, hipStream_t st) {
   double2 *Gp = ws, *X2 = ws + P.x1;
-  Geo g = P.g;
-  g.corder = tuned(p, method).corder_fwd;
+  const Geo &g = P.g;
   NBase nb{P.nbase};
   if (g.steer) {
     timed(4, st, [&] { launch_gamma_steer(g, f, X2, -1, 2 * M_PI / g.N, st); });
@@ -469,7 +478,16 @@ void enqueue_forward_locked(const Plan &P, const so3_parameters_t *p, Method met
   timed(7, st, [&] {
     launch_fwd_zero(g, (double2 *)flmn, nb, method == kDirect, 0, (int)p->storage, (int)p->n_order, st);
   });
-  timed(8, st, [&] { launch_contract_fwd(g, Gp, nb, dl, (double2 *)flmn, st); });
+}
+
+void enqueue_forward_locked(const Plan &P, const so3_parameters_t *p, Method method, const void *f, void *flmn,
+                            double2 *ws, hipStream_t st) {
+  const DlTables &dl = dl_tables(p->L, (int)p->dl_method);
+  forward_pre_locked(P, p, method, f, flmn, ws, st);
+  Geo g = P.g;
+  g.corder = tuned(p, method).corder_fwd;
+  NBase nb{P.nbase};
+  timed(8, st, [&] { launch_contract_fwd(g, ws, nb, dl, (double2 *)flmn, st); });
 }
 
 }  // namespace
@@ -734,6 +752,38 @@ int tune_load(const char *path) {
   }
   fclose(fp);
   return n;
+}
+
+// ---- grouped batches: a transform split at its contraction
+void contract_item(const so3_parameters_t *p, Method method, bool inverse, const void *in, void *out, void *ws,
+                   CItem *it) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  const Plan &P = get_plan(p, method);
+  const DlTables &dl = dl_tables(p->L, (int)p->dl_method);
+  const TuneVal tv = tuned(p, method);
+  *it = CItem{};
+  it->g = P.g;
+  it->g.corder = inverse ? tv.corder_inv : tv.corder_fwd;
+  it->in = inverse ? in : ws;
+  it->out = inverse ? (double2 *)ws : (double2 *)out;
+  it->nbase = P.nbase;
+  it->tab = inverse ? dl.ti : dl.tf;
+  it->off_tab = inverse ? dl.offi : dl.offf;
+  it->nmt = dl.nmt;
+  it->nblk = contract_blocks(it->g);
+}
+
+void enqueue_inverse_post(const so3_parameters_t *p, Method method, void *f, void *ws, hipStream_t st) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  const Plan &P = get_plan(p, method);
+  launch_zero_rows(P.g, (double2 *)ws, st);
+  inverse_post_locked(P, f, (double2 *)ws, st);
+}
+
+void enqueue_forward_pre(const so3_parameters_t *p, Method method, const void *f, void *flmn, void *ws,
+                         hipStream_t st) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  forward_pre_locked(get_plan(p, method), p, method, f, flmn, (double2 *)ws, st);
 }
 
 void set_stage_timing(bool on) {

@@ -46,6 +46,17 @@ void enqueue_inverse(const so3_parameters_t *p, Method method, const void *flmn_
 void enqueue_forward(const so3_parameters_t *p, Method method, const void *f_dev, void *flmn_dev, void *ws,
                      hipStream_t st);
 size_t workspace_bytes(const so3_parameters_t *p, Method method);
+// Grouped batches: one transform split at its contraction.  contract_item
+// fills the item of a grouped contraction launch (inverse: flmn `in` -> X1
+// at ws; forward: G' at ws -> flmn `out`); enqueue_inverse_post runs the
+// inverse's stages after its contraction, enqueue_forward_pre the forward's
+// stages before it.  Same kernels and arithmetic as enqueue_inverse /
+// enqueue_forward, so the results are bit-identical.
+void contract_item(const so3_parameters_t *p, Method method, bool inverse, const void *in, void *out, void *ws,
+                   CItem *it);
+void enqueue_inverse_post(const so3_parameters_t *p, Method method, void *f, void *ws, hipStream_t st);
+void enqueue_forward_pre(const so3_parameters_t *p, Method method, const void *f, void *flmn, void *ws,
+                         hipStream_t st);
 void release_all();
 
 // Stage profiler: when enabled, every stage launch is bracketed by HIP

@@ -185,8 +185,8 @@ def _batched_workload(seed=4, count=96):
 
 
 def test_c3_full_batch_bit_identical():
-    """all 96 items of the uneven batch: batched (eager and graph replay) ==
-    alone, bit for bit"""
+    """all 96 items of the uneven batch: batched (eager and graph replay, in
+    the lanes and the persistent mode) == alone, bit for bit"""
     ps = _batched_workload()
     stream = torch.cuda.current_stream()
     sid = stream.cuda_stream
@@ -211,18 +211,27 @@ def test_c3_full_batch_bit_identical():
     torch.cuda.synchronize()
     for i, ((f, b), (rf, rb)) in enumerate(zip(outs, refs)):
         assert torch.equal(f, rf) and torch.equal(b, rb), (i, ps[i])
-    g = A.graph_create(items, lanes=8)
-    try:
-        for f, b in outs:
-            f.zero_()
-            b.zero_()
-        torch.cuda.synchronize()
-        A.graph_launch(g, sid)
-        torch.cuda.synchronize()
-        for i, ((f, b), (rf, rb)) in enumerate(zip(outs, refs)):
-            assert torch.equal(f, rf) and torch.equal(b, rb), (i, ps[i])
-    finally:
-        A.graph_destroy(g)
+    for mode in (A.BATCH_LANES, A.BATCH_PERSISTENT):
+        g = A.graph_create(items, lanes=8, mode=mode)
+        try:
+            for f, b in outs:
+                f.zero_()
+                b.zero_()
+            torch.cuda.synchronize()
+            A.graph_launch(g, sid)
+            torch.cuda.synchronize()
+            for i, ((f, b), (rf, rb)) in enumerate(zip(outs, refs)):
+                assert torch.equal(f, rf) and torch.equal(b, rb), (mode, i, ps[i])
+        finally:
+            A.graph_destroy(g)
+    # the persistent mode, eager
+    for f, b in outs:
+        f.zero_()
+        b.zero_()
+    A.batch_persistent(items, lanes=8, stream=sid)
+    torch.cuda.synchronize()
+    for i, ((f, b), (rf, rb)) in enumerate(zip(outs, refs)):
+        assert torch.equal(f, rf) and torch.equal(b, rb), ("persistent", i, ps[i])
 
 
 def test_c4_serving_slice_bit_identical():
