@@ -126,13 +126,11 @@ def batched_workload(P, seed=4, count=96):
     return out
 
 
-def batched_bench(A, P, torch, reps=5):
-    """96 uneven fwd+inv pairs: plain calls one after another on one stream
-    versus one batch call (tuned lane count) and its captured graph."""
+def _pair_specs(A, torch, ps, seed):
+    """device buffers and batch items of fwd+inv pairs for parameter list ps"""
     dev = torch.device("cuda:0")
-    ps = batched_workload(P)
     g = torch.Generator(device=dev)
-    g.manual_seed(4)
+    g.manual_seed(seed)
     specs, keep = [], []
     for p in ps:
         nfl = A.flmn_size(p)
@@ -143,6 +141,12 @@ def batched_bench(A, P, torch, reps=5):
         keep += [flmn, out, f]
         specs.append({"p": p, "kind": A.ITEM_PAIR, "f": f.data_ptr(), "flmn_in": flmn.data_ptr(),
                       "flmn_out": out.data_ptr()})
+    return specs, keep
+
+
+def _batch_legs(A, torch, specs, reps):
+    """plain sequential calls, then the batch in both modes, eager and as a
+    captured graph (medians of `reps` runs, milliseconds)"""
     items = A.make_items(specs)
     stream = torch.cuda.current_stream()
     sid = stream.cuda_stream
@@ -163,30 +167,81 @@ def batched_bench(A, P, torch, reps=5):
             A.inverse_device(sp["f"], sp["flmn_in"], sp["p"], A.VIA_SSHT, sid)
             A.forward_device(sp["flmn_out"], sp["f"], sp["p"], A.VIA_SSHT, sid)
 
-    t_plain = timeit(plain)
+    out = {"plain": timeit(plain)}
     lanes = A.batch_tune(items, stream=sid)
-    t_batch = timeit(lambda: A.batch(items, lanes=lanes, stream=sid))
-    gh = A.graph_create(items, lanes=lanes)
-    try:
-        t_graph = timeit(lambda: A.graph_launch(gh, sid))
-    finally:
-        A.graph_destroy(gh)
-    n = len(items)
-    best = min(np.median(t_batch), np.median(t_graph))
+    mode, _ = A.batch_choice(items)
+    out["lanes"] = timeit(lambda: A.batch(items, lanes=lanes, stream=sid))
+    out["persistent"] = timeit(lambda: A.batch_persistent(items, lanes=lanes, stream=sid))
+    for m, name in ((A.BATCH_LANES, "lanes_graph"), (A.BATCH_PERSISTENT, "persistent_graph")):
+        gh = A.graph_create(items, lanes=lanes, mode=m)
+        try:
+            out[name] = timeit(lambda: A.graph_launch(gh, sid))
+        finally:
+            A.graph_destroy(gh)
+    med = {k: float(np.median(v)) for k, v in out.items()}
+    best = min((v, k) for k, v in med.items() if k != "plain")
+    n = len(specs)
     return {
-        "workload": "96 uneven fwd+inv pairs: L in {32,48,...,384}, N in [1,min(L,64)], MW/MW_SS, "
-                    "complex/real (seed 4), via SSHT, Risbo, padded",
-        "items": n,
-        "lanes": lanes,
-        "plain_ms": float(np.median(t_plain)),
-        "batch_ms": float(np.median(t_batch)),
-        "graph_ms": float(np.median(t_graph)),
-        "plain_pairs_per_s": n / (np.median(t_plain) * 1e-3),
-        "batched_pairs_per_s": n / (best * 1e-3),
-        "speedup": float(np.median(t_plain) / best),
-        "p50_ms": float(np.percentile(np.concatenate([t_batch, t_graph]), 50)),
-        "p99_ms": float(np.percentile(np.concatenate([t_batch, t_graph]), 99)),
+        "items": n, "tuned_lanes": lanes, "tuned_mode": "persistent" if mode == A.BATCH_PERSISTENT else "lanes",
+        "ms": {k: round(v, 3) for k, v in med.items()},
+        "plain_pairs_per_s": n / (med["plain"] * 1e-3),
+        "batched_pairs_per_s": n / (best[0] * 1e-3),
+        "best": best[1],
+        "speedup": med["plain"] / best[0],
+        "p50_ms": float(np.percentile(out[best[1]], 50)),
+        "p99_ms": float(np.percentile(out[best[1]], 99)),
     }
+
+
+def batched_bench(A, P, torch, reps=5):
+    """96 uneven fwd+inv pairs: plain calls one after another on one stream
+    versus one batch call (lanes or persistent mode, tuned) and its captured
+    graph, after the (L, N) autotuner has run for every item shape; and the
+    same comparison on a stream of 256 small pairs (L 16..63), where the
+    batch modes matter most."""
+    ps = batched_workload(P)
+    specs, keep = _pair_specs(A, torch, ps, 4)
+    # plain before tuning, for reference
+    t0 = time.perf_counter()
+    sid = torch.cuda.current_stream().cuda_stream
+
+    def plain_once():
+        for sp in specs:
+            A.inverse_device(sp["f"], sp["flmn_in"], sp["p"], A.VIA_SSHT, sid)
+            A.forward_device(sp["flmn_out"], sp["f"], sp["p"], A.VIA_SSHT, sid)
+    plain_once()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain_once()
+    torch.cuda.synchronize()
+    untuned_ms = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    shapes, defaults = {}, {}
+    for p in ps:
+        key = (p.L, p.N, p.reality, p.sampling_scheme)
+        if key not in shapes:
+            defaults[key] = A.tune_get(p)["pw"]
+            shapes[key] = A.tune(p)[0]
+    tune_s = time.perf_counter() - t0
+    progress("batched: autotuner over %d shapes" % len(shapes))
+    out = _batch_legs(A, torch, specs, reps)
+    progress("batched: 96 uneven pairs")
+    out["workload"] = ("96 uneven fwd+inv pairs: L in {32,48,...,384}, N in [1,min(L,64)], MW/MW_SS, "
+                       "complex/real (seed 4), via SSHT, Risbo, padded")
+    out["plain_untuned_ms"] = untuned_ms
+    out["autotuner"] = {"shapes": len(shapes), "seconds": tune_s,
+                        "pw_changed": sum(1 for k, c in shapes.items() if c["pw"] != defaults[k]),
+                        "inverse_order_1": sum(1 for c in shapes.values() if c["corder_inv"]),
+                        "forward_order_1": sum(1 for c in shapes.values() if c["corder_fwd"])}
+    del keep
+    small = [P.SO3Parameters(L=16 + (i * 7) % 48, N=1 + (i * 5) % min(16 + (i * 7) % 48, 16),
+                             n_order=P.SO3_N_ORDER_ZERO_FIRST) for i in range(256)]
+    for p in small:
+        A.prepare(p)
+    specs, keep = _pair_specs(A, torch, small, 5)
+    out["small_stream"] = _batch_legs(A, torch, specs, reps)
+    out["small_stream"]["workload"] = "256 complex MW fwd+inv pairs, L = 16 + 7i mod 48, N = 1 + 5i mod min(L,16)"
+    return out
 
 
 def serving_bench(A, rates=(100, 400, 800, 1600, 3200, 6400), count=120, capacity_count=400):
@@ -203,18 +258,20 @@ def serving_bench(A, rates=(100, 400, 800, 1600, 3200, 6400), count=120, capacit
     out = {"workload": "Poisson arrivals; 70%% fwd+inv pairs (L log-uniform [16,512], N in [1,min(L,32)]), "
                        "30%% s2toso3 + inverse (L log-uniform [64,1024], N in {4,8}); MW complex (seed 5); "
                        "%d requests per rate; tables and plans resident (built before timing); inputs "
-                       "drawn ahead in pinned host memory, copied to HBM inside the loop" % count,
+                       "drawn ahead (on the GPU) into pinned host memory, copied to HBM inside the loop; "
+                       "admission rounds run as persistent batches" % count,
            "rates": {}}
     sustained = 0.0
     for rate in rates:
         reqs = serve.request_mix(count, rate)
         srv.prepare(reqs)
-        fn = serve.pinned_inputs(reqs, serve.random_inputs(5))
+        fn = serve.device_drawn_pinned_inputs(reqs, 5)
         warm = [serve.Request(r.rid, r.kind, r.p, 0.0) for r in reqs[:8]]
         srv.run(warm, fn)
         srv.run(reqs, fn)
         st = serve.latency_stats(reqs)
         out["rates"][str(rate)] = st
+        progress("serving: %d requests/s offered" % rate)
         if st["throughput_rps"] >= 0.95 * rate:
             sustained = rate
     out["max_sustained_rps"] = sustained
@@ -223,10 +280,11 @@ def serving_bench(A, rates=(100, 400, 800, 1600, 3200, 6400), count=120, capacit
     for r in reqs:
         r.arrival = 0.0
     srv.prepare(reqs)
-    fn = serve.pinned_inputs(reqs, serve.random_inputs(5))
+    fn = serve.device_drawn_pinned_inputs(reqs, 5)
     srv.run([serve.Request(r.rid, r.kind, r.p, 0.0) for r in reqs[:8]], fn)
     srv.run(reqs, fn)
     cap = serve.latency_stats(reqs)
+    progress("serving: capacity")
     # plain: one request at a time, copy in, plain calls, wait for the result
     dev = torch.device("cuda:0")
     sid = torch.cuda.current_stream().cuda_stream
@@ -456,6 +514,14 @@ def cpu_baseline_mt(p, per_thread=3):
     }
 
 
+_T0 = time.time()
+
+
+def progress(msg):
+    """one line on stderr per finished leg (the JSON line goes to stdout)"""
+    print("bench: %s (%.1f s)" % (msg, time.time() - _T0), file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -584,19 +650,28 @@ def main():
         "path_roofline": {"t_min_ms": t_min * 1e3, "frac": t_min * 1e3 / ms_step,
                           "bytes_alg": bytes_alg, "contraction_flops": 2 * fl_contract},
     }
+    progress("plain fwd+inv at L=%d N=%d: %.3f ms per pair" % (L, N, ms_step))
     if not args.no_cpu:
         out["cpu_baseline_config0"] = config0_bench(A, P, torch)
+        progress("configuration 0")
         out["error_bounds"] = error_bounds(A, P, torch)
+        progress("error bounds")
     if not args.no_variants:
         out["variants"] = variants_bench(A, P, torch)
+        progress("variants")
         out["host_abi"] = host_abi_bench(A, p, flmn_h)
+        progress("host-pointer ABI")
     if not args.no_batch:
         out["batched"] = batched_bench(A, P, torch)
+        progress("batched")
     if not args.no_serve:
         out["serving"] = serving_bench(A)
+        progress("serving")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(p, args.cpu_slices)
+        progress("cpu baseline")
         out["cpu_baseline_mt"] = cpu_baseline_mt(p)
+        progress("cpu baseline, threads")
     print(json.dumps(out))
     if args.copy_calibration:
         a = torch.ones(1 << 27, dtype=torch.float64, device=dev)

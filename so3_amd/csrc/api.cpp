@@ -13,6 +13,8 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "batch.hpp"
 #include "conv.hpp"
@@ -46,6 +48,131 @@ bool is_device_ptr(const void *p) {
     return false;
   }
   return a.type == hipMemoryTypeDevice;
+}
+
+// ---- host <-> device copies of the host-pointer entry points
+//
+// A host array that is already pinned (hipHostMalloc) is copied by DMA
+// directly.  A pageable one goes through two pinned 64 MiB staging buffers
+// per device, chunk by chunk: host to device, the CPU copy of chunk i into
+// one buffer (several threads) overlaps the DMA of chunk i - 1 from the
+// other; device to host, the DMA of chunk i overlaps the CPU copy of chunk
+// i - 1 out of the other buffer.  The copies run on a stream of their own
+// and end synchronised, as the drop-in calls are synchronous.
+constexpr size_t kChunk = (size_t)64 << 20;
+
+struct Pinned {
+  char *buf[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  bool busy[2] = {false, false};
+  hipStream_t cs = nullptr;
+};
+std::mutex g_pin_mu;
+Pinned g_pin[64];
+
+bool is_pinned_host(const void *p) {
+  if (!p) return false;
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeHost;
+}
+
+void par_memcpy(void *dst, const void *src, size_t n) {
+  const unsigned hw = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+  const size_t parts = std::min<size_t>(hw, std::max<size_t>(1, n >> 22));  // >= 4 MiB per thread
+  if (parts <= 1) {
+    std::memcpy(dst, src, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  const size_t step = (n + parts - 1) / parts;
+  for (size_t k = 0; k < parts; ++k) {
+    const size_t o = k * step;
+    if (o >= n) break;
+    th.emplace_back([=] { std::memcpy((char *)dst + o, (const char *)src + o, std::min(step, n - o)); });
+  }
+  for (auto &t : th) t.join();
+}
+
+Pinned &pinned_for_device() {
+  int dev = 0;
+  SO3_HIP(hipGetDevice(&dev));
+  Pinned &P = g_pin[dev & 63];
+  if (!P.cs) {
+    SO3_HIP(hipStreamCreateWithFlags(&P.cs, hipStreamNonBlocking));
+    for (int b = 0; b < 2; ++b) {
+      SO3_HIP(hipHostMalloc((void **)&P.buf[b], kChunk, hipHostMallocDefault));
+      SO3_HIP(hipEventCreateWithFlags(&P.ev[b], hipEventDisableTiming));
+    }
+  }
+  return P;
+}
+
+void copy_h2d(void *dst, const void *src, size_t n) {
+  if (!n) return;
+  std::lock_guard<std::mutex> lk(g_pin_mu);
+  Pinned &P = pinned_for_device();
+  if (is_pinned_host(src)) {
+    SO3_HIP(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, P.cs));
+    SO3_HIP(hipStreamSynchronize(P.cs));
+    return;
+  }
+  for (size_t o = 0, i = 0; o < n; o += kChunk, ++i) {
+    const int b = (int)(i & 1);
+    const size_t m = std::min(kChunk, n - o);
+    if (P.busy[b]) SO3_HIP(hipEventSynchronize(P.ev[b]));  // its previous DMA is done
+    par_memcpy(P.buf[b], (const char *)src + o, m);
+    SO3_HIP(hipMemcpyAsync((char *)dst + o, P.buf[b], m, hipMemcpyHostToDevice, P.cs));
+    SO3_HIP(hipEventRecord(P.ev[b], P.cs));
+    P.busy[b] = true;
+  }
+  SO3_HIP(hipStreamSynchronize(P.cs));
+  P.busy[0] = P.busy[1] = false;
+}
+
+void copy_d2h(void *dst, const void *src, size_t n) {
+  if (!n) return;
+  std::lock_guard<std::mutex> lk(g_pin_mu);
+  Pinned &P = pinned_for_device();
+  if (is_pinned_host(dst)) {
+    SO3_HIP(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, P.cs));
+    SO3_HIP(hipStreamSynchronize(P.cs));
+    return;
+  }
+  const size_t nch = (n + kChunk - 1) / kChunk;
+  auto issue = [&](size_t i) {
+    const int b = (int)(i & 1);
+    const size_t o = i * kChunk, m = std::min(kChunk, n - o);
+    SO3_HIP(hipMemcpyAsync(P.buf[b], (const char *)src + o, m, hipMemcpyDeviceToHost, P.cs));
+    SO3_HIP(hipEventRecord(P.ev[b], P.cs));
+  };
+  issue(0);
+  for (size_t i = 0; i < nch; ++i) {
+    const int b = (int)(i & 1);
+    // the DMA of chunk i + 1 goes into the other buffer, whose copy-out
+    // (chunk i - 1) finished in the previous iteration
+    if (i + 1 < nch) issue(i + 1);
+    SO3_HIP(hipEventSynchronize(P.ev[b]));
+    const size_t o = i * kChunk;
+    par_memcpy((char *)dst + o, P.buf[b], std::min(kChunk, n - o));
+  }
+}
+
+void release_pinned() {
+  std::lock_guard<std::mutex> lk(g_pin_mu);
+  for (Pinned &P : g_pin) {
+    if (!P.cs) continue;
+    (void)hipStreamSynchronize(P.cs);
+    for (int b = 0; b < 2; ++b) {
+      (void)hipHostFree(P.buf[b]);
+      (void)hipEventDestroy(P.ev[b]);
+    }
+    (void)hipStreamDestroy(P.cs);
+    P = Pinned{};
+  }
 }
 
 // staging buffers for host-pointer calls (one pair per device)
@@ -94,9 +221,10 @@ void host_inverse(const char *fn, void *f, const void *flmn, const so3_parameter
     const size_t nf = f_bytes(p, m), nl = flmn_bytes(p);
     void *dl = grow(s.a, s.ca, nl);
     void *df = grow(s.b, s.cb, nf);
-    SO3_HIP(hipMemcpy(dl, flmn, nl, hipMemcpyHostToDevice));
+    copy_h2d(dl, flmn, nl);
     so3g::run_inverse(p, m, dl, df, nullptr);
-    SO3_HIP(hipMemcpy(f, df, nf, hipMemcpyDeviceToHost));
+    SO3_HIP(hipDeviceSynchronize());
+    copy_d2h(f, df, nf);
   } catch (const std::exception &e) {
     fatal(fn, e.what());
   }
@@ -117,11 +245,12 @@ void host_forward(const char *fn, void *flmn, const void *f, const so3_parameter
     const size_t nf = f_bytes(p, m), nl = flmn_bytes(p);
     void *dl = grow(s.a, s.ca, nl);
     void *df = grow(s.b, s.cb, nf);
-    SO3_HIP(hipMemcpy(df, f, nf, hipMemcpyHostToDevice));
+    copy_h2d(df, f, nf);
     // blocks the transform does not write keep the caller's values
-    if (!so3g::forward_writes_all(p, m)) SO3_HIP(hipMemcpy(dl, flmn, nl, hipMemcpyHostToDevice));
+    if (!so3g::forward_writes_all(p, m)) copy_h2d(dl, flmn, nl);
     so3g::run_forward(p, m, df, dl, nullptr);
-    SO3_HIP(hipMemcpy(flmn, dl, nl, hipMemcpyDeviceToHost));
+    SO3_HIP(hipDeviceSynchronize());
+    copy_d2h(flmn, dl, nl);
   } catch (const std::exception &e) {
     fatal(fn, e.what());
   }
@@ -138,10 +267,13 @@ struct DevBuf {
     }
     SO3_HIP(hipMalloc(&d, bytes ? bytes : 16));
     owned = true;
-    if (copy_in && bytes) SO3_HIP(hipMemcpy(d, host, bytes, hipMemcpyHostToDevice));
+    if (copy_in && bytes) copy_h2d(d, host, bytes);
   }
   void copy_out(void *host, size_t bytes) {
-    if (owned && bytes) SO3_HIP(hipMemcpy(host, d, bytes, hipMemcpyDeviceToHost));
+    if (owned && bytes) {
+      SO3_HIP(hipDeviceSynchronize());
+      copy_d2h(host, d, bytes);
+    }
   }
   ~DevBuf() {
     if (owned) (void)hipFree(d);
@@ -464,6 +596,7 @@ long long so3x_workspace_bytes(const so3_parameters_t *parameters) {
 int so3x_release_all(void) {
   try {
     so3g::release_batch();
+    release_pinned();
     so3g::release_all();
     so3g::release_conv_caches();
     return 0;

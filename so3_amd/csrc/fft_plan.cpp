@@ -176,6 +176,13 @@ FftDesc build(int n, bool reg1024) {
 
 }  // namespace
 
+void host_dft_ld(std::vector<std::complex<long double>> &x, int sign) {
+  const size_t n = x.size();
+  const bool pow2 = (n & (n - 1)) == 0, three = n % 3 == 0 && ((n / 3) & (n / 3 - 1)) == 0;
+  if (!pow2 && !three) throw HipError("host_dft_ld: length not 2^a or 3 * 2^a");
+  host_fft(x, sign);
+}
+
 int fft_capacity(const FftDesc &d) {
   int cap = kFftThreads * kFftElems * 4;
   unsigned long long code = d.radcode;

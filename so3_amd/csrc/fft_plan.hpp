@@ -1,6 +1,9 @@
 // fft_plan.hpp -- host construction of device FFT descriptors (cached per
 // length and device).
 #pragma once
+#include <complex>
+#include <vector>
+
 #include "fft_device.hpp"
 
 namespace so3g {
@@ -17,6 +20,10 @@ FftDesc fft_plan_lds(int n);
 // Largest S * len one workgroup can run through every pass of d (each
 // thread holds fft_nb(R) butterflies of radix R).
 int fft_capacity(const FftDesc &d);
+
+// In-place DFT in long double on the host (plan tables only), length 2^a or
+// 3 * 2^a; sign -1 forward (exp(-2 pi i jk/n)), +1 backward.
+void host_dft_ld(std::vector<std::complex<long double>> &x, int sign);
 
 // LDS stride (in double2) needed per sequence to run FftDesc d.
 inline int fft_ld(const FftDesc &d) { return d.len + (d.len % 32 == 0 ? 1 : 0); }

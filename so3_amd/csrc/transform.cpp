@@ -177,24 +177,20 @@ Plan build_plan(const so3_parameters_t *p, Method method) {
       else if (q % 2 == 0) v = cld(2.0L / (1.0L - (long double)q * q), 0);
       w[((q % Pw) + Pw) % Pw] = v;
     }
-    // wr[r] = sum_q w[q] exp(+2 pi i q r / Pw); w has 2L-1 non-zero even taps + 2
-    // odd ones (q = +-1); wr_even leaves the odd ones out
+    // wr[r] = sum_q w[q] exp(+2 pi i q r / Pw) (a backward DFT in long
+    // double, Pw = 2^a or 3 * 2^a); w has 2L-1 non-zero even taps + 2 odd
+    // ones (q = +-1); wr_even leaves the odd ones out
+    std::vector<cld> acc(w), ev(w);
+    if (L >= 2) {  // the odd taps q = +-1 exist
+      ev[1] = cld(0, 0);
+      ev[Pw - 1] = cld(0, 0);
+    }
+    host_dft_ld(acc, +1);
+    host_dft_ld(ev, +1);
     std::vector<double2> wr(Pw), wre(Pw);
     for (int r = 0; r < Pw; ++r) {
-      cld acc(0, 0), odd(0, 0);
-      for (int q = -2 * (L - 1); q <= 2 * (L - 1); ++q) {
-        const cld wq = w[((q % Pw) + Pw) % Pw];
-        if (wq == cld(0, 0)) continue;
-        long long k = ((long long)q * r) % Pw;
-        if (k < 0) k += Pw;
-        const long double a = 2.0L * kPiL * k / Pw;
-        const cld t = wq * cld(cosl(a), sinl(a));
-        acc += t;
-        if (q == 1 || q == -1) odd += t;
-      }
-      wr[r] = make_double2((double)acc.real(), (double)acc.imag());
-      const cld ev = acc - odd;
-      wre[r] = make_double2((double)ev.real(), (double)ev.imag());
+      wr[r] = make_double2((double)acc[r].real(), (double)acc[r].imag());
+      wre[r] = make_double2((double)ev[r].real(), (double)ev[r].imag());
     }
     P.wr = upload(wr);
     P.wr_even = upload(wre);
@@ -620,7 +616,7 @@ template <typename F>
 float best_ms(hipStream_t st, hipEvent_t a, hipEvent_t b, F &&fn) {
   fn();  // warm-up (sets launch attributes)
   float best = 1e30f;
-  for (int r = 0; r < 4; ++r) {
+  for (int r = 0; r < 2; ++r) {
     SO3_HIP(hipEventRecord(a, st));
     fn();
     SO3_HIP(hipEventRecord(b, st));

@@ -7,9 +7,13 @@ followed by the inverse transform) arrive over time.  The loop
 * admits queued requests in order while the HBM footprint of everything in
   flight (the request's input/output buffers plus one transform workspace)
   stays within a budget, at most `max_batch` per round;
-* launches each admission round as one batch (so3x_batch: items spread over
-  lanes of HIP streams) on the next of a few caller streams, so that a new
-  round starts while earlier ones still run, and takes a completion ticket;
+* launches each admission round as one batch on the next of a few caller
+  streams, so that a new round starts while earlier ones still run, and
+  takes a completion ticket.  In the persistent mode (the default) the
+  round's contractions, whatever the sizes of its requests, run in one
+  persistent launch per direction fed by device queues, the other stages
+  per request on lanes of HIP streams (so3x_batch_persistent); in the lanes
+  mode every request runs whole on one lane (so3x_batch);
 * polls the tickets and records each request's latency (completion minus
   arrival) and, on request, its outputs.
 
@@ -63,12 +67,14 @@ def request_mix(count, rate, seed=5):
 class Server:
     """Continuous-batching executor over the library's batch entry point."""
 
-    def __init__(self, hbm_budget=48 << 30, lanes=4, max_batch=32, streams=3, device="cuda:0"):
+    def __init__(self, hbm_budget=48 << 30, lanes=4, max_batch=32, streams=3, device="cuda:0",
+                 mode=A.BATCH_PERSISTENT):
         import torch
         self.torch = torch
         self.dev = torch.device(device)
         self.budget = int(hbm_budget)
         self.lanes = lanes
+        self.mode = mode
         self.max_batch = max_batch
         self.streams = [torch.cuda.Stream(device=self.dev) for _ in range(streams)]
         self._next = 0
@@ -78,7 +84,7 @@ class Server:
         p = r.p
         fb = A.f_size(p) * (8 if p.reality else 16)
         nb = A.flmn_size(p) * 16
-        ws = A.workspace_bytes(p)
+        ws = A.workspace_bytes(p)  # each request's own in the persistent mode
         if r.kind == A.ITEM_PAIR:
             return fb + 2 * nb + ws
         return fb + nb + 2 * 16 * p.L * p.L + ws
@@ -167,7 +173,10 @@ class Server:
                         specs.append(spec)
                         held.append((r, bufs, nbytes))
                 items = A.make_items(specs)
-                A.batch(items, lanes=self.lanes, stream=stream.cuda_stream)
+                if self.mode == A.BATCH_PERSISTENT:
+                    A.batch_persistent(items, lanes=self.lanes, stream=stream.cuda_stream)
+                else:
+                    A.batch(items, lanes=self.lanes, stream=stream.cuda_stream)
                 inflight.append((A.ticket(stream.cuda_stream), held))
             elif not pending or pending[0].arrival > now + poll_s:
                 time.sleep(poll_s)
@@ -185,6 +194,35 @@ def random_inputs(device_rng_seed=0):
         return {"flm": rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n),
                 "glm": rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)}
     return fn
+
+
+def device_drawn_pinned_inputs(requests, seed=0, device="cuda:0"):
+    """input_fn over `requests` whose U[-1, 1) inputs are drawn on the GPU
+    (torch's Philox generator seeded by (seed, request id)) ahead of time and
+    held in pinned host memory: the same role as pinned_inputs, without the
+    CPU time of drawing gigabytes with numpy."""
+    import torch
+    dev = torch.device(device)
+    g = torch.Generator(device=dev)
+    cache = {}
+
+    def draw(n):
+        re = torch.rand(n, generator=g, device=dev, dtype=torch.float64) * 2 - 1
+        im = torch.rand(n, generator=g, device=dev, dtype=torch.float64) * 2 - 1
+        out = torch.empty(n, dtype=torch.complex128, pin_memory=True)
+        out.copy_(torch.complex(re, im))
+        return out
+
+    for r in requests:
+        if r.rid in cache:
+            continue
+        g.manual_seed(seed * 1000003 + r.rid)
+        if r.kind == A.ITEM_PAIR:
+            cache[r.rid] = {"flmn": draw(A.flmn_size(r.p))}
+        else:
+            n = r.p.L * r.p.L
+            cache[r.rid] = {"flm": draw(n), "glm": draw(n)}
+    return lambda r: cache[r.rid]
 
 
 def pinned_inputs(requests, input_fn):

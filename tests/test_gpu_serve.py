@@ -35,11 +35,12 @@ def _alone(r, fn):
     return {"f": A.inverse(h, r.p), "flmn_out": h}
 
 
+@pytest.mark.parametrize("mode", [A.BATCH_PERSISTENT, A.BATCH_LANES])
 @pytest.mark.parametrize("budget,max_batch", [(8 << 30, 8), (1, 3)])
-def test_served_equals_alone(budget, max_batch):
+def test_served_equals_alone(budget, max_batch, mode):
     reqs = _mix()
     fn = serve.random_inputs(1)
-    srv = serve.Server(hbm_budget=budget, lanes=3, max_batch=max_batch, streams=2)
+    srv = serve.Server(hbm_budget=budget, lanes=3, max_batch=max_batch, streams=2, mode=mode)
     srv.prepare(reqs)
     srv.run(reqs, fn, keep_outputs=True)
     for r in reqs:
