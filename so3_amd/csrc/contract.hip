@@ -308,30 +308,38 @@ __device__ __forceinline__ void contract_inv_block(unsigned blk, const Geo &g, c
   }
   if (!yv) return;
 
-  // store: lane holds m' = mp0 + (lane & 15), half q >> 1, registers j
+  // store: lane holds m' = mpb + 16 t + (lane & 15), pair half q >> 1,
+  // registers 2 e, 2 e + 1 of its combination.  The X1 row, the power of i
+  // and the sign pattern depend on (x, combination) only, so they are set
+  // once per pair and the tiles follow (per-store slot arithmetic with
+  // divisions was a large part of the kernel's instructions).
   const int q = lane >> 4, hh = q >> 1;
+  const int mp0 = mpb + (lane & 15);
 #pragma unroll
-  for (int t = 0; t < kNT; ++t) {
-    const int mp = mpb + 16 * t + (lane & 15);
-    if (mp >= g.L) continue;
+  for (int a = 0; a < kXG; ++a) {
+    if (!(hh == 0 ? pv[a] : pp[a])) continue;
+    const int x = x0 + a;
+    const int mi = hh == 0 ? x : y, nj = hh == 0 ? y : x;
 #pragma unroll
-    for (int a = 0; a < kXG; ++a) {
-      if (!(hh == 0 ? pv[a] : pp[a])) continue;
-      const int x = x0 + a;
-      const int mi = hh == 0 ? x : y, nj = hh == 0 ? y : x;
+    for (int e = 0; e < (C == 4 ? 2 : 1); ++e) {
+      const int c = C == 4 ? 2 * (q & 1) + e : (q & 1);
+      const int sm = c & 1, sn = c >> 1;
+      if (sm && mi == 0) continue;
+      if (sn && nj == 0) continue;
+      const int m = sm ? -mi : mi, n = sn ? -nj : nj;
+      // |n| < nG and |m| < nA: a negative index wraps by one period
+      const int nslot = (g.real || n >= 0) ? n : n + g.nG;
+      const int mslot = m >= 0 ? m : m + g.nA;
+      double2 *row = X1 + ((int64_t)nslot * g.nA + mslot) * g.L;
+      const int ip = (n - m) & 3;
+      const bool alt = sm != sn;  // (-1)^{m'} of the mixed-sign combinations
 #pragma unroll
-      for (int e = 0; e < (C == 4 ? 2 : 1); ++e) {
-        const int c = C == 4 ? 2 * (q & 1) + e : (q & 1);
-        const int sm = c & 1, sn = c >> 1;
-        if (sm && mi == 0) continue;
-        if (sn && nj == 0) continue;
-        const int m = sm ? -mi : mi, n = sn ? -nj : nj;
+      for (int t = 0; t < kNT; ++t) {
+        const int mp = mp0 + 16 * t;
+        if (mp >= g.L) continue;
         double2 v = make_double2(acc[a][t][2 * e], acc[a][t][2 * e + 1]);
-        if ((sm != sn) && (mp & 1)) v = make_double2(-v.x, -v.y);
-        v = cmul_ipow(v, n - m);
-        const int nslot = g.real ? n : (n + g.nG) % g.nG;
-        const int mslot = (m + g.nA) % g.nA;
-        X1[((int64_t)nslot * g.nA + mslot) * g.L + mp] = v;
+        if (alt && (mp & 1)) v = make_double2(-v.x, -v.y);
+        row[mp] = cmul_ipow(v, ip);
       }
     }
   }
@@ -477,33 +485,39 @@ __device__ __forceinline__ void contract_fwd_block(unsigned blk, const Geo &g, c
   }
   if (!yv) return;
 
-  // store: lane holds l = l0 + (lane & 15), half q >> 1, registers j
+  // store: lane holds l = lo + 16 t + (lane & 15), pair half q >> 1,
+  // registers 2 e, 2 e + 1 of its combination.  The block base, the power of
+  // i and the sign pattern depend on (x, combination) only, so they are set
+  // (and the block base loaded) once per pair and the tiles follow.
   const int q = lane >> 4, hh = q >> 1;
+  const int l0 = lo + (lane & 15);
 #pragma unroll
-  for (int t = 0; t < kNT; ++t) {
-    const int l = lo + 16 * t + (lane & 15);
-    if (l >= g.L || l < g.L0) continue;
-    const int64_t ll = (int64_t)l * l + l;
+  for (int a = 0; a < kXG; ++a) {
+    if (!(hh == 0 ? pv[a] : pp[a])) continue;
+    const int x = x0 + a;
+    const int mi = hh == 0 ? x : y, nj = hh == 0 ? y : x;
 #pragma unroll
-    for (int a = 0; a < kXG; ++a) {
-      if (!(hh == 0 ? pv[a] : pp[a])) continue;
-      const int x = x0 + a;
-      const int mi = hh == 0 ? x : y, nj = hh == 0 ? y : x;
-      if (mi > l || nj > l) continue;
-      if (g.nmodeL && nj != l) continue;
+    for (int e = 0; e < (C == 4 ? 2 : 1); ++e) {
+      const int c = C == 4 ? 2 * (q & 1) + e : (q & 1);
+      const int sm = c & 1, sn = c >> 1;
+      if (sm && mi == 0) continue;
+      if (sn && nj == 0) continue;
+      const int m = sm ? -mi : mi, n = sn ? -nj : nj;
+      const int64_t bb = nbase[n + g.N - 1];
+      if (bb == kNoBlock) continue;
+      double2 *blk = flmn + bb + m;
+      const int ip = (m - n) & 3;
+      const bool alt = sm != sn;  // (-1)^l of the mixed-sign combinations
+      const int lmin = max(max(g.L0, max(mi, nj)), g.nmodeL ? nj : 0);
+      const int lmax = g.nmodeL ? nj : g.L - 1;
 #pragma unroll
-      for (int e = 0; e < (C == 4 ? 2 : 1); ++e) {
-        const int c = C == 4 ? 2 * (q & 1) + e : (q & 1);
-        const int sm = c & 1, sn = c >> 1;
-        if (sm && mi == 0) continue;
-        if (sn && nj == 0) continue;
-        const int m = sm ? -mi : mi, n = sn ? -nj : nj;
-        const int64_t bb = nbase[n + g.N - 1];
-        if (bb == kNoBlock) continue;
+      for (int t = 0; t < kNT; ++t) {
+        const int l = l0 + 16 * t;
+        if (l < lmin || l > lmax) continue;
         double2 v = make_double2(acc[a][t][2 * e], acc[a][t][2 * e + 1]);
-        if ((sm != sn) && (l & 1)) v = make_double2(-v.x, -v.y);
+        if (alt && (l & 1)) v = make_double2(-v.x, -v.y);
         if (g.clmode == 2) v = cscale(v, (2.0 * l + 1.0) * kInv8Pi2);
-        flmn[bb + ll + m] = cmul_ipow(v, m - n);
+        blk[(int64_t)l * l + l] = cmul_ipow(v, ip);
       }
     }
   }

@@ -1188,16 +1188,16 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
     r32_fence();
     const bool okA = mA >= -(L - 1) && mA <= L - 1, okB = mB >= -(L - 1) && mB <= L - 1;
     const int an = n < 0 ? -n : n, sn = n < 0;
-    // G' row of each member when it has a single combination (sign m, sign n)
-    bool one[2], flip[2];
+    const int cn0 = an == 0 ? 0 : sn, cn1 = an == 0 ? g.NS - 1 : sn;
+    int cm0[2], cm1[2];
     double2 *row[2];
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
       const int m = e ? mB : mA;
       const int am = m < 0 ? -m : m, sm = m < 0;
-      one[e] = am != 0 && an != 0;
-      flip[e] = sm != sn;
-      row[e] = Gp + (((int64_t)an * L + am) * C2 + sm + 2 * sn) * L;
+      cm0[e] = am == 0 ? 0 : sm;
+      cm1[e] = am == 0 ? 1 : sm;
+      row[e] = Gp + ((int64_t)an * L + am) * C2 * L;
     }
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -1229,18 +1229,12 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
       for (int e = 0; e < 2; ++e) {
         if (!(e ? okB : okA)) continue;
         const double2 G0 = e ? GB : GA;
-        if (one[e]) {
-          row[e][mp] = (flip[e] && (mp & 1)) ? make_double2(-G0.x, -G0.y) : G0;
-          continue;
-        }
-        const int m = e ? mB : mA;
-        const int am = m < 0 ? -m : m, sm = m < 0;
-        double2 *dst = Gp + ((int64_t)an * L + am) * C2 * L;
-        for (int cm = 0; cm < 2; ++cm) {
-          if (cm != sm && am != 0) continue;
-          for (int cn = 0; cn < g.NS; ++cn) {
-            if (cn != sn && an != 0) continue;
-            dst[(int64_t)(cm + 2 * cn) * L + mp] = ((cm != cn) && (mp & 1)) ? make_double2(-G0.x, -G0.y) : G0;
+#pragma nounroll
+        for (int cn = cn0; cn <= cn1; ++cn) {
+#pragma nounroll
+          for (int cm = cm0[e]; cm <= cm1[e]; ++cm) {
+            const double sg = ((cm != cn) && (mp & 1)) ? -1.0 : 1.0;
+            row[e][(int64_t)(cm + 2 * cn) * L + mp] = make_double2(sg * G0.x, sg * G0.y);
           }
         }
       }
