@@ -94,9 +94,11 @@ double2 *upload(const std::vector<cld> &v) {
   return d;
 }
 
-// reg1024: lengths in [257, 512] take Bluestein with M = 1024 (for the
-// register-resident FFTs); otherwise they follow the general rule
-FftDesc build(int n, bool reg1024) {
+// mode 0: the general rule (the workgroup engine); 1: lengths in [257, 512]
+// take Bluestein with M = 1024 (the register-resident FFTs); 2: as 1, and
+// lengths in [513, 1024] take Bluestein with M = 2048
+FftDesc build(int n, int mode) {
+  const bool reg1024 = mode >= 1;
   FftDesc d{};
   d.n = n;
   std::vector<int> rad;
@@ -112,7 +114,10 @@ FftDesc build(int n, bool reg1024) {
   // runs on the register-resident FFTs (fft_reg.hpp, fft_reg32.hpp) faster
   // than the workgroup engine runs even a smooth n directly
   const bool r1024 = reg1024 && n >= 257 && n <= 512;
-  bool direct = factor(n, rad) && !r1024;
+  // mode 2: every n in [513, 1024] takes Bluestein with M = 2048, which the
+  // one-sequence 2048-point register FFT (fft_reg2k.hpp) runs
+  const bool r2048 = mode == 2 && n >= 513 && n <= 1024;
+  bool direct = factor(n, rad) && !r1024 && !r2048;
   if (direct) {
     int cap = kFftThreads * kFftElems * 4;
     for (int R : rad) cap = std::min(cap, kFftThreads * fft_nb(R) * R);
@@ -126,7 +131,7 @@ FftDesc build(int n, bool reg1024) {
     while (M < 2 * n - 1) M *= 2;
     // 3 * 2^k when that suffices and one workgroup holds it (fewer points
     // than the next power of two); n in [257, 512] keeps M = 1024
-    if (!r1024 && M >= 4 && 3 * (M / 4) >= 2 * n - 1 && 3 * (M / 4) <= 1536) M = 3 * (M / 4);
+    if (!r1024 && !r2048 && M >= 4 && 3 * (M / 4) >= 2 * n - 1 && 3 * (M / 4) <= 1536) M = 3 * (M / 4);
     d.blue = 1;
     d.len = M;
     factor(M, rad);
@@ -193,22 +198,27 @@ int fft_capacity(const FftDesc &d) {
   return cap;
 }
 
-static FftDesc plan_cached(int n, bool reg1024) {
+static FftDesc plan_cached(int n, int mode) {
   static std::mutex mu;
-  static std::map<std::tuple<int, int, bool>, FftDesc> cache;
+  static std::map<std::tuple<int, int, int>, FftDesc> cache;
   int dev = 0;
   SO3_HIP(hipGetDevice(&dev));
   std::lock_guard<std::mutex> g(mu);
-  auto key = std::make_tuple(dev, n, reg1024);
+  auto key = std::make_tuple(dev, n, mode);
   auto it = cache.find(key);
   if (it != cache.end()) return it->second;
-  FftDesc d = build(n, reg1024);
+  FftDesc d = build(n, mode);
   cache[key] = d;
   return d;
 }
 
-FftDesc fft_plan(int n) { return plan_cached(n, true); }
+FftDesc fft_plan(int n) { return plan_cached(n, 1); }
 
-FftDesc fft_plan_lds(int n) { return n >= 257 && n <= 512 ? plan_cached(n, false) : plan_cached(n, true); }
+FftDesc fft_plan_lds(int n) { return n >= 257 && n <= 512 ? plan_cached(n, 0) : plan_cached(n, 1); }
+
+FftDesc fft_plan_r2k(int n) {
+  if (n < 513 || n > 1024) throw HipError("fft_plan_r2k: length outside [513, 1024]");
+  return plan_cached(n, 2);
+}
 
 }  // namespace so3g

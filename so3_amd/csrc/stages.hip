@@ -22,6 +22,7 @@
 
 #include "fft_plan.hpp"
 #include "fft_reg.hpp"
+#include "fft_reg2k.hpp"
 #include "fft_reg32.hpp"
 #include "kernels.hpp"
 
@@ -228,6 +229,51 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
     for (int q = 0; q < 16; ++q) {
       const int k = l + 32 * q;
       if (q < 8 || k < n) row[k] = cscale(v[q], scale);  // n > 256
+    }
+  }
+}
+
+// Rows of a Bluestein length n in [513, 1024] on the 2048-point register FFT
+// (fft_reg2k.hpp): one row per wavefront, each access instruction moving 64
+// consecutive entries (even samples in the lower half-wavefront, odd ones in
+// the upper); row maps as in k_rows_reg.  One workgroup per CU shares the
+// tables; two wavefronts per SIMD.
+template <int DIR>
+__global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_rows_r2k(
+    FftDesc fd, const double2 *src, double2 *dst, int64_t nrows, int n, int64_t rs, double scale, Geo g,
+    int map) {
+  extern __shared__ double2 lds[];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  double2 *tab = lds;
+  double *T = (double *)(lds + kR2kTab) + wv * kR32T;
+  r2k_stage_blue(tab, fd);
+  __syncthreads();
+  const double2 *bh = DIR < 0 ? fd.bhat_f : fd.bhat_b;
+  const int64_t groups = (nrows + kRows32Waves - 1) / kRows32Waves;
+  auto row_at = [&](int64_t r, bool IN) -> int64_t {
+    if (map == kRowsFlat) return r * rs;
+    const int k = (int)(r / g.nB), b = (int)(r % g.nB);
+    const bool x2 = (map == kRowsX2F) == IN;
+    return x2 ? x2_row(g, k, b) * g.ldA : r * g.nA;
+  };
+  for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
+    const int64_t r = t * kRows32Waves + wv;
+    if (r >= nrows) break;  // the whole wavefront, last group
+    const int lane = loop_lane(), l = lane & 31, h = lane >> 5;
+    const double2 *in = src + row_at(r, true);
+    double2 v[32];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = in[min(2 * (l + 32 * q) + h, n - 1)];  // past n: cleared by the chirp
+    blue2k<DIR>(v, n, tab, bh, T, lane);
+    // store addresses from a fresh opaque lane index: the compiler would
+    // otherwise keep the 16 load addresses live across the transform
+    const int ls = loop_lane();
+    double2 *row = dst + row_at(r, false);
+    const double sc = 0.5 * scale;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int j = 2 * ((ls & 31) + 32 * q) + (ls >> 5);
+      if (j < n) row[j] = cscale(v[q], sc);
     }
   }
 }
@@ -1022,6 +1068,108 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
   }
 }
 
+// Inverse beta, paired, on the 2048-point register FFT (Pb in [513, 1024],
+// L = 257 ... 512): wavefront w of group t takes the pair (mA, mA + 1),
+// mA = m0 + 2 w, as one sequence A + i B, so a group covers 2 W consecutive
+// m of one n slot.  Samples k of the sequence sit in layout E (k = 2 (l + 32
+// q) + h); X1 rows are read straight (64 consecutive m' per instruction),
+// Z(b') comes through the wavefront's transpose buffer, and each lane stores
+// its A and B entries straight into X2.  ODD: Pb odd (MW).
+template <int ODD>
+__global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_inv_beta_r2k(
+    Geo g, FftDesc fd, const double2 *__restrict__ X1, double2 *__restrict__ X2,
+    const int64_t *__restrict__ nbase, const double2 *__restrict__ phase) {
+  constexpr int C = 2 * kRows32Waves;  // m per group
+  constexpr int kQB = 9;               // b = 2 (l + 32 j) + h < nB <= 513
+  extern __shared__ double2 lds[];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  double2 *tab = lds;
+  double *T = (double *)(lds + kR2kTab) + wv * kR32T;
+  r2k_stage_blue(tab, fd);
+  __syncthreads();
+  const int L = g.L, Pb = g.Pb;  // 257 <= L <= 512
+  const int mlo = -(L - 1) - g.ss;
+  const int chunks = (g.nA + C - 1) / C;
+  const int64_t groups = (int64_t)g.nG * chunks;
+  for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
+    // one barrier per group keeps the wavefronts that share 128-byte lines
+    // of X2 in step (their partial-line stores meet in L2)
+    __syncthreads();
+    const int lane = loop_lane(), l = lane & 31, h = lane >> 5;
+    const int nslot = (int)(t / chunks);
+    const int n = n_of_slot(g, nslot);
+    const int mA = mlo + (int)(t % chunks) * C + 2 * wv, mB = mA + 1;
+    const bool act = nbase[n + g.N - 1] != kNoBlock;
+    // outputs: columns past m = L - 1 are not stored, the MW_SS Nyquist
+    // column m = -L holds zeros
+    const bool stA = mA <= L - 1, stB = mB <= L - 1;
+    const bool dA = act && mA >= -(L - 1), dB = act && mB >= -(L - 1);
+    const int cA = m_slot_of(g, stA ? mA : 0), cB = m_slot_of(g, stB ? mB : 0);
+    if (!act) {  // block n not in the transform: zeros
+#pragma unroll
+      for (int j = 0; j < kQB; ++j) {
+        const int b = 2 * (l + 32 * j) + h;
+        if (b >= g.nB) continue;
+        double2 *row = X2 + x2_row(g, nslot, b) * g.ldA;
+        if (stA) row[cA] = make_double2(0.0, 0.0);
+        if (stB) row[cB] = make_double2(0.0, 0.0);
+      }
+      continue;
+    }
+    // A member outside [-(L-1), L-1] reads an in-range row of the same
+    // parity: the pair keeps its symmetries, so the separation of the other
+    // member stays exact, and the member's own outputs are not stored.
+    auto fixm = [&](int m) { return m < -(L - 1) ? m + 2 : m > L - 1 ? (L - 1) - ((m - (L - 1)) & 1) : m; };
+    const double2 *rA = X1 + ((int64_t)nslot * g.nA + m_slot_of(g, fixm(mA))) * L;
+    const double2 *rB = X1 + ((int64_t)nslot * g.nA + m_slot_of(g, fixm(mB))) * L;
+    const bool negA = ((mA + n) & 1) != 0;  // s_A = -1 (then s_B = +1); wave-uniform
+    double2 v[32];
+    // entry k holds m' = k (k <= L - 1) or k - Pb; k >= Pb only needs to be
+    // finite (the chirp clears it)
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int k = min(2 * (l + 32 * q) + h, Pb - 1);
+      const int mp = k <= L - 1 ? k : k - Pb;
+      const int am = mp < 0 ? -mp : mp;
+      double2 a = rA[am], b = rB[am];
+      if (mp < 0) {  // X1(-m') = s X1(m')
+        if (negA) a = make_double2(-a.x, -a.y);
+        else b = make_double2(-b.x, -b.y);
+      }
+      if (k == 0) {  // the antisymmetric member vanishes at m' = 0
+        if (negA) a = make_double2(0.0, 0.0);
+        else b = make_double2(0.0, 0.0);
+      }
+      if (!ODD && k == L) a = b = make_double2(0.0, 0.0);  // MW_SS: m' = -L
+      v[q] = cmul(make_double2(a.x - b.y, a.y + b.x), phase[k]);
+    }
+    blue2k<+1>(v, Pb, tab, fd.bhat_b, T, lane);  // twice the DFT
+    // Z(b') for b < nB through the buffer (entry q * 64 + lane holds
+    // register q of the lane): b' = Pb - 1 - b (MW) or Pb - b (MW_SS, b > 0)
+    double2 *T2 = (double2 *)T;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) T2[q * 64 + lane] = v[q];
+    __builtin_amdgcn_wave_barrier();
+    r32_fence();
+    const double sA = negA ? -1.0 : 1.0;
+#pragma unroll
+    for (int j = 0; j < kQB; ++j) {
+      const int b = 2 * (l + 32 * j) + h;
+      const int bp = ODD ? Pb - 1 - b : (b == 0 ? 0 : Pb - b);
+      const double2 zp = T2[max(bp, 0) >> 6 << 6 | (bp & 1) << 5 | ((max(bp, 0) >> 1) & 31)];
+      if (b >= g.nB) continue;
+      // A = (Z + s_A Z') / 2, B = (Z - s_A Z') / (2 i), with the factor 2 of blue2k
+      const double2 su = make_double2(v[j].x + sA * zp.x, v[j].y + sA * zp.y);
+      const double2 df = make_double2(v[j].x - sA * zp.x, v[j].y - sA * zp.y);
+      double2 *row = X2 + x2_row(g, nslot, b) * g.ldA;
+      if (stA) row[cA] = dA ? make_double2(0.25 * su.x, 0.25 * su.y) : make_double2(0.0, 0.0);
+      if (stB) row[cB] = dB ? make_double2(0.25 * df.y, -0.25 * df.x) : make_double2(0.0, 0.0);
+    }
+    __builtin_amdgcn_wave_barrier();
+    r32_fence();  // the buffer is read before the next group's transforms
+  }
+}
+
 // forward beta, paired (not the adjoint): the Bluestein DFT and the
 // convolution with the even weight taps run once per pair.  The two odd
 // taps w(+-1) = -+ i pi/2 cancel in the fold G'(m') = G(m') + s G(-m') for
@@ -1635,6 +1783,19 @@ void launch_inv_beta(const Geo &g, const double2 *X1, double2 *X2, NBase nb, con
     SO3_LAUNCH_CHECK();
     return;
   }
+  if (g.Pb >= 513 && g.Pb <= 1024 && g.nB <= 513) {
+    const FftDesc f2 = fft_plan_r2k(g.Pb);
+    constexpr int C = 2 * kRows32Waves;
+    const size_t sh = sizeof(double2) * kR2kTab + sizeof(double) * kRows32Waves * kR32T;
+    const int chunks = (g.nA + C - 1) / C;
+    auto go = [&](auto kern) {
+      const unsigned grid = reg_grid(kern, 64 * kRows32Waves, sh, (int64_t)g.nG * chunks);
+      kern<<<grid, 64 * kRows32Waves, sh, st>>>(g, f2, X1, X2, nb.base, phase);
+    };
+    (g.Pb & 1) ? go(k_inv_beta_r2k<1>) : go(k_inv_beta_r2k<0>);
+    SO3_LAUNCH_CHECK();
+    return;
+  }
   const int ld = fft_ld(fd);
   const int tb = tw_count(fd) * (int)sizeof(double2);
   const int K = choose_k(ld, fft_capacity(fd), fd.len, tb, 8);
@@ -1701,9 +1862,26 @@ void launch_alpha_fx2(const Geo &g, const double2 *f, double2 *X2, int dir, doub
   launch_rows_reg(fft_plan(g.nA), f, X2, (int64_t)g.nG * g.nB, g.nA, 0, dir, scale, g, kRowsFX2, st);
 }
 
+// rows of length n in [513, 1024] on the 2048-point register FFT
+static void launch_rows_r2k(const double2 *src, double2 *dst, int64_t nrows, int n, int64_t rs, int dir,
+                            double scale, const Geo &g, int map, hipStream_t st) {
+  const FftDesc fd = fft_plan_r2k(n);
+  const size_t sh = sizeof(double2) * kR2kTab + sizeof(double) * kRows32Waves * kR32T;
+  auto go = [&](auto kern) {
+    const unsigned grid = reg_grid(kern, 64 * kRows32Waves, sh, (nrows + kRows32Waves - 1) / kRows32Waves);
+    kern<<<grid, 64 * kRows32Waves, sh, st>>>(fd, src, dst, nrows, n, rs, scale, g, map);
+  };
+  dir > 0 ? go(k_rows_r2k<+1>) : go(k_rows_r2k<-1>);
+  SO3_LAUNCH_CHECK();
+}
+
 void launch_rows(double2 *X, int64_t nrows, int n, int ldr, int dir, double scale, hipStream_t st) {
-  const FftDesc fd = fft_plan(n);
   if (nrows <= 0) return;
+  if (n >= 513 && n <= 1024) {
+    launch_rows_r2k(X, X, nrows, n, ldr, dir, scale, Geo{}, kRowsFlat, st);
+    return;
+  }
+  const FftDesc fd = fft_plan(n);
   if (fd.len == 1024) {
     launch_rows_reg(fd, X, X, nrows, n, ldr, dir, scale, Geo{}, kRowsFlat, st);
     return;

@@ -72,3 +72,31 @@ def test_dft_register_lengths(n, howmany, direction):
     ref = np.fft.fft(x, axis=1) if direction < 0 else np.fft.ifft(x, axis=1) * n
     err = np.abs(y - ref).max() / np.abs(ref).max()
     assert err < 5e-15 * np.log2(n), err
+
+
+# Lengths with a Bluestein inner length of 2048 (513 <= n <= 1024) run on the
+# 2048-point register FFT, one row per wavefront (eight per workgroup): odd
+# and even n around the chirp table's halves (j = 512 reads c_{n-512}), and
+# row counts below, at and past one workgroup.
+@pytest.mark.parametrize("n", [513, 514, 600, 767, 768, 769, 1000, 1023, 1024])
+@pytest.mark.parametrize("howmany", [1, 8, 9])
+@pytest.mark.parametrize("direction", [-1, 1])
+def test_dft_register_2048(n, howmany, direction):
+    lib()
+    h = _hip()
+    rng = np.random.default_rng(7000 * n + howmany)
+    x = rng.standard_normal((howmany, n)) + 1j * rng.standard_normal((howmany, n))
+    x = np.ascontiguousarray(x)
+    d = ctypes.c_void_p()
+    assert h.hipMalloc(ctypes.byref(d), x.nbytes) == 0
+    try:
+        assert h.hipMemcpy(d, x.ctypes.data, x.nbytes, 1) == 0
+        A.dft_device(d.value, n, howmany, direction)
+        assert h.hipDeviceSynchronize() == 0
+        y = np.empty_like(x)
+        assert h.hipMemcpy(y.ctypes.data, d, x.nbytes, 2) == 0
+    finally:
+        h.hipFree(d)
+    ref = np.fft.fft(x, axis=1) if direction < 0 else np.fft.ifft(x, axis=1) * n
+    err = np.abs(y - ref).max() / np.abs(ref).max()
+    assert err < 5e-15 * np.log2(n), err

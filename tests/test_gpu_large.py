@@ -57,8 +57,9 @@ def _fit(coefs, L, N, real):
 
 # L = N = 256 and sizes with every Bluestein length between 257 and 512 on the
 # register FFT: alpha and beta of 2L - 1 (MW) or 2L (MW_SS), gamma of 2N - 1;
-# L = 300 runs alpha and beta on the workgroup engine and gamma between X2
-# and f (complex: the gamma tile kernel; real: two columns per DFT)
+# L = 300 runs alpha and the inverse beta on the 2048-point register FFT and
+# gamma between X2 and f (complex: the gamma tile kernel; real: two columns
+# per DFT)
 @pytest.mark.parametrize("L,N,real,ss", [(256, 256, 0, 0), (256, 256, 1, 0), (256, 256, 0, 1), (256, 256, 1, 1),
                                          (150, 140, 0, 0), (170, 131, 0, 1), (129, 129, 0, 0),
                                          (211, 180, 1, 0), (300, 140, 0, 0), (300, 150, 1, 0)])
@@ -109,7 +110,11 @@ def test_sparse_inverse_and_round_trip(oracle, L, N, real, ss):
                                          # 1024-point convolution on the register kernel
                                          (150, 2, 0, 0), (129, 3, 1, 1),
                                          # workgroup engine, 512-point direct convolution
-                                         (120, 3, 0, 0), (100, 2, 1, 1)])
+                                         (120, 3, 0, 0), (100, 2, 1, 1),
+                                         # alpha and the inverse beta on the 2048-point
+                                         # register FFT (Pb = 513 ... 1024)
+                                         (257, 3, 0, 0), (300, 2, 1, 0), (384, 2, 0, 1),
+                                         (511, 1, 1, 1), (512, 2, 0, 0)])
 def test_forward_arbitrary_samples_paired_beta(oracle, L, N, real, ss):
     p = P.SO3Parameters(L=L, N=N, reality=real, sampling_scheme=ss, n_order=P.SO3_N_ORDER_ZERO_FIRST)
     rng = np.random.default_rng(L + 10 * real + 100 * ss)
