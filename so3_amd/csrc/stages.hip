@@ -1392,6 +1392,187 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
   }
 }
 
+// Forward beta, paired, on the 2048-point register FFT (Pb in [513, 1024],
+// L = 257 ... 512, convolution length Pw = 2048): the algorithm of
+// k_fwd_beta_r32 with one pair (mA, mA + 1), mA = m0 + 2 w, per wavefront,
+// samples k in layout E (k = 2 (l + 32 q) + h).  After the Bluestein DFT the
+// remap to the convolution index r keeps m'' = r >= 0 in the lane's own
+// registers 0..7 and gathers m'' = r - 2048 < 0 (registers 24..31) through
+// the transpose buffer; the convolution is a forward 2048-point FFT, the
+// product with the real weight spectrum (global memory) and the inverse;
+// the fold takes G(m') from registers 0..7 and G(-m') = G(2048 - m') from
+// registers 24..31 through the buffer.  ODD: Pb odd (MW).
+template <int ODD>
+__global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_fwd_beta_r2k(
+    Geo g, FftDesc fb, const double2 *__restrict__ X2, double2 *__restrict__ Gp,
+    const int64_t *__restrict__ nbase, const double2 *__restrict__ phase, const double2 *__restrict__ wre,
+    const double2 *__restrict__ pole) {
+  constexpr int C = 2 * kRows32Waves;  // m per group
+  constexpr double kPi = 3.141592653589793238462643383279502884;
+  extern __shared__ double2 lds[];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  double2 *tab = lds;
+  double *T = (double *)(lds + kR2kTab) + wv * kR32T;
+  double2 *T2 = (double2 *)T;
+  r2k_stage_blue(tab, fb);
+  __syncthreads();  // tables staged
+  const int L = g.L, Pb = g.Pb;  // 257 <= L <= 512
+  const int mlo = -(L - 1) - g.ss;
+  const int chunks = (g.nA + C - 1) / C;
+  const int64_t groups = (int64_t)g.nG * chunks;
+  const int C2 = 2 * g.NS;
+  auto fixm = [&](int m) { return m < -(L - 1) ? m + 2 : m > L - 1 ? (L - 1) - ((m - (L - 1)) & 1) : m; };
+  for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
+    // one barrier per group keeps the wavefronts that share 128-byte lines
+    // of X2 in step
+    __syncthreads();
+    const int nslot = (int)(t / chunks);
+    const int n = n_of_slot(g, nslot);
+    if (nbase[n + g.N - 1] == kNoBlock) continue;  // block not transformed (the whole workgroup)
+    const int mA = mlo + (int)(t % chunks) * C + 2 * wv, mB = mA + 1;
+    const bool negA = ((mA + n) & 1) != 0;  // s_A = -1 (then s_B = +1); wave-uniform
+    const double sA = negA ? -1.0 : 1.0;
+    double2 v[32];
+    {
+      // extended samples: MW F(2L-2-b) for b >= L, MW_SS F(2L-b) for b > L;
+      // entries k >= Pb only need to be finite (the chirp clears them); a
+      // member outside [-(L-1), L-1] reads an in-range column (the extension
+      // imposes its symmetry; its outputs are not stored)
+      const int lane = loop_lane(), l = lane & 31, h = lane >> 5;
+      const double2 *colA = X2 + m_slot_of(g, fixm(mA)), *colB = X2 + m_slot_of(g, fixm(mB));
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int k = min(2 * (l + 32 * q) + h, Pb - 1);
+        const bool direct = ODD ? k < L : k <= L;
+        const int kr = direct ? k : (ODD ? 2 * L - 2 - k : 2 * L - k);
+        const int64_t off = x2_row(g, nslot, kr) * g.ldA;
+        double2 a = colA[off], b = colB[off];
+        if (!direct) {
+          if (negA) a = make_double2(-a.x, -a.y);
+          else b = make_double2(-b.x, -b.y);
+        }
+        const bool pk = ODD ? k == L - 1 : (k == 0 || k == L);
+        if (pk) {  // the antisymmetric member's pole samples go through the pole table
+          if (negA) a = make_double2(0.0, 0.0);
+          else b = make_double2(0.0, 0.0);
+        }
+        v[q] = make_double2(a.x - b.y, a.y + b.x);
+      }
+    }
+    blue2k<-1>(v, Pb, tab, fb.bhat_f, T, loop_lane());  // twice the DFT, registers 0..15
+    double2 z1, zm1;
+    {
+      // remap: u(r) = F~(m'') phase(m''), m'' = r (r <= L-1) or r - 2048
+      // (r >= 2048 - (L-1)), zero otherwise; F~(+-1) for the m' = 0 fold.
+      // Entry s of F~ sits at (s >> 6) 64 + 32 (s & 1) + ((s >> 1) & 31).
+      const int lane = loop_lane(), l = lane & 31, h = lane >> 5;
+      auto at = [](int s) { return ((s >> 6) << 6) | ((s & 1) << 5) | ((s >> 1) & 31); };
+#pragma unroll
+      for (int q = 0; q < 16; ++q) T2[q * 64 + lane] = v[q];
+      __builtin_amdgcn_wave_barrier();
+      r32_fence();
+#pragma unroll
+      for (int q = 24; q < 32; ++q) {
+        const int r = 2 * (l + 32 * q) + h;
+        const bool ok = r >= 2048 - (L - 1);
+        const double2 x = T2[at(r - 2048 + Pb)];  // r - 2048 + Pb >= Pb - 512 >= 1
+        v[q] = ok ? cmul(x, phase[max(r - 2048 + L - 1, 0)]) : make_double2(0.0, 0.0);
+      }
+      z1 = cscale(cmul(T2[at(1)], phase[L]), 0.5);
+      zm1 = cscale(cmul(T2[at(Pb - 1)], phase[L - 2]), 0.5);
+      __builtin_amdgcn_wave_barrier();
+      r32_fence();
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int r = 2 * (l + 32 * q) + h;
+        v[q] = r <= L - 1 ? cmul(v[q], phase[min(r, L - 1) + L - 1]) : make_double2(0.0, 0.0);
+      }
+#pragma unroll
+      for (int q = 8; q < 24; ++q) v[q] = make_double2(0.0, 0.0);
+    }
+    {
+      // convolution with the even weight taps: real spectrum, natural order
+      const int lane = loop_lane(), l = lane & 31, h = lane >> 5;
+      fft2k_fwd(v, T, tab, lane, [&](int q, double2 &z) { z = cscale(z, wre[l + 32 * q + 1024 * h].x); });
+      fft2k_inv<false>(v, T, tab, lane, [](int, double2 &) {});
+    }
+    // separate and fold, m' = 2 (l + 32 j) + h < L: G(m') is register j of
+    // the lane, G(-m') = G(2048 - m') comes through the buffer (registers
+    // 24..31 at (q - 24) 64 + lane, register 0 at 512 + lane)
+    const int lane = loop_lane(), l = lane & 31, h = lane >> 5;
+    // pole samples of the antisymmetric member (one address per wavefront)
+    const double2 *colX = X2 + m_slot_of(g, fixm(negA ? mA : mB));
+    const double2 p0 = colX[x2_row(g, nslot, ODD ? L - 1 : 0) * g.ldA];
+    const double2 p1 = ODD ? make_double2(0.0, 0.0) : colX[x2_row(g, nslot, L) * g.ldA];
+    // 2 pi / 2048 with the factor 2 of the Bluestein DFT
+    constexpr double sc = 2.0 * kPi / 4096.0;
+#pragma unroll
+    for (int q = 24; q < 32; ++q) T2[(q - 24) * 64 + lane] = v[q];
+    T2[512 + lane] = v[0];
+    __builtin_amdgcn_wave_barrier();
+    r32_fence();
+    const bool okA = mA >= -(L - 1) && mA <= L - 1, okB = mB >= -(L - 1) && mB <= L - 1;
+    const int an = n < 0 ? -n : n, sn = n < 0;
+    // G' rows of each member: combinations (cm, cn), cm in [cm0, cm1], cn in
+    // [cn0, cn1] -- a single one unless m = 0 or n = 0 (both signs of that
+    // index hold the same row)
+    const int cn0 = an == 0 ? 0 : sn, cn1 = an == 0 ? g.NS - 1 : sn;
+    int cm0[2], cm1[2];
+    double2 *row[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int m = e ? mB : mA;
+      const int am = m < 0 ? -m : m, sm = m < 0;
+      cm0[e] = am == 0 ? 0 : sm;
+      cm1[e] = am == 0 ? 1 : sm;
+      row[e] = Gp + ((int64_t)an * L + am) * C2 * L;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int mp = 2 * (l + 32 * j) + h;
+      const int rp = 2048 - mp;
+      const double2 zmv = T2[(((rp >> 6) - 24) << 6) + ((rp & 1) << 5) + ((rp >> 1) & 31)];
+      if (mp >= L) continue;
+      const double2 zp = make_double2(v[j].x * sc, v[j].y * sc);
+      const double2 zm = make_double2(zmv.x * sc, zmv.y * sc);
+      double2 GA, GB;
+      if (mp > 0) {
+        GA = make_double2(zp.x + sA * zm.x, zp.y + sA * zm.y);
+        GB = make_double2(zp.y - sA * zm.y, -(zp.x - sA * zm.x));
+      } else {
+        const double pp = kPi * kPi;
+        const double2 dz = make_double2(z1.x - zm1.x, z1.y - zm1.y);
+        if (!negA) {
+          GA = zp;
+          GB = make_double2(pp * dz.x, pp * dz.y);
+        } else {
+          GA = make_double2(-pp * dz.y, pp * dz.x);
+          GB = make_double2(zp.y, -zp.x);
+        }
+      }
+      double2 corr = cmul(p0, pole[mp]);
+      if (!ODD) corr = cadd(corr, cmul(p1, pole[L + mp]));
+      if (negA) GA = cadd(GA, corr);
+      else GB = cadd(GB, corr);
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        if (!(e ? okB : okA)) continue;
+        const double2 G0 = e ? GB : GA;
+#pragma nounroll
+        for (int cn = cn0; cn <= cn1; ++cn) {
+#pragma nounroll
+          for (int cm = cm0[e]; cm <= cm1[e]; ++cm) {
+            const double sg = ((cm != cn) && (mp & 1)) ? -1.0 : 1.0;
+            row[e][(int64_t)(cm + 2 * cn) * L + mp] = make_double2(sg * G0.x, sg * G0.y);
+          }
+        }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    r32_fence();  // the buffer is read before the next group's transforms
+  }
+}
+
 // ---------------------------------------------------------------- paired beta, LDS engine
 // The paired beta stages (see above) for the other DFT lengths: a
 // workgroup transforms K pairs, 2K consecutive m from m_lo, in the LDS
@@ -1994,6 +2175,19 @@ void launch_fwd_beta(const Geo &g, const double2 *X2, double2 *Gp, NBase nb, con
       kern<<<grid, 64 * kRows32Waves, sh, st>>>(g, fb, X2, Gp, nb.base, phase, wr_even, pole);
     };
     (g.Pb & 1) ? go(k_fwd_beta_r32<1>) : go(k_fwd_beta_r32<0>);
+    SO3_LAUNCH_CHECK();
+    return;
+  }
+  if (!adjoint && wr_even && pole && Pw == 2048 && g.L >= 257 && g.L <= 512 && g.Pb >= 513 && g.Pb <= 1024) {
+    const FftDesc fb = fft_plan_r2k(g.Pb);
+    constexpr int C = 2 * kRows32Waves;
+    const size_t sh = sizeof(double2) * kR2kTab + sizeof(double) * kRows32Waves * kR32T;
+    const int chunks = (g.nA + C - 1) / C;
+    auto go = [&](auto kern) {
+      const unsigned grid = reg_grid(kern, 64 * kRows32Waves, sh, (int64_t)g.nG * chunks);
+      kern<<<grid, 64 * kRows32Waves, sh, st>>>(g, fb, X2, Gp, nb.base, phase, wr_even, pole);
+    };
+    (g.Pb & 1) ? go(k_fwd_beta_r2k<1>) : go(k_fwd_beta_r2k<0>);
     SO3_LAUNCH_CHECK();
     return;
   }

@@ -166,6 +166,8 @@ Plan build_plan(const so3_parameters_t *p, Method method) {
   // the register-resident forward beta (beta DFTs on Bluestein M = 1024, so
   // L = 129..256) convolves at 1024 points; any Pw >= 4L - 3 is exact
   if (L <= 256 && fft_plan(g.Pb).blue && fft_plan(g.Pb).len == 1024) P.Pw = 1024;
+  // the 2048-point register forward beta (L = 257..512) convolves at 2048
+  if (L >= 257 && L <= 512) P.Pw = 2048;
   if (const int pw = tuned(p, method).pw) P.Pw = pw;  // the tuner's choice (>= 4L - 3)
   {
     const int Pw = P.Pw;
