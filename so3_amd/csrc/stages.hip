@@ -452,34 +452,34 @@ __global__ __launch_bounds__(kThreads) void k_inv_gamma(Geo g, FftDesc fd, int K
   double2 *A = lds;
   const double2 *tw = fft_load_twiddles(lds + K * ld, fd);
   // columns p = b nA + a of f; column (b, a) of X2 holds n slot k at
-  // x2_row(k, b) ldA + a
+  // x2_row(k, b) ldA + a.  K divides the workgroup: thread t always handles
+  // column s = t % K (one division per thread) and slots k = t / K + i (256 / K)
   const int64_t S = (int64_t)g.nB * g.nA;
-  const int64_t p0 = (int64_t)blockIdx.x * K;
   const int nG = 2 * g.N - 1;
-  for (int idx = threadIdx.x; idx < K * nG; idx += blockDim.x) {
-    const int k = idx / K, s = idx - k * K;
-    const int64_t p = p0 + s;
+  const int s = threadIdx.x % K, k0 = threadIdx.x / K, ks = blockDim.x / K;
+  const int64_t p = (int64_t)blockIdx.x * K + s;
+  const bool okp = p < S;
+  const int b = okp ? (int)p / g.nA : 0, a = okp ? (int)p - b * g.nA : 0;
+  const int64_t xb = x2_row(g, 0, b) * g.ldA + a, xs = (int64_t)x2_w(g, b) * g.ldA;
+  for (int k = k0; k < nG; k += ks) {
     double2 v = make_double2(0.0, 0.0);
-    if (p < S) {
-      const int b = (int)(p / g.nA), a = (int)(p % g.nA);
+    if (okp) {
       if (!g.real) {
-        v = X2[x2_row(g, k, b) * g.ldA + a];
+        v = X2[xb + k * xs];
       } else if (k == 0) {
-        v = make_double2(X2[x2_row(g, 0, b) * g.ldA + a].x, 0.0);
+        v = make_double2(X2[xb].x, 0.0);
       } else if (k < g.N) {
-        v = X2[x2_row(g, k, b) * g.ldA + a];
+        v = X2[xb + k * xs];
       } else {
-        v = cconj(X2[x2_row(g, nG - k, b) * g.ldA + a]);
+        v = cconj(X2[xb + (nG - k) * xs]);
       }
     }
     A[s * ld + k] = v;
   }
   __syncthreads();
   fft_run<+1>(A, K, ld, fd, tw);
-  for (int idx = threadIdx.x; idx < K * nG; idx += blockDim.x) {
-    const int k = idx / K, s = idx - k * K;
-    const int64_t p = p0 + s;
-    if (p >= S) continue;
+  if (!okp) return;
+  for (int k = k0; k < nG; k += ks) {
     if (g.real)
       fr[k * S + p] = A[s * ld + k].x * scale;
     else
@@ -495,25 +495,25 @@ __global__ __launch_bounds__(kThreads) void k_fwd_gamma(Geo g, FftDesc fd, int K
   extern __shared__ double2 lds[];
   double2 *A = lds;
   const double2 *tw = fft_load_twiddles(lds + K * ld, fd);
-  // columns p = b nA + a of f map to columns (b, a) of X2
+  // columns p = b nA + a of f map to columns (b, a) of X2; K divides the
+  // workgroup: thread t always handles column s = t % K, slots k = t / K + i (256 / K)
   const int64_t S = (int64_t)g.nB * g.nA;
-  const int64_t p0 = (int64_t)blockIdx.x * K;
   const int nG = 2 * g.N - 1;
-  for (int idx = threadIdx.x; idx < K * nG; idx += blockDim.x) {
-    const int k = idx / K, s = idx - k * K;
-    const int64_t p = p0 + s;
+  const int s = threadIdx.x % K, k0 = threadIdx.x / K, ks = blockDim.x / K;
+  const int64_t p = (int64_t)blockIdx.x * K + s;
+  const bool okp = p < S;
+  for (int k = k0; k < nG; k += ks) {
     double2 v = make_double2(0.0, 0.0);
-    if (p < S) v = g.real ? make_double2(fr[k * S + p], 0.0) : fc[k * S + p];
+    if (okp) v = g.real ? make_double2(fr[k * S + p], 0.0) : fc[k * S + p];
     A[s * ld + k] = v;
   }
   __syncthreads();
   fft_run<-1>(A, K, ld, fd, tw);
+  if (!okp) return;
+  const int b = (int)p / g.nA, a = (int)p - b * g.nA;
+  const int64_t xb = x2_row(g, 0, b) * g.ldA + a, xs = (int64_t)x2_w(g, b) * g.ldA;
   const int nout = g.real ? g.N : nG;
-  for (int idx = threadIdx.x; idx < K * nout; idx += blockDim.x) {
-    const int k = idx / K, s = idx - k * K;
-    const int64_t p = p0 + s;
-    if (p < S) X2[x2_row(g, k, (int)(p / g.nA)) * g.ldA + p % g.nA] = cscale(A[s * ld + k], sc);
-  }
+  for (int k = k0; k < nout; k += ks) X2[xb + k * xs] = cscale(A[s * ld + k], sc);
 }
 
 // ---------------------------------------------------------------- steerable gamma
@@ -1887,6 +1887,7 @@ __global__ void k_fwd_zero(Geo g, double2 *__restrict__ flmn, const int64_t *__r
 // sequences per workgroup for a kernel that runs two descriptors (each must
 // fit its own register capacity)
 int choose_k2(int ld, const FftDesc &a, const FftDesc &b, int tables, int want);
+int pow2_floor(int k);
 
 int choose_k(int ld, int cap, int len, int tables, int want) {
   int k = std::min(want, cap / len);
@@ -1895,6 +1896,12 @@ int choose_k(int ld, int cap, int len, int tables, int want) {
   if (k < 1 || (size_t)k * ld * sizeof(double2) + tables > (size_t)lds_limit() || len > cap)
     throw HipError("DFT length " + std::to_string(len) + " exceeds the LDS / register budget");
   return k;
+}
+
+int pow2_floor(int k) {
+  int p = 1;
+  while (2 * p <= k) p *= 2;
+  return p;
 }
 
 int choose_k2(int ld, const FftDesc &a, const FftDesc &b, int tables, int want) {
@@ -2111,7 +2118,8 @@ void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, double scale, hi
   }
   const int ld = fft_ld(fd);
   const int tb = tw_count(fd) * (int)sizeof(double2);
-  const int K = choose_k(ld, fft_capacity(fd), fd.len, tb, 64);
+  // a power of two: the kernel's threads keep one column each
+  const int K = pow2_floor(choose_k(ld, fft_capacity(fd), fd.len, tb, 64));
   const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
   set_lds(k_inv_gamma, sh);
   const int64_t S = (int64_t)g.nB * g.nA;
@@ -2132,7 +2140,8 @@ void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, double scale, hi
   }
   const int ld = fft_ld(fd);
   const int tb = tw_count(fd) * (int)sizeof(double2);
-  const int K = choose_k(ld, fft_capacity(fd), fd.len, tb, 64);
+  // a power of two: the kernel's threads keep one column each
+  const int K = pow2_floor(choose_k(ld, fft_capacity(fd), fd.len, tb, 64));
   const size_t sh = (size_t)K * ld * sizeof(double2) + tb;
   set_lds(k_fwd_gamma, sh);
   const int64_t S = (int64_t)g.nB * g.nA;
