@@ -96,7 +96,7 @@ double2 *upload(const std::vector<cld> &v) {
 
 // mode 0: the general rule (the workgroup engine); 1: lengths in [257, 512]
 // take Bluestein with M = 1024 (the register-resident FFTs); 2: as 1, and
-// lengths in [513, 1024] take Bluestein with M = 2048
+// lengths in [513, 2048] take Bluestein with M = 2048 or 4096
 FftDesc build(int n, int mode) {
   const bool reg1024 = mode >= 1;
   FftDesc d{};
@@ -114,9 +114,10 @@ FftDesc build(int n, int mode) {
   // runs on the register-resident FFTs (fft_reg.hpp, fft_reg32.hpp) faster
   // than the workgroup engine runs even a smooth n directly
   const bool r1024 = reg1024 && n >= 257 && n <= 512;
-  // mode 2: every n in [513, 1024] takes Bluestein with M = 2048, which the
-  // one-sequence 2048-point register FFT (fft_reg2k.hpp) runs
-  const bool r2048 = mode == 2 && n >= 513 && n <= 1024;
+  // mode 2: every n in [513, 2048] takes Bluestein with M = 2048 or 4096,
+  // which the 2048- and 4096-point register FFTs (fft_reg2k.hpp,
+  // fft_reg4k.hpp) run
+  const bool r2048 = mode == 2 && n >= 513 && n <= 2048;
   bool direct = factor(n, rad) && !r1024 && !r2048;
   if (direct) {
     int cap = kFftThreads * kFftElems * 4;
@@ -216,8 +217,8 @@ FftDesc fft_plan(int n) { return plan_cached(n, 1); }
 
 FftDesc fft_plan_lds(int n) { return n >= 257 && n <= 512 ? plan_cached(n, 0) : plan_cached(n, 1); }
 
-FftDesc fft_plan_r2k(int n) {
-  if (n < 513 || n > 1024) throw HipError("fft_plan_r2k: length outside [513, 1024]");
+FftDesc fft_plan_reg(int n) {
+  if (n < 513 || n > 2048) throw HipError("fft_plan_reg: length outside [513, 2048]");
   return plan_cached(n, 2);
 }
 

@@ -17,9 +17,10 @@ FftDesc fft_plan(int n);
 // (which only the register-resident FFTs run faster).
 FftDesc fft_plan_lds(int n);
 
-// The plan of the 2048-point register FFT (fft_reg2k.hpp): Bluestein with
-// M = 2048 for every n in [513, 1024].
-FftDesc fft_plan_r2k(int n);
+// The plan of the 2048- and 4096-point register FFTs (fft_reg2k.hpp,
+// fft_reg4k.hpp): Bluestein with M = 2048 for every n in [513, 1024] and
+// M = 4096 for every n in [1025, 2048].
+FftDesc fft_plan_reg(int n);
 
 // Largest S * len one workgroup can run through every pass of d (each
 // thread holds fft_nb(R) butterflies of radix R).

@@ -23,6 +23,7 @@
 #include "fft_plan.hpp"
 #include "fft_reg.hpp"
 #include "fft_reg2k.hpp"
+#include "fft_reg4k.hpp"
 #include "fft_reg32.hpp"
 #include "kernels.hpp"
 
@@ -274,6 +275,59 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
     for (int q = 0; q < 16; ++q) {
       const int j = 2 * ((ls & 31) + 32 * q) + (ls >> 5);
       if (j < n) row[j] = cscale(v[q], sc);
+    }
+  }
+}
+
+// wavefronts per workgroup of the 4096-point kernels (eight measured faster
+// than two, whose barriers would couple only the pair, and four)
+#ifndef SO3_R4K_WAVES
+#define SO3_R4K_WAVES 8
+#endif
+constexpr int kR4kWaves = SO3_R4K_WAVES;
+
+// Rows of a Bluestein length n in [1025, 2048] on the 4096-point register
+// FFT (fft_reg4k.hpp): one row per pair of wavefronts (even samples in
+// wavefront 0, odd ones in wavefront 1), four rows per workgroup; row maps
+// as in k_rows_reg.  Every wavefront runs every group (the transform
+// synchronises the workgroup); a pair past the last row transforms a repeat
+// of that row and stores nothing.
+template <int DIR, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_rows_r4k(
+    FftDesc fd, const double2 *src, double2 *dst, int64_t nrows, int n, int64_t rs, double scale, Geo g,
+    int map) {
+  constexpr int kPairs = W / 2;
+  extern __shared__ double2 lds[];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), e = wv & 1;
+  double2 *tab = lds;
+  double *T = (double *)(lds + kR4kTab) + wv * kR32T;
+  const double *Tp = (double *)(lds + kR4kTab) + (wv ^ 1) * kR32T;
+  r4k_stage_blue(tab, fd);
+  __syncthreads();
+  const int64_t groups = (nrows + kPairs - 1) / kPairs;
+  auto row_at = [&](int64_t r, bool IN) -> int64_t {
+    if (map == kRowsFlat) return r * rs;
+    const int k = (int)(r / g.nB), b = (int)(r % g.nB);
+    const bool x2 = (map == kRowsX2F) == IN;
+    return x2 ? x2_row(g, k, b) * g.ldA : r * g.nA;
+  };
+  for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
+    const int64_t r = t * kPairs + (wv >> 1);
+    const int64_t rr = r < nrows ? r : nrows - 1;
+    const int lane = loop_lane(), l = lane & 31, h = lane >> 5;
+    const double2 *in = src + row_at(rr, true);
+    double2 v[32];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = in[min(4 * (l + 32 * q) + 2 * h + e, n - 1)];  // past n: cleared by the chirp
+    blue4k<DIR>(v, n, tab, fd, T, Tp, e, lane);
+    if (r >= nrows) continue;
+    // store addresses from a fresh opaque lane index (see k_rows_r2k)
+    const int ls = loop_lane();
+    double2 *row = dst + row_at(r, false);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int j = 4 * ((ls & 31) + 32 * q) + 2 * (ls >> 5) + e;
+      if (j < n) row[j] = cscale(v[q], scale);
     }
   }
 }
@@ -1170,6 +1224,102 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
   }
 }
 
+// Inverse beta, paired, on the 4096-point register FFT (Pb in [1025, 2048],
+// L = 513 ... 1024): the pair (mA, mA + 1), mA = m0 + 2 p, of wavefront pair
+// p as one sequence A + i B, sample k in wavefront k & 1 (layout E of
+// fft_reg4k.hpp, k = 4 (l + 32 q) + 2 h + e), so a group covers W
+// consecutive m of one n slot.  Z(b') has the parity of b (b' = Pb - 1 - b
+// with Pb odd, or Pb - b with Pb even) and comes through the wavefront's own
+// transpose buffer.  ODD: Pb odd (MW).
+template <int ODD>
+__global__ __launch_bounds__(64 * kR4kWaves) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_inv_beta_r4k(
+    Geo g, FftDesc fd, const double2 *__restrict__ X1, double2 *__restrict__ X2,
+    const int64_t *__restrict__ nbase, const double2 *__restrict__ phase) {
+  constexpr int W = kR4kWaves;
+  constexpr int C = W;        // m per group (a pair per two wavefronts)
+  constexpr int kQB = 9;      // b = 4 (l + 32 j) + 2 h + e < nB <= 1025
+  extern __shared__ double2 lds[];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), e = wv & 1;
+  double2 *tab = lds;
+  double *T = (double *)(lds + kR4kTab) + wv * kR32T;
+  const double *Tp = (double *)(lds + kR4kTab) + (wv ^ 1) * kR32T;
+  r4k_stage_blue(tab, fd);
+  __syncthreads();
+  const int L = g.L, Pb = g.Pb;  // 513 <= L <= 1024
+  const int mlo = -(L - 1) - g.ss;
+  const int chunks = (g.nA + C - 1) / C;
+  const int64_t groups = (int64_t)g.nG * chunks;
+  for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
+    __syncthreads();  // one barrier per group keeps the workgroup's stores to X2 rows in step
+    const int lane = loop_lane(), l = lane & 31, h = lane >> 5;
+    const int nslot = (int)(t / chunks);
+    const int n = n_of_slot(g, nslot);
+    const int mA = mlo + (int)(t % chunks) * C + 2 * (wv >> 1), mB = mA + 1;
+    const bool act = nbase[n + g.N - 1] != kNoBlock;  // the whole workgroup
+    const bool stA = mA <= L - 1, stB = mB <= L - 1;
+    const bool dA = act && mA >= -(L - 1), dB = act && mB >= -(L - 1);
+    const int cA = m_slot_of(g, stA ? mA : 0), cB = m_slot_of(g, stB ? mB : 0);
+    if (!act) {  // block n not in the transform: zeros
+#pragma unroll
+      for (int j = 0; j < kQB; ++j) {
+        const int b = 4 * (l + 32 * j) + 2 * h + e;
+        if (b >= g.nB) continue;
+        double2 *row = X2 + x2_row(g, nslot, b) * g.ldA;
+        if (stA) row[cA] = make_double2(0.0, 0.0);
+        if (stB) row[cB] = make_double2(0.0, 0.0);
+      }
+      continue;
+    }
+    auto fixm = [&](int m) { return m < -(L - 1) ? m + 2 : m > L - 1 ? (L - 1) - ((m - (L - 1)) & 1) : m; };
+    const double2 *rA = X1 + ((int64_t)nslot * g.nA + m_slot_of(g, fixm(mA))) * L;
+    const double2 *rB = X1 + ((int64_t)nslot * g.nA + m_slot_of(g, fixm(mB))) * L;
+    const bool negA = ((mA + n) & 1) != 0;  // s_A = -1 (then s_B = +1); uniform over the pair
+    double2 v[32];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int k = min(4 * (l + 32 * q) + 2 * h + e, Pb - 1);
+      const int mp = k <= L - 1 ? k : k - Pb;
+      const int am = mp < 0 ? -mp : mp;
+      double2 a = rA[am], b = rB[am];
+      if (mp < 0) {  // X1(-m') = s X1(m')
+        if (negA) a = make_double2(-a.x, -a.y);
+        else b = make_double2(-b.x, -b.y);
+      }
+      if (k == 0) {  // the antisymmetric member vanishes at m' = 0
+        if (negA) a = make_double2(0.0, 0.0);
+        else b = make_double2(0.0, 0.0);
+      }
+      if (!ODD && k == L) a = b = make_double2(0.0, 0.0);  // MW_SS: m' = -L
+      v[q] = cmul(make_double2(a.x - b.y, a.y + b.x), phase[k]);
+    }
+    blue4k<+1>(v, Pb, tab, fd, T, Tp, e, lane);
+    // Z(b') through the buffer: entry q 64 + lane holds register q; sample
+    // j = 2 i + e of this wavefront sits at (i >> 6) 64 + 32 (i & 1) + ((i >> 1) & 31)
+    double2 *T2 = (double2 *)T;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) T2[q * 64 + lane] = v[q];
+    __builtin_amdgcn_wave_barrier();
+    r32_fence();
+    const double sA = negA ? -1.0 : 1.0;
+#pragma unroll
+    for (int j = 0; j < kQB; ++j) {
+      const int b = 4 * (l + 32 * j) + 2 * h + e;
+      const int bp = max(ODD ? Pb - 1 - b : (b == 0 ? 0 : Pb - b), e);
+      const int ip = bp >> 1;
+      const double2 zp = T2[((ip >> 6) << 6) | ((ip & 1) << 5) | ((ip >> 1) & 31)];
+      if (b >= g.nB) continue;
+      // A = (Z + s_A Z') / 2, B = (Z - s_A Z') / (2 i)
+      const double2 su = make_double2(v[j].x + sA * zp.x, v[j].y + sA * zp.y);
+      const double2 df = make_double2(v[j].x - sA * zp.x, v[j].y - sA * zp.y);
+      double2 *row = X2 + x2_row(g, nslot, b) * g.ldA;
+      if (stA) row[cA] = dA ? make_double2(0.5 * su.x, 0.5 * su.y) : make_double2(0.0, 0.0);
+      if (stB) row[cB] = dB ? make_double2(0.5 * df.y, -0.5 * df.x) : make_double2(0.0, 0.0);
+    }
+    __builtin_amdgcn_wave_barrier();
+    r32_fence();  // the buffer is read before the next group's transforms
+  }
+}
+
 // forward beta, paired (not the adjoint): the Bluestein DFT and the
 // convolution with the even weight taps run once per pair.  The two odd
 // taps w(+-1) = -+ i pi/2 cancel in the fold G'(m') = G(m') + s G(-m') for
@@ -1971,8 +2121,21 @@ void launch_inv_beta(const Geo &g, const double2 *X1, double2 *X2, NBase nb, con
     SO3_LAUNCH_CHECK();
     return;
   }
+  if (g.Pb >= 1025 && g.Pb <= 2048 && g.nB <= 1025) {
+    const FftDesc f4 = fft_plan_reg(g.Pb);
+    constexpr int C = kR4kWaves;
+    const size_t sh = sizeof(double2) * kR4kTab + sizeof(double) * kR4kWaves * kR32T;
+    const int chunks = (g.nA + C - 1) / C;
+    auto go = [&](auto kern) {
+      const unsigned grid = reg_grid(kern, 64 * kR4kWaves, sh, (int64_t)g.nG * chunks);
+      kern<<<grid, 64 * kR4kWaves, sh, st>>>(g, f4, X1, X2, nb.base, phase);
+    };
+    (g.Pb & 1) ? go(k_inv_beta_r4k<1>) : go(k_inv_beta_r4k<0>);
+    SO3_LAUNCH_CHECK();
+    return;
+  }
   if (g.Pb >= 513 && g.Pb <= 1024 && g.nB <= 513) {
-    const FftDesc f2 = fft_plan_r2k(g.Pb);
+    const FftDesc f2 = fft_plan_reg(g.Pb);
     constexpr int C = 2 * kRows32Waves;
     const size_t sh = sizeof(double2) * kR2kTab + sizeof(double) * kRows32Waves * kR32T;
     const int chunks = (g.nA + C - 1) / C;
@@ -2053,7 +2216,7 @@ void launch_alpha_fx2(const Geo &g, const double2 *f, double2 *X2, int dir, doub
 // rows of length n in [513, 1024] on the 2048-point register FFT
 static void launch_rows_r2k(const double2 *src, double2 *dst, int64_t nrows, int n, int64_t rs, int dir,
                             double scale, const Geo &g, int map, hipStream_t st) {
-  const FftDesc fd = fft_plan_r2k(n);
+  const FftDesc fd = fft_plan_reg(n);
   const size_t sh = sizeof(double2) * kR2kTab + sizeof(double) * kRows32Waves * kR32T;
   auto go = [&](auto kern) {
     const unsigned grid = reg_grid(kern, 64 * kRows32Waves, sh, (nrows + kRows32Waves - 1) / kRows32Waves);
@@ -2063,10 +2226,29 @@ static void launch_rows_r2k(const double2 *src, double2 *dst, int64_t nrows, int
   SO3_LAUNCH_CHECK();
 }
 
+// rows of length n in [1025, 2048] on the 4096-point register FFT
+static void launch_rows_r4k(const double2 *src, double2 *dst, int64_t nrows, int n, int64_t rs, int dir,
+                            double scale, const Geo &g, int map, hipStream_t st) {
+  const FftDesc fd = fft_plan_reg(n);
+  constexpr int W = kR4kWaves;
+  const size_t sh = sizeof(double2) * kR4kTab + sizeof(double) * W * kR32T;
+  const int64_t groups = (nrows + W / 2 - 1) / (W / 2);
+  auto go = [&](auto kern) {
+    const unsigned grid = reg_grid(kern, 64 * W, sh, groups);
+    kern<<<grid, 64 * W, sh, st>>>(fd, src, dst, nrows, n, rs, scale, g, map);
+  };
+  dir > 0 ? go(k_rows_r4k<+1, W>) : go(k_rows_r4k<-1, W>);
+  SO3_LAUNCH_CHECK();
+}
+
 void launch_rows(double2 *X, int64_t nrows, int n, int ldr, int dir, double scale, hipStream_t st) {
   if (nrows <= 0) return;
   if (n >= 513 && n <= 1024) {
     launch_rows_r2k(X, X, nrows, n, ldr, dir, scale, Geo{}, kRowsFlat, st);
+    return;
+  }
+  if (n >= 1025 && n <= 2048) {
+    launch_rows_r4k(X, X, nrows, n, ldr, dir, scale, Geo{}, kRowsFlat, st);
     return;
   }
   const FftDesc fd = fft_plan(n);
@@ -2188,7 +2370,7 @@ void launch_fwd_beta(const Geo &g, const double2 *X2, double2 *Gp, NBase nb, con
     return;
   }
   if (!adjoint && wr_even && pole && Pw == 2048 && g.L >= 257 && g.L <= 512 && g.Pb >= 513 && g.Pb <= 1024) {
-    const FftDesc fb = fft_plan_r2k(g.Pb);
+    const FftDesc fb = fft_plan_reg(g.Pb);
     constexpr int C = 2 * kRows32Waves;
     const size_t sh = sizeof(double2) * kR2kTab + sizeof(double) * kRows32Waves * kR32T;
     const int chunks = (g.nA + C - 1) / C;

@@ -274,7 +274,7 @@ Plan build_plan(const so3_parameters_t *p, Method method) {
   fft_plan_lds(P.Pw);
   fft_plan_lds(2 * N);
   for (int n : {g.Pb, g.nA, 2 * N - 1})
-    if (n >= 513 && n <= 1024) fft_plan_r2k(n);
+    if (n >= 513 && n <= 2048) fft_plan_reg(n);
   return P;
 }
 

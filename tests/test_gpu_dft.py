@@ -100,3 +100,31 @@ def test_dft_register_2048(n, howmany, direction):
     ref = np.fft.fft(x, axis=1) if direction < 0 else np.fft.ifft(x, axis=1) * n
     err = np.abs(y - ref).max() / np.abs(ref).max()
     assert err < 5e-15 * np.log2(n), err
+
+
+# Lengths with a Bluestein inner length of 4096 (1025 <= n <= 2048) run on the
+# 4096-point register FFT, one row per pair of wavefronts (four per
+# workgroup): odd and even n, and row counts that leave the last workgroup
+# with idle pairs (1, 5) or none (4).
+@pytest.mark.parametrize("n", [1025, 1026, 1500, 1537, 2047, 2048])
+@pytest.mark.parametrize("howmany", [1, 4, 5])
+@pytest.mark.parametrize("direction", [-1, 1])
+def test_dft_register_4096(n, howmany, direction):
+    lib()
+    h = _hip()
+    rng = np.random.default_rng(9000 * n + howmany)
+    x = rng.standard_normal((howmany, n)) + 1j * rng.standard_normal((howmany, n))
+    x = np.ascontiguousarray(x)
+    d = ctypes.c_void_p()
+    assert h.hipMalloc(ctypes.byref(d), x.nbytes) == 0
+    try:
+        assert h.hipMemcpy(d, x.ctypes.data, x.nbytes, 1) == 0
+        A.dft_device(d.value, n, howmany, direction)
+        assert h.hipDeviceSynchronize() == 0
+        y = np.empty_like(x)
+        assert h.hipMemcpy(y.ctypes.data, d, x.nbytes, 2) == 0
+    finally:
+        h.hipFree(d)
+    ref = np.fft.fft(x, axis=1) if direction < 0 else np.fft.ifft(x, axis=1) * n
+    err = np.abs(y - ref).max() / np.abs(ref).max()
+    assert err < 5e-15 * np.log2(n), err

@@ -114,7 +114,10 @@ def test_sparse_inverse_and_round_trip(oracle, L, N, real, ss):
                                          # alpha and the inverse beta on the 2048-point
                                          # register FFT (Pb = 513 ... 1024)
                                          (257, 3, 0, 0), (300, 2, 1, 0), (384, 2, 0, 1),
-                                         (511, 1, 1, 1), (512, 2, 0, 0)])
+                                         (511, 1, 1, 1), (512, 2, 0, 0),
+                                         # alpha and the inverse beta on the 4096-point
+                                         # register FFT (Pb = 1025 ... 2048)
+                                         (513, 1, 0, 0), (700, 1, 1, 1), (640, 2, 0, 1)])
 def test_forward_arbitrary_samples_paired_beta(oracle, L, N, real, ss):
     p = P.SO3Parameters(L=L, N=N, reality=real, sampling_scheme=ss, n_order=P.SO3_N_ORDER_ZERO_FIRST)
     rng = np.random.default_rng(L + 10 * real + 100 * ss)
