@@ -1723,6 +1723,182 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
   }
 }
 
+// Forward beta, paired, on the 4096-point register FFT (Pb in [1025, 2048],
+// L = 513 ... 1024, convolution length Pw = 4096): the algorithm of
+// k_fwd_beta_r2k with one pair (mA, mA + 1), mA = m0 + 2 p, per pair of
+// wavefronts p, sample k in wavefront k & 1 (layout E of fft_reg4k.hpp).
+// The remap keeps m'' = r >= 0 in the wavefront's own registers 0..7 and
+// gathers m'' = r - 4096 < 0 (registers 24..31) from entry s = r - 4096 + Pb
+// of the Bluestein output, in the partner wavefront when Pb is odd (through
+// its transpose buffer, between workgroup barriers); the fold takes G(-m') =
+// G(4096 - m'), of the parity of m', from the wavefront's own registers
+// 24..31.  ODD: Pb odd (MW).
+template <int ODD>
+__global__ __launch_bounds__(64 * kR4kWaves) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_fwd_beta_r4k(
+    Geo g, FftDesc fb, const double2 *__restrict__ X2, double2 *__restrict__ Gp,
+    const int64_t *__restrict__ nbase, const double2 *__restrict__ phase, const double2 *__restrict__ wre,
+    const double2 *__restrict__ pole) {
+  constexpr int W = kR4kWaves;
+  constexpr int C = W;  // m per group
+  constexpr double kPi = 3.141592653589793238462643383279502884;
+  extern __shared__ double2 lds[];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), e = wv & 1;
+  double2 *tab = lds;
+  double *T = (double *)(lds + kR4kTab) + wv * kR32T;
+  const double *Tp = (double *)(lds + kR4kTab) + (wv ^ 1) * kR32T;
+  double2 *T2 = (double2 *)T;
+  r4k_stage_blue(tab, fb);
+  __syncthreads();  // tables staged
+  const int L = g.L, Pb = g.Pb;  // 513 <= L <= 1024
+  const int mlo = -(L - 1) - g.ss;
+  const int chunks = (g.nA + C - 1) / C;
+  const int64_t groups = (int64_t)g.nG * chunks;
+  const int C2 = 2 * g.NS;
+  auto fixm = [&](int m) { return m < -(L - 1) ? m + 2 : m > L - 1 ? (L - 1) - ((m - (L - 1)) & 1) : m; };
+  // entry i of a wavefront's samples (j = 2 i + e) in its buffer
+  auto at = [](int i) { return ((i >> 6) << 6) | ((i & 1) << 5) | ((i >> 1) & 31); };
+  for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
+    __syncthreads();  // one barrier per group keeps the workgroup's X2 reads in step
+    const int nslot = (int)(t / chunks);
+    const int n = n_of_slot(g, nslot);
+    if (nbase[n + g.N - 1] == kNoBlock) continue;  // block not transformed (the whole workgroup)
+    const int mA = mlo + (int)(t % chunks) * C + 2 * (wv >> 1), mB = mA + 1;
+    const bool negA = ((mA + n) & 1) != 0;  // s_A = -1 (then s_B = +1); uniform over the pair
+    const double sA = negA ? -1.0 : 1.0;
+    double2 v[32];
+    {
+      // extended samples: MW F(2L-2-b) for b >= L, MW_SS F(2L-b) for b > L
+      const int lane = loop_lane(), l = lane & 31, h = lane >> 5;
+      const double2 *colA = X2 + m_slot_of(g, fixm(mA)), *colB = X2 + m_slot_of(g, fixm(mB));
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int k = min(4 * (l + 32 * q) + 2 * h + e, Pb - 1);
+        const bool direct = ODD ? k < L : k <= L;
+        const int kr = direct ? k : (ODD ? 2 * L - 2 - k : 2 * L - k);
+        const int64_t off = x2_row(g, nslot, kr) * g.ldA;
+        double2 a = colA[off], b = colB[off];
+        if (!direct) {
+          if (negA) a = make_double2(-a.x, -a.y);
+          else b = make_double2(-b.x, -b.y);
+        }
+        const bool pk = ODD ? k == L - 1 : (k == 0 || k == L);
+        if (pk) {  // the antisymmetric member's pole samples go through the pole table
+          if (negA) a = make_double2(0.0, 0.0);
+          else b = make_double2(0.0, 0.0);
+        }
+        v[q] = make_double2(a.x - b.y, a.y + b.x);
+      }
+    }
+    blue4k<-1>(v, Pb, tab, fb, T, Tp, e, loop_lane());  // the DFT, registers 0..15
+    double2 z1, zm1;
+    {
+      // remap: u(r) = F~(m'') phase(m''), m'' = r (r <= L-1) or r - 4096
+      // (r >= 4096 - (L-1)), zero otherwise; F~(+-1) for the m' = 0 fold
+      const int lane = loop_lane(), l = lane & 31, h = lane >> 5;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) T2[q * 64 + lane] = v[q];
+      __syncthreads();
+      // entry s = r - 4096 + Pb has the parity e ^ (Pb & 1)
+      const double2 *S2 = (const double2 *)(ODD ? Tp : T);
+#pragma unroll
+      for (int q = 24; q < 32; ++q) {
+        const int r = 4 * (l + 32 * q) + 2 * h + e;
+        const bool ok = r >= 4096 - (L - 1);
+        const double2 x = S2[at((r - 4096 + Pb) >> 1)];  // r - 4096 + Pb >= Pb - 1024 >= 1
+        v[q] = ok ? cmul(x, phase[max(r - 4096 + L - 1, 0)]) : make_double2(0.0, 0.0);
+      }
+      // F~(1) in wavefront 1 at i = 0; F~(-1) = F~(Pb - 1) in wavefront (Pb - 1) & 1
+      const double2 *B1 = (const double2 *)(e ? T : Tp), *Bm1 = (const double2 *)(((Pb - 1) & 1) == e ? T : Tp);
+      z1 = cmul(B1[at(0)], phase[L]);
+      zm1 = cmul(Bm1[at((Pb - 1) >> 1)], phase[L - 2]);
+      __syncthreads();  // both wavefronts have read the buffers before the transforms reuse them
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int r = 4 * (l + 32 * q) + 2 * h + e;
+        v[q] = r <= L - 1 ? cmul(v[q], phase[min(r, L - 1) + L - 1]) : make_double2(0.0, 0.0);
+      }
+#pragma unroll
+      for (int q = 8; q < 24; ++q) v[q] = make_double2(0.0, 0.0);
+    }
+    {
+      // convolution with the even weight taps: real spectrum, natural order
+      const int lane = loop_lane(), l = lane & 31, h = lane >> 5;
+      fft4k_fwd(v, T, Tp, tab, e, lane,
+                [&](int q, double2 &z) { z = cscale(z, wre[l + 32 * q + 1024 * h + 2048 * e].x); });
+      fft4k_inv<false>(v, T, Tp, tab, e, lane, [](int, double2 &) {});
+    }
+    // separate and fold, m' = 4 (l + 32 j) + 2 h + e < L: G(m') is register
+    // j, G(-m') = G(4096 - m') sits at i' = (4096 - m') >> 1 of this
+    // wavefront, registers 24..31 (at (q - 24) 64 + ...) or register 0 (512 + ...)
+    const int lane = loop_lane(), l = lane & 31, h = lane >> 5;
+    const double2 *colX = X2 + m_slot_of(g, fixm(negA ? mA : mB));
+    const double2 p0 = colX[x2_row(g, nslot, ODD ? L - 1 : 0) * g.ldA];
+    const double2 p1 = ODD ? make_double2(0.0, 0.0) : colX[x2_row(g, nslot, L) * g.ldA];
+    constexpr double sc = 2.0 * kPi / 4096.0;
+#pragma unroll
+    for (int q = 24; q < 32; ++q) T2[(q - 24) * 64 + lane] = v[q];
+    T2[512 + lane] = v[0];
+    __builtin_amdgcn_wave_barrier();
+    r32_fence();
+    const bool okA = mA >= -(L - 1) && mA <= L - 1, okB = mB >= -(L - 1) && mB <= L - 1;
+    const int an = n < 0 ? -n : n, sn = n < 0;
+    const int cn0 = an == 0 ? 0 : sn, cn1 = an == 0 ? g.NS - 1 : sn;
+    int cm0[2], cm1[2];
+    double2 *row[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int m = c ? mB : mA;
+      const int am = m < 0 ? -m : m, smn = m < 0;
+      cm0[c] = am == 0 ? 0 : smn;
+      cm1[c] = am == 0 ? 1 : smn;
+      row[c] = Gp + ((int64_t)an * L + am) * C2 * L;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int mp = 4 * (l + 32 * j) + 2 * h + e;
+      const int ip = (4096 - mp) >> 1;
+      const double2 zmv = T2[(((ip >> 6) - 24) << 6) + ((ip & 1) << 5) + ((ip >> 1) & 31)];
+      if (mp >= L) continue;
+      const double2 zp = make_double2(v[j].x * sc, v[j].y * sc);
+      const double2 zm = make_double2(zmv.x * sc, zmv.y * sc);
+      double2 GA, GB;
+      if (mp > 0) {
+        GA = make_double2(zp.x + sA * zm.x, zp.y + sA * zm.y);
+        GB = make_double2(zp.y - sA * zm.y, -(zp.x - sA * zm.x));
+      } else {
+        const double pp = kPi * kPi;
+        const double2 dz = make_double2(z1.x - zm1.x, z1.y - zm1.y);
+        if (!negA) {
+          GA = zp;
+          GB = make_double2(pp * dz.x, pp * dz.y);
+        } else {
+          GA = make_double2(-pp * dz.y, pp * dz.x);
+          GB = make_double2(zp.y, -zp.x);
+        }
+      }
+      double2 corr = cmul(p0, pole[mp]);
+      if (!ODD) corr = cadd(corr, cmul(p1, pole[L + mp]));
+      if (negA) GA = cadd(GA, corr);
+      else GB = cadd(GB, corr);
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        if (!(c ? okB : okA)) continue;
+        const double2 G0 = c ? GB : GA;
+#pragma nounroll
+        for (int cn = cn0; cn <= cn1; ++cn) {
+#pragma nounroll
+          for (int cm = cm0[c]; cm <= cm1[c]; ++cm) {
+            const double sg = ((cm != cn) && (mp & 1)) ? -1.0 : 1.0;
+            row[c][(int64_t)(cm + 2 * cn) * L + mp] = make_double2(sg * G0.x, sg * G0.y);
+          }
+        }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    r32_fence();  // the buffer is read before the next group's transforms
+  }
+}
+
 // ---------------------------------------------------------------- paired beta, LDS engine
 // The paired beta stages (see above) for the other DFT lengths: a
 // workgroup transforms K pairs, 2K consecutive m from m_lo, in the LDS
@@ -2366,6 +2542,19 @@ void launch_fwd_beta(const Geo &g, const double2 *X2, double2 *Gp, NBase nb, con
       kern<<<grid, 64 * kRows32Waves, sh, st>>>(g, fb, X2, Gp, nb.base, phase, wr_even, pole);
     };
     (g.Pb & 1) ? go(k_fwd_beta_r32<1>) : go(k_fwd_beta_r32<0>);
+    SO3_LAUNCH_CHECK();
+    return;
+  }
+  if (!adjoint && wr_even && pole && Pw == 4096 && g.L >= 513 && g.L <= 1024 && g.Pb >= 1025 && g.Pb <= 2048) {
+    const FftDesc fb = fft_plan_reg(g.Pb);
+    constexpr int C = kR4kWaves;
+    const size_t sh = sizeof(double2) * kR4kTab + sizeof(double) * kR4kWaves * kR32T;
+    const int chunks = (g.nA + C - 1) / C;
+    auto go = [&](auto kern) {
+      const unsigned grid = reg_grid(kern, 64 * kR4kWaves, sh, (int64_t)g.nG * chunks);
+      kern<<<grid, 64 * kR4kWaves, sh, st>>>(g, fb, X2, Gp, nb.base, phase, wr_even, pole);
+    };
+    (g.Pb & 1) ? go(k_fwd_beta_r4k<1>) : go(k_fwd_beta_r4k<0>);
     SO3_LAUNCH_CHECK();
     return;
   }

@@ -168,6 +168,8 @@ Plan build_plan(const so3_parameters_t *p, Method method) {
   if (L <= 256 && fft_plan(g.Pb).blue && fft_plan(g.Pb).len == 1024) P.Pw = 1024;
   // the 2048-point register forward beta (L = 257..512) convolves at 2048
   if (L >= 257 && L <= 512) P.Pw = 2048;
+  // and the 4096-point one (L = 513..1024) at 4096
+  if (L >= 513 && L <= 1024) P.Pw = 4096;
   if (const int pw = tuned(p, method).pw) P.Pw = pw;  // the tuner's choice (>= 4L - 3)
   {
     const int Pw = P.Pw;
