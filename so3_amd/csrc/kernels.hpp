@@ -101,6 +101,15 @@ void launch_alpha(const Geo &g, double2 *X2, int dir, double scale, hipStream_t 
 void launch_rows(double2 *X, int64_t nrows, int n, int ld, int dir, double scale, hipStream_t st);
 void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, double scale, hipStream_t st);
 void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, double scale, hipStream_t st);
+// gamma of short lengths 2N - 1 <= kGammaShortMax as a direct DFT, one
+// column per thread (gamma_short.hip): dir = +1 X2 -> f (as
+// launch_inv_gamma), -1 f -> X2 (as launch_fwd_gamma); false when the length
+// is not short.  gamma_short_prepare makes its table ahead of a timed or
+// captured call.
+constexpr int kGammaShortMax = 31;
+bool gamma_short_applies(const Geo &g);
+void gamma_short_prepare(const Geo &g);
+bool launch_gamma_short(const Geo &g, const void *src, void *dst, int dir, double scale, hipStream_t st);
 // steerable gamma: dir = +1 X2 -> f (N samples kept), dir = -1 f -> X2
 void launch_gamma_steer(const Geo &g, const void *src, void *dst, int dir, double scale, hipStream_t st);
 // adjoint != 0: zero-fill instead of the beta extension and no weights (the

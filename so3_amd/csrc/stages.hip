@@ -2465,6 +2465,7 @@ static void launch_gamma_real(const Geo &g, const FftDesc &fd, double2 *X2, doub
 }
 
 void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, double scale, hipStream_t st) {
+  if (launch_gamma_short(g, X2, f, +1, scale, st)) return;
   const FftDesc fd = fft_plan(2 * g.N - 1);
   if (fd.len == 1024 && fd.blue && g.real) {
     launch_gamma_real(g, fd, (double2 *)X2, (double *)f, +1, scale, st);
@@ -2487,6 +2488,7 @@ void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, double scale, hi
 }
 
 void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, double scale, hipStream_t st) {
+  if (launch_gamma_short(g, f, X2, -1, scale, st)) return;
   const FftDesc fd = fft_plan(2 * g.N - 1);
   if (fd.len == 1024 && fd.blue && g.real) {
     launch_gamma_real(g, fd, X2, (double *)f, -1, scale, st);

@@ -271,6 +271,7 @@ Plan build_plan(const so3_parameters_t *p, Method method) {
   fft_plan(g.Pb);
   fft_plan(g.nA);
   fft_plan(2 * N - 1);
+  gamma_short_prepare(g);
   fft_plan(P.Pw);
   fft_plan_lds(g.Pb);
   fft_plan_lds(P.Pw);
