@@ -220,7 +220,7 @@ def batched_bench(A, P, torch, reps=5):
     for p in ps:
         key = (p.L, p.N, p.reality, p.sampling_scheme)
         if key not in shapes:
-            defaults[key] = A.tune_get(p)["pw"]
+            defaults[key] = A.tune_get(p)
             shapes[key] = A.tune(p)[0]
     tune_s = time.perf_counter() - t0
     progress("batched: autotuner over %d shapes" % len(shapes))
@@ -230,9 +230,13 @@ def batched_bench(A, P, torch, reps=5):
                        "complex/real (seed 4), via SSHT, Risbo, padded")
     out["plain_untuned_ms"] = untuned_ms
     out["autotuner"] = {"shapes": len(shapes), "seconds": tune_s,
-                        "pw_changed": sum(1 for k, c in shapes.items() if c["pw"] != defaults[k]),
+                        "pw_changed": sum(1 for k, c in shapes.items() if c["pw"] != defaults[k]["pw"]),
                         "inverse_order_1": sum(1 for c in shapes.values() if c["corder_inv"]),
-                        "forward_order_1": sum(1 for c in shapes.values() if c["corder_fwd"])}
+                        "forward_order_1": sum(1 for c in shapes.values() if c["corder_fwd"]),
+                        "gamma_register": sum(1 for c in shapes.values() if c["gamma_engine"] == 2),
+                        "gamma_workgroup": sum(1 for c in shapes.values() if c["gamma_engine"] == 1),
+                        "gamma_changed": sum(1 for k, c in shapes.items()
+                                             if c["gamma_engine"] != defaults[k]["gamma_engine"])}
     del keep
     small = [P.SO3Parameters(L=16 + (i * 7) % 48, N=1 + (i * 5) % min(16 + (i * 7) % 48, 16),
                              n_order=P.SO3_N_ORDER_ZERO_FIRST) for i in range(256)]

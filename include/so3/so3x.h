@@ -67,20 +67,24 @@ int so3x_prepare(const so3_parameters_t *parameters);
 
 /* Autotuner keyed on (L, N, reality, sampling).  so3x_tune times, on
  * `stream`, the choices the library can make for these parameters -- the
- * workgroup order of the inverse and forward contractions, and the length Pw
- * of the forward transform's weight convolution (every 2^a or 3 * 2^a from
+ * workgroup order of the inverse and forward contractions, the length Pw of
+ * the forward transform's weight convolution (every 2^a or 3 * 2^a from
  * 4L - 3 up to twice the default, which decides the FFT engine and plan that
- * run it) -- and keeps the fastest for every later transform with the same
- * (L, N, reality, sampling), single or batched.  It allocates and
- * synchronises; call it outside timed regions.  choice (3 ints, may be NULL)
- * receives (inverse order, forward order, Pw), ms (3 doubles, may be NULL)
- * the best inverse-contraction, forward-contraction and forward-beta times.
- * The contraction order never changes a result; another Pw changes only the
- * rounding (the convolution is exact for every candidate).
- * so3x_tune_get reports the choices in force; so3x_tune_save writes the
- * table as text ("L N reality mw_ss corder_inv corder_fwd pw" per line) and
- * so3x_tune_load reads one back (returns the number of entries, merged into
- * the table in force). */
+ * run it), and for gamma lengths 2N - 1 in [33, 127] the engine of the gamma
+ * DFTs (1: the workgroup engine, 2: the 256-point register FFT) -- and keeps
+ * the fastest for every later transform with the same (L, N, reality,
+ * sampling), single or batched.  It allocates and synchronises; call it
+ * outside timed regions.  choice (4 ints, may be NULL) receives (inverse
+ * order, forward order, Pw, gamma engine; 0 where the gamma engine is not a
+ * choice), ms (4 doubles, may be NULL) the best inverse-contraction,
+ * forward-contraction, forward-beta and inverse+forward gamma times (0 where
+ * not timed).  The contraction order never changes a result; another Pw or
+ * gamma engine changes only the rounding (the convolution is exact for every
+ * candidate).  so3x_tune_get reports the choices in force; so3x_tune_save
+ * writes the table as text ("L N reality mw_ss corder_inv corder_fwd pw
+ * gamma" per line, 0 for the default rule) and so3x_tune_load reads one back
+ * (lines without the gamma column keep its default rule; returns the number
+ * of entries, merged into the table in force). */
 int so3x_tune(const so3_parameters_t *parameters, int method, void *stream, int *choice, double *ms);
 int so3x_tune_get(const so3_parameters_t *parameters, int method, int *choice);
 int so3x_tune_save(const char *path);

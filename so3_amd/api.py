@@ -258,18 +258,20 @@ def prepare(p):
 
 def tune(p, method=VIA_SSHT, stream=None):
     """Run the (L, N)-keyed autotuner for p; returns ({"corder_inv", "corder_fwd",
-    "pw"}, {"contract_inv_ms", "contract_fwd_ms", "beta_fwd_ms"})."""
-    ch = (ctypes.c_int * 3)()
-    ms = (ctypes.c_double * 3)()
+    "pw", "gamma_engine"}, {"contract_inv_ms", "contract_fwd_ms", "beta_fwd_ms",
+    "gamma_ms"}).  gamma_engine: 1 workgroup, 2 256-point register FFT, 0 where
+    the gamma length 2N-1 is outside [33, 127] (gamma_ms is then 0)."""
+    ch = (ctypes.c_int * 4)()
+    ms = (ctypes.c_double * 4)()
     check(lib().so3x_tune(_s(p), method, ctypes.c_void_p(stream or 0), ch, ms))
-    return ({"corder_inv": ch[0], "corder_fwd": ch[1], "pw": ch[2]},
-            {"contract_inv_ms": ms[0], "contract_fwd_ms": ms[1], "beta_fwd_ms": ms[2]})
+    return ({"corder_inv": ch[0], "corder_fwd": ch[1], "pw": ch[2], "gamma_engine": ch[3]},
+            {"contract_inv_ms": ms[0], "contract_fwd_ms": ms[1], "beta_fwd_ms": ms[2], "gamma_ms": ms[3]})
 
 
 def tune_get(p, method=VIA_SSHT):
-    ch = (ctypes.c_int * 3)()
+    ch = (ctypes.c_int * 4)()
     check(lib().so3x_tune_get(_s(p), method, ch))
-    return {"corder_inv": ch[0], "corder_fwd": ch[1], "pw": ch[2]}
+    return {"corder_inv": ch[0], "corder_fwd": ch[1], "pw": ch[2], "gamma_engine": ch[3]}
 
 
 def tune_save(path):

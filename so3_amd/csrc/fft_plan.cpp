@@ -96,7 +96,8 @@ double2 *upload(const std::vector<cld> &v) {
 
 // mode 0: the general rule (the workgroup engine); 1: lengths in [257, 512]
 // take Bluestein with M = 1024 (the register-resident FFTs); 2: as 1, and
-// lengths in [513, 2048] take Bluestein with M = 2048 or 4096
+// lengths in [513, 2048] take Bluestein with M = 2048 or 4096; 3: lengths in
+// [33, 128] take Bluestein with M = 256
 FftDesc build(int n, int mode) {
   const bool reg1024 = mode >= 1;
   FftDesc d{};
@@ -118,7 +119,10 @@ FftDesc build(int n, int mode) {
   // which the 2048- and 4096-point register FFTs (fft_reg2k.hpp,
   // fft_reg4k.hpp) run
   const bool r2048 = mode == 2 && n >= 513 && n <= 2048;
-  bool direct = factor(n, rad) && !r1024 && !r2048;
+  // mode 3: every n in [33, 128] takes Bluestein with M = 256, which the
+  // 256-point register FFT (fft_reg16.hpp) runs
+  const bool r256 = mode == 3 && n >= 33 && n <= 128;
+  bool direct = factor(n, rad) && !r1024 && !r2048 && !r256;
   if (direct) {
     int cap = kFftThreads * kFftElems * 4;
     for (int R : rad) cap = std::min(cap, kFftThreads * fft_nb(R) * R);
@@ -132,7 +136,8 @@ FftDesc build(int n, int mode) {
     while (M < 2 * n - 1) M *= 2;
     // 3 * 2^k when that suffices and one workgroup holds it (fewer points
     // than the next power of two); n in [257, 512] keeps M = 1024
-    if (!r1024 && !r2048 && M >= 4 && 3 * (M / 4) >= 2 * n - 1 && 3 * (M / 4) <= 1536) M = 3 * (M / 4);
+    if (!r1024 && !r2048 && !r256 && M >= 4 && 3 * (M / 4) >= 2 * n - 1 && 3 * (M / 4) <= 1536) M = 3 * (M / 4);
+    if (r256) M = 256;
     d.blue = 1;
     d.len = M;
     factor(M, rad);
@@ -216,6 +221,11 @@ static FftDesc plan_cached(int n, int mode) {
 FftDesc fft_plan(int n) { return plan_cached(n, 1); }
 
 FftDesc fft_plan_lds(int n) { return n >= 257 && n <= 512 ? plan_cached(n, 0) : plan_cached(n, 1); }
+
+FftDesc fft_plan_r256(int n) {
+  if (n < 33 || n > 128) throw HipError("fft_plan_r256: length outside [33, 128]");
+  return plan_cached(n, 3);
+}
 
 FftDesc fft_plan_reg(int n) {
   if (n < 513 || n > 2048) throw HipError("fft_plan_reg: length outside [513, 2048]");

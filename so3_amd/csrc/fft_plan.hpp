@@ -22,6 +22,10 @@ FftDesc fft_plan_lds(int n);
 // M = 4096 for every n in [1025, 2048].
 FftDesc fft_plan_reg(int n);
 
+// The plan of the 256-point register FFT (fft_reg16.hpp): Bluestein with
+// M = 256 for every n in [33, 128].
+FftDesc fft_plan_r256(int n);
+
 // Largest S * len one workgroup can run through every pass of d (each
 // thread holds fft_nb(R) butterflies of radix R).
 int fft_capacity(const FftDesc &d);

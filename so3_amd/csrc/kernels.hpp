@@ -25,6 +25,8 @@ struct Geo {
                    // (synthesis), 2 in the forward one (adjoint of the
                    // inverse), 0 nowhere (forward, adjoint of the forward)
   int corder;      // contraction workgroup order (0 or 1, see contract.hip)
+  int geng;        // gamma engine for 2N-1 in [33, 127]: 0 the default rule,
+                   // 1 the workgroup engine, 2 the 256-point register FFT
 };
 
 // Per-n data: base[n + N - 1] = storage index of (l=0, m=0, n) in the flmn
@@ -110,6 +112,10 @@ constexpr int kGammaShortMax = 31;
 bool gamma_short_applies(const Geo &g);
 void gamma_short_prepare(const Geo &g);
 bool launch_gamma_short(const Geo &g, const void *src, void *dst, int dir, double scale, hipStream_t st);
+// the engine launch_inv_gamma / launch_fwd_gamma run for g when 2N-1 is in
+// [33, 127] (1 workgroup, 2 256-point register FFT; g.geng overrides the
+// default rule), else 0
+int gamma_engine(const Geo &g);
 // steerable gamma: dir = +1 X2 -> f (N samples kept), dir = -1 f -> X2
 void launch_gamma_steer(const Geo &g, const void *src, void *dst, int dir, double scale, hipStream_t st);
 // adjoint != 0: zero-fill instead of the beta extension and no weights (the

@@ -22,6 +22,7 @@
 
 #include "fft_plan.hpp"
 #include "fft_reg.hpp"
+#include "fft_reg16.hpp"
 #include "fft_reg2k.hpp"
 #include "fft_reg4k.hpp"
 #include "fft_reg32.hpp"
@@ -493,6 +494,135 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
     for (int q = 0; q < 16; ++q) {
       const int k = l + 32 * q;
       if (q < 8 || k < nG) col[k * xs] = cscale(v[q], scale);  // nG > 256
+    }
+  }
+}
+
+// ---------------------------------------------------------------- gamma, 256-point register FFT
+// Gamma of a complex signal between X2 and f for the Bluestein lengths
+// nG = 2N - 1 in [33, 127] on the 256-point register FFT (fft_reg16.hpp):
+// wavefront w of group t takes the four columns p = 4 (t W + w) + h, one
+// per sequence h, and reads and writes their nG slots straight from X2 and
+// f.  An access instruction then moves 16 runs of 64 bytes (four
+// consecutive columns of 16 rows); the neighbouring wavefront uses the other
+// half of each 128-byte line, and one workgroup barrier per group keeps the
+// wavefronts of a workgroup in step so that the line is still in L2 for it.
+// DIR = +1: inverse (X2 -> f), -1: forward (f -> X2).
+constexpr int kR16Waves = 4;  // one wavefront per SIMD and workgroup, three workgroups per CU
+
+template <int DIR>
+__global__ __launch_bounds__(64 * kR16Waves) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_gamma_r16(
+    Geo g, FftDesc fd, const double2 *__restrict__ src, double2 *__restrict__ dst, double scale) {
+  extern __shared__ double2 lds[];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  double2 *tab = lds;
+  double *T = (double *)(lds + kR16Tab) + wv * kR16T;
+  r16_stage_blue<DIR>(tab, fd);
+  __syncthreads();  // tables staged
+  const int nG = 2 * g.N - 1;
+  const int S = g.nB * g.nA;
+  const int groups = (S + 4 * kR16Waves - 1) / (4 * kR16Waves);
+  for (int t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
+    __syncthreads();  // every wavefront passes once per group
+    const int lane = loop_lane(), l = lane & 15, h = lane >> 4;
+    const int p = 4 * (t * kR16Waves + wv) + h;
+    const bool ok = p < S;
+    const int pc = ok ? p : S - 1;
+    const int b = pc / g.nA, a = pc - b * g.nA;
+    const int64_t xb = x2_row(g, 0, b) * g.ldA + a, xs = (int64_t)x2_w(g, b) * g.ldA;
+    double2 v[16];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int j = min(l + 16 * q, nG - 1);  // entries j >= nG meet a zero chirp factor
+      v[q] = DIR > 0 ? src[xb + j * xs] : src[(int64_t)j * S + pc];
+    }
+    double2 *const o = DIR > 0 ? dst + pc : dst + xb;
+    const int64_t os = DIR > 0 ? (int64_t)S : xs;
+    blue16(v, tab, T, lane, [&](int q, double2 z) {
+      const int k = l + 16 * q;
+      if (ok && k < nG) o[k * os] = cscale(z, scale);
+    });
+  }
+}
+
+// Gamma of a real signal between X2 (n slots 0..N-1) and f (nf = 2N - 1 real
+// samples per column, nf in [33, 127]) on the 256-point register FFT:
+// sequence h of wavefront w in group t takes the columns
+// pA = 8 (t W + w) + 2 h and pB = pA + 1 as one complex DFT (the algebra of
+// k_gamma_real_r32):
+//   inverse (c2r): the Hermitian extension of X_A + i X_B (slot 0 real,
+//     slots k >= N from conj(slot nf - k)); f_A, f_B are the real and
+//     imaginary parts of the DFT;
+//   forward (r2c): the DFT Y of f_A + i f_B separates into
+//     X_A = (Y(k) + conj Y(-k)) / 2, X_B = (Y(k) - conj Y(-k)) / (2 i), k < N,
+//     Y(-k) through the wavefront's transpose buffer.
+template <int DIR>
+__global__ __launch_bounds__(64 * kR16Waves) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_gamma_real_r16(
+    Geo g, FftDesc fd, double2 *X2, double *f, double scale) {
+  extern __shared__ double2 lds[];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  double2 *tab = lds;
+  double *T = (double *)(lds + kR16Tab) + wv * kR16T;
+  double2 *T2 = (double2 *)T;
+  r16_stage_blue<DIR>(tab, fd);
+  __syncthreads();  // tables staged
+  const int S = g.nB * g.nA, N = g.N, nf = 2 * g.N - 1;
+  const int groups = (S + 8 * kR16Waves - 1) / (8 * kR16Waves);
+  for (int t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
+    __syncthreads();  // every wavefront passes once per group
+    const int lane = loop_lane(), l = lane & 15, h = lane >> 4;
+    const int pA = 8 * (t * kR16Waves + wv) + 2 * h, pB = pA + 1;
+    const bool okA = pA < S, okB = pB < S;  // a column past the last one repeats it, unstored
+    const int qA = okA ? pA : S - 1, qB = okB ? pB : S - 1;
+    const int bA = qA / g.nA, aA = qA - bA * g.nA, bB = qB / g.nA, aB = qB - bB * g.nA;
+    double2 *cA = X2 + x2_row(g, 0, bA) * g.ldA + aA, *cB = X2 + x2_row(g, 0, bB) * g.ldA + aB;
+    const int64_t sA = (int64_t)x2_w(g, bA) * g.ldA, sB = (int64_t)x2_w(g, bB) * g.ldA;
+    double2 v[16];
+    if (DIR > 0) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int k = min(l + 16 * q, nf - 1);  // past nf: cleared by the chirp
+        const bool dir = k < N;
+        const int kk = dir ? k : nf - k;
+        double2 a = cA[kk * sA], b = cB[kk * sB];
+        if (!dir) a.y = -a.y, b.y = -b.y;
+        if (k == 0) a.y = 0.0, b.y = 0.0;  // slot 0 of a real signal is real
+        v[q] = make_double2(a.x - b.y, a.y + b.x);
+      }
+      blue16(v, tab, T, lane, [&](int q, double2 z) {
+        const int k = l + 16 * q;
+        if (k >= nf) return;
+        if (okA) f[(int64_t)k * S + pA] = z.x * scale;
+        if (okB) f[(int64_t)k * S + pB] = z.y * scale;
+      });
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int k = min(l + 16 * q, nf - 1);
+        v[q] = make_double2(f[(int64_t)k * S + qA], f[(int64_t)k * S + qB]);
+      }
+      blue16(v, tab, T, lane, [&](int q, double2 z) { v[q] = z; });
+      // Y(-k) = Y(nf - k) (Y(0) for k = 0) through the buffer: entry c of
+      // the sequence sits at (c >> 4) 64 + 16 h + (c & 15)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) T2[q * 64 + lane] = v[q];
+      __builtin_amdgcn_wave_barrier();
+      r32_fence();
+      const double hs = 0.5 * scale;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int k = l + 16 * j;
+        const int c = k == 0 ? 0 : max(nf - k, 0);
+        const double2 y = T2[(c >> 4) * 64 + 16 * h + (c & 15)];
+        if (k >= N) continue;
+        // X_A = (Y + conj Y(-k)) / 2, X_B = (Y - conj Y(-k)) / (2 i)
+        const double2 xa = make_double2(hs * (v[j].x + y.x), hs * (v[j].y - y.y));
+        const double2 xb = make_double2(hs * (v[j].y + y.y), -hs * (v[j].x - y.x));
+        if (okA) cA[k * sA] = xa;
+        if (okB) cB[k * sB] = xb;
+      }
+      __builtin_amdgcn_wave_barrier();
+      r32_fence();  // the buffer is read before the next group's transposes
     }
   }
 }
@@ -2464,8 +2594,45 @@ static void launch_gamma_real(const Geo &g, const FftDesc &fd, double2 *X2, doub
   SO3_LAUNCH_CHECK();
 }
 
+// The default rule for nG = 2N - 1 in [33, 127]: the register FFT for the
+// lengths the workgroup engine runs through Bluestein (2-3x faster there),
+// the workgroup engine for the smooth ones it runs directly (up to 2x faster
+// than Bluestein on the register FFT at L = 256).
+int gamma_engine(const Geo &g) {
+  const int nG = 2 * g.N - 1;
+  if (g.steer || nG < 33 || nG > 127 || (int64_t)g.nB * g.nA >= ((int64_t)1 << 30)) return 0;
+  if (g.geng == 1 || g.geng == 2) return g.geng;
+  return fft_plan(nG).blue ? 2 : 1;
+}
+
+// gamma for nG = 2N - 1 in [33, 127] on the 256-point register FFT
+// (dir = +1: X2 -> f, -1: f -> X2); false when gamma_engine(g) is not it
+static bool launch_gamma_r16(const Geo &g, double2 *X2, void *f, int dir, double scale, hipStream_t st) {
+  if (gamma_engine(g) != 2) return false;
+  const int nG = 2 * g.N - 1;
+  const FftDesc fd = fft_plan_r256(nG);
+  const size_t sh = sizeof(double2) * kR16Tab + sizeof(double) * kR16Waves * kR16T;
+  const int64_t S = (int64_t)g.nB * g.nA;
+  if (g.real) {
+    auto go = [&](auto kern) {
+      const unsigned grid = reg_grid(kern, 64 * kR16Waves, sh, (S + 8 * kR16Waves - 1) / (8 * kR16Waves));
+      kern<<<grid, 64 * kR16Waves, sh, st>>>(g, fd, X2, (double *)f, scale);
+    };
+    dir > 0 ? go(k_gamma_real_r16<+1>) : go(k_gamma_real_r16<-1>);
+  } else {
+    auto go = [&](auto kern, const double2 *src, double2 *dst) {
+      const unsigned grid = reg_grid(kern, 64 * kR16Waves, sh, (S + 4 * kR16Waves - 1) / (4 * kR16Waves));
+      kern<<<grid, 64 * kR16Waves, sh, st>>>(g, fd, src, dst, scale);
+    };
+    dir > 0 ? go(k_gamma_r16<+1>, X2, (double2 *)f) : go(k_gamma_r16<-1>, (const double2 *)f, X2);
+  }
+  SO3_LAUNCH_CHECK();
+  return true;
+}
+
 void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, double scale, hipStream_t st) {
   if (launch_gamma_short(g, X2, f, +1, scale, st)) return;
+  if (launch_gamma_r16(g, (double2 *)X2, f, +1, scale, st)) return;
   const FftDesc fd = fft_plan(2 * g.N - 1);
   if (fd.len == 1024 && fd.blue && g.real) {
     launch_gamma_real(g, fd, (double2 *)X2, (double *)f, +1, scale, st);
@@ -2489,6 +2656,7 @@ void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, double scale, hi
 
 void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, double scale, hipStream_t st) {
   if (launch_gamma_short(g, f, X2, -1, scale, st)) return;
+  if (launch_gamma_r16(g, X2, (void *)f, -1, scale, st)) return;
   const FftDesc fd = fft_plan(2 * g.N - 1);
   if (fd.len == 1024 && fd.blue && g.real) {
     launch_gamma_real(g, fd, X2, (double *)f, -1, scale, st);

@@ -98,6 +98,7 @@ Geo make_geo(const so3_parameters_t *p, Method method) {
 struct TuneVal {
   int corder_inv = 0, corder_fwd = 0;  // contraction workgroup orders
   int pw = 0;                          // weight-convolution length (0: the default rule)
+  int geng = 0;                        // gamma engine (Geo::geng; 0: the default rule)
 };
 using TuneKey = std::tuple<int, int, int, int>;
 std::map<TuneKey, TuneVal> g_tune;
@@ -112,13 +113,13 @@ TuneVal tuned(const so3_parameters_t *p, Method method) {
   return it == g_tune.end() ? TuneVal{} : it->second;
 }
 
-using Key = std::tuple<int, int, int, int, int, int, int, int, int, int, int, int>;
+using Key = std::tuple<int, int, int, int, int, int, int, int, int, int, int, int, int>;
 
 Key key_of(const so3_parameters_t *p, Method method, int dev) {
   const int ss = (method == kViaSsht && p->sampling_scheme == SO3_SAMPLING_MW_SS);
   const int steer = (method == kViaSsht && p->steerable) ? 1 : 0;
   return Key(dev, p->L, p->N, p->L0, ss, p->reality ? 1 : 0, (int)p->storage, (int)p->n_order,
-             (int)p->n_mode, (int)method, steer, tuned(p, method).pw);
+             (int)p->n_mode, (int)method, steer, tuned(p, method).pw, tuned(p, method).geng);
 }
 
 Plan build_plan(const so3_parameters_t *p, Method method) {
@@ -171,6 +172,7 @@ Plan build_plan(const so3_parameters_t *p, Method method) {
   // and the 4096-point one (L = 513..1024) at 4096
   if (L >= 513 && L <= 1024) P.Pw = 4096;
   if (const int pw = tuned(p, method).pw) P.Pw = pw;  // the tuner's choice (>= 4L - 3)
+  g.geng = tuned(p, method).geng;
   {
     const int Pw = P.Pw;
     std::vector<cld> w(Pw, cld(0, 0));
@@ -272,6 +274,7 @@ Plan build_plan(const so3_parameters_t *p, Method method) {
   fft_plan(g.nA);
   fft_plan(2 * N - 1);
   gamma_short_prepare(g);
+  if (2 * N - 1 >= 33 && 2 * N - 1 <= 127) fft_plan_r256(2 * N - 1);
   fft_plan(P.Pw);
   fft_plan_lds(g.Pb);
   fft_plan_lds(P.Pw);
@@ -600,11 +603,14 @@ void run_inverse_noalloc(const so3_parameters_t *p, Method method, const void *f
 //     Pw: the default rule and every other 2^a or 3 * 2^a in [4L - 3, 4096]
 //     up to twice the default that the workgroup FFT engine runs directly
 //     without Bluestein (any Pw >= 4L - 3 gives the exact linear
-//     convolution; the length decides which FFT engine and plan run it).
+//     convolution; the length decides which FFT engine and plan run it);
+//   * for gamma lengths 2N - 1 in [33, 127], the inverse and forward gamma
+//     stages on the workgroup engine (its plan: direct for a smooth length,
+//     else Bluestein) and on the 256-point register FFT (Bluestein).
 // The fastest of each is kept and applies to every later transform with
 // these parameters, alone or batched (so batched results stay bit-identical
 // to single ones).  The workgroup order never changes a result; another Pw
-// changes the rounding only.
+// or gamma engine changes the rounding only.
 namespace {
 
 // the default, then the other lengths the workgroup engine runs directly
@@ -649,6 +655,7 @@ void tune(const so3_parameters_t *p, Method method, hipStream_t st, int *out, do
   hipEvent_t a = nullptr, b = nullptr;
   TuneVal best;
   double ms[4] = {0, 0, 0, 0};
+  double2 *fbuf = nullptr;
   try {
     SO3_HIP(hipMalloc(&flmn, sizeof(double2) * nl));
     SO3_HIP(hipMalloc(&ws, sizeof(double2) * (P0.x1 + P0.x2)));
@@ -678,9 +685,27 @@ void tune(const so3_parameters_t *p, Method method, hipStream_t st, int *out, do
       const float tw = best_ms(st, a, b, [&] {
         launch_fwd_beta(Pc.g, X2, X1, nb, Pc.phase_fwd, Pc.wr, Pc.wr_even, Pc.pole, Pc.Pw, 0, st);
       });
-      if (tw < bw) bw = tw, best.pw = t.pw, ms[3] = pw;
+      if (tw < bw) bw = tw, best.pw = t.pw;
     }
     ms[2] = bw;
+    if (gamma_engine(P0.g) != 0) {
+      const size_t nf = (size_t)so3_sampling_f_size(p);
+      SO3_HIP(hipMalloc(&fbuf, sizeof(double2) * nf));
+      SO3_HIP(hipMemsetAsync(fbuf, 0, sizeof(double2) * nf, st));
+      float bg = 1e30f;
+      for (int e : {1, 2}) {
+        TuneVal t = best;
+        t.geng = e;
+        g_tune[tk] = t;
+        const Plan &Pc = get_plan(p, method);
+        const float tg = best_ms(st, a, b, [&] {
+          launch_inv_gamma(Pc.g, X2, fbuf, 1.0, st);
+          launch_fwd_gamma(Pc.g, fbuf, X2, 1.0, st);
+        });
+        if (tg < bg) bg = tg, best.geng = e;
+      }
+      ms[3] = bg;
+    }
   } catch (...) {
     g_tune.erase(tk);
     if (a) (void)hipEventDestroy(a);
@@ -688,6 +713,7 @@ void tune(const so3_parameters_t *p, Method method, hipStream_t st, int *out, do
     (void)hipStreamSynchronize(st);
     (void)hipFree(flmn);
     (void)hipFree(ws);
+    (void)hipFree(fbuf);
     throw;
   }
   g_tune[tk] = best;
@@ -696,13 +722,15 @@ void tune(const so3_parameters_t *p, Method method, hipStream_t st, int *out, do
   (void)hipEventDestroy(b);
   SO3_HIP(hipFree(flmn));
   SO3_HIP(hipFree(ws));
+  SO3_HIP(hipFree(fbuf));
   if (out) {
     out[0] = best.corder_inv;
     out[1] = best.corder_fwd;
     out[2] = best.pw ? best.pw : P0.Pw;
+    out[3] = gamma_engine(get_plan(p, method).g);
   }
   if (ms_out)
-    for (int i = 0; i < 3; ++i) ms_out[i] = ms[i];
+    for (int i = 0; i < 4; ++i) ms_out[i] = ms[i];
 }
 
 void tune_clear() {
@@ -717,17 +745,21 @@ void tune_get(const so3_parameters_t *p, Method method, int *out) {
   out[0] = t.corder_inv;
   out[1] = t.corder_fwd;
   out[2] = t.pw ? t.pw : get_plan(p, method).Pw;
+  out[3] = gamma_engine(get_plan(p, method).g);
 }
 
-// one line per entry: "L N reality mw_ss corder_inv corder_fwd pw" (pw 0: default)
+// one line per entry: "L N reality mw_ss corder_inv corder_fwd pw gamma" (pw,
+// gamma 0: the default rule); tune_load also reads lines without the gamma
+// column
 void tune_save(const char *path) {
   std::lock_guard<std::mutex> lk(g_mu);
   FILE *fp = fopen(path, "w");
   if (!fp) throw HipError(std::string("cannot write ") + path);
-  fprintf(fp, "# so3 MI355X tuning table: L N reality mw_ss corder_inv corder_fwd pw\n");
+  fprintf(fp, "# so3 MI355X tuning table: L N reality mw_ss corder_inv corder_fwd pw gamma\n");
   for (auto &kv : g_tune)
-    fprintf(fp, "%d %d %d %d %d %d %d\n", std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first),
-            std::get<3>(kv.first), kv.second.corder_inv, kv.second.corder_fwd, kv.second.pw);
+    fprintf(fp, "%d %d %d %d %d %d %d %d\n", std::get<0>(kv.first), std::get<1>(kv.first),
+            std::get<2>(kv.first), std::get<3>(kv.first), kv.second.corder_inv, kv.second.corder_fwd,
+            kv.second.pw, kv.second.geng);
   if (fclose(fp) != 0) throw HipError(std::string("cannot write ") + path);
 }
 
@@ -739,9 +771,10 @@ int tune_load(const char *path) {
   int n = 0;
   while (fgets(line, sizeof line, fp)) {
     if (line[0] == '#') continue;
-    int L, N, r, ss, ci, cf, pw;
-    if (sscanf(line, "%d %d %d %d %d %d %d", &L, &N, &r, &ss, &ci, &cf, &pw) != 7) continue;
-    if (L < 1 || N < 1 || (ci != 0 && ci != 1) || (cf != 0 && cf != 1) ||
+    int L, N, r, ss, ci, cf, pw, ge = 0;
+    const int got = sscanf(line, "%d %d %d %d %d %d %d %d", &L, &N, &r, &ss, &ci, &cf, &pw, &ge);
+    if (got != 7 && got != 8) continue;
+    if (L < 1 || N < 1 || (ci != 0 && ci != 1) || (cf != 0 && cf != 1) || ge < 0 || ge > 2 ||
         (pw != 0 && (pw < 4 * L - 3 || pw > 4096 || (pw & (pw - 1) && (pw % 3 || (pw / 3) & (pw / 3 - 1)))))) {
       fclose(fp);
       throw HipError(std::string("invalid tuning entry in ") + path + ": " + line);
@@ -750,6 +783,7 @@ int tune_load(const char *path) {
     t.corder_inv = ci;
     t.corder_fwd = cf;
     t.pw = pw;
+    t.geng = ge;
     g_tune[TuneKey(L, N, r, ss)] = t;
     ++n;
   }

@@ -50,7 +50,11 @@ def test_tuned_transform_matches_oracle(oracle, L, N, real, ss):
     choice, ms = A.tune(p)
     assert choice["corder_inv"] in (0, 1) and choice["corder_fwd"] in (0, 1)
     assert 4 * L - 3 <= choice["pw"] <= 4096
-    assert all(v > 0 for v in ms.values())
+    # the gamma engine is a choice for gamma lengths 2N-1 in [33, 127] only
+    assert choice["gamma_engine"] == (0 if 2 * N - 1 < 33 else choice["gamma_engine"])
+    assert choice["gamma_engine"] in ((1, 2) if 33 <= 2 * N - 1 <= 127 else (0,))
+    assert (ms["gamma_ms"] > 0) == (choice["gamma_engine"] != 0)
+    assert all(v > 0 for k, v in ms.items() if k != "gamma_ms")
     assert A.tune_get(p) == choice
     flmn = synth.gen_flmn(p, 8)
     f = A.inverse(flmn, p)
