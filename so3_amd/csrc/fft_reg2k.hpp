@@ -13,9 +13,9 @@
 //   A[k] = Z[k] + Z[k + 1024],  B[k] = (Z[k] - Z[k + 1024]) conj(w)^k
 //   y[2 j] = DFT1024^+(A)[j],  y[2 j + 1] = DFT1024^+(B)[j].
 // The butterflies pair register q of lane (0, l) with register q of lane
-// (1, l).  They run in two rounds of 16 registers through the wavefront's
-// transpose buffer: each lane writes its 16 values and reads its partner's
-// (lane ^ 32).  Each lane then computes
+// (1, l).  They run in four rounds of kR2kRound = 8 registers through the
+// wavefront's transpose buffer: each lane writes its 8 values and reads its
+// partner's (lane ^ 32).  (Rounds of 16 kept 64 more VGPRs live and spilled.)  Each lane then computes
 //   forward:  sigma own' + partner',  own' = own w^(h k)      (sigma = 1 - 2 h)
 //   inverse:  (sigma own + partner) conj(w)^(h k)
 // which gives X[k] / A[k] in the lower half and X[k + 1024] / B[k] in the
@@ -40,6 +40,7 @@ constexpr int kR2kHi = kR32Tw + 32;
 constexpr int kR2kCh = kR32Tw + 64;
 constexpr int kR2kChN = 513;
 constexpr int kR2kTab = kR2kCh + kR2kChN;
+constexpr int kR2kRound = 8;  // registers per round of the radix-2 exchange
 
 // copies the LDS tables of a Bluestein descriptor with inner length 2048 (the
 // chirp unconjugated; the caller conjugates for DIR > 0); the caller
@@ -79,16 +80,16 @@ __device__ __forceinline__ double2 r2k_tw(const double2 *tab, int h, int l, int 
   return cmul(lo, hi);
 }
 
-// one round of the butterfly exchange over registers q0 .. q0 + 15: the
-// partner's values arrive in p[]
-__device__ __forceinline__ void r2k_swap16(const double2 *v, double2 *p, double *T, int lane) {
+// one round of the butterfly exchange over registers q0 .. q0 + kR2kRound - 1:
+// the partner's values arrive in p[]
+__device__ __forceinline__ void r2k_swap(const double2 *v, double2 *p, double *T, int lane) {
   double2 *T2 = (double2 *)T;
 #pragma unroll
-  for (int i = 0; i < 16; ++i) T2[i * 64 + lane] = v[i];
+  for (int i = 0; i < kR2kRound; ++i) T2[i * 64 + lane] = v[i];
   __builtin_amdgcn_wave_barrier();
   r32_fence();
 #pragma unroll
-  for (int i = 0; i < 16; ++i) p[i] = T2[i * 64 + (lane ^ 32)];
+  for (int i = 0; i < kR2kRound; ++i) p[i] = T2[i * 64 + (lane ^ 32)];
   __builtin_amdgcn_wave_barrier();
   r32_fence();
 }
@@ -100,22 +101,21 @@ __device__ __forceinline__ void fft2k_fwd(double2 *v, double *T, const double2 *
   const int h = lane >> 5, l = lane & 31;
   const double sg = h ? -1.0 : 1.0;
   fft32<-1>(v, T, tab, lane, NoPost());
+  constexpr int R = kR2kRound;
 #pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    double2 p[16];
+  for (int r = 0; r < 32 / R; ++r) {
+    double2 p[R];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      v[16 * r + i] = cmul(v[16 * r + i], r2k_tw(tab, h, l, 16 * r + i));
-      if ((i & 7) == 7) r32_fence();  // table reads issued 8 at a time
-    }
-    r2k_swap16(v + 16 * r, p, T, lane);
+    for (int i = 0; i < R; ++i) v[R * r + i] = cmul(v[R * r + i], r2k_tw(tab, h, l, R * r + i));
+    r32_fence();  // table reads issued one round at a time
+    r2k_swap(v + R * r, p, T, lane);
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      double2 &z = v[16 * r + i];
+    for (int i = 0; i < R; ++i) {
+      double2 &z = v[R * r + i];
       z = make_double2(sg * z.x + p[i].x, sg * z.y + p[i].y);
-      post(16 * r + i, z);
-      if ((i & 7) == 7) r32_fence();
+      post(R * r + i, z);
     }
+    r32_fence();
   }
 }
 
@@ -126,16 +126,17 @@ template <bool HALF, class Post>
 __device__ __forceinline__ void fft2k_inv(double2 *v, double *T, const double2 *tab, int lane, Post post) {
   const int h = lane >> 5, l = lane & 31;
   const double sg = h ? -1.0 : 1.0;
+  constexpr int R = kR2kRound;
 #pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    double2 p[16];
-    r2k_swap16(v + 16 * r, p, T, lane);
+  for (int r = 0; r < 32 / R; ++r) {
+    double2 p[R];
+    r2k_swap(v + R * r, p, T, lane);
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      double2 &z = v[16 * r + i];
-      z = cmulc(make_double2(sg * z.x + p[i].x, sg * z.y + p[i].y), r2k_tw(tab, h, l, 16 * r + i));
-      if ((i & 7) == 7) r32_fence();
+    for (int i = 0; i < R; ++i) {
+      double2 &z = v[R * r + i];
+      z = cmulc(make_double2(sg * z.x + p[i].x, sg * z.y + p[i].y), r2k_tw(tab, h, l, R * r + i));
     }
+    r32_fence();
   }
   fft32<+1>(v, T, tab, lane, [&](int k, double2 &z) {
     if (!HALF || k < 16) post(k, z);
@@ -149,9 +150,12 @@ __device__ __forceinline__ void fft2k_inv(double2 *v, double *T, const double2 *
 // exit.  bh: the kernel spectrum of sign DIR (2048 entries, global memory).
 // The result is twice the DFT (the chirp tables carry 1/1024 of the 1/2048
 // of the inverse FFT): callers scale by 1/2.
-template <int DIR>
-__device__ __forceinline__ void blue2k(double2 *v, int n, const double2 *tab, const double2 *__restrict__ bh,
-                                       double *T, int lane) {
+// out(q, X) (blue2k_out) receives output register q < 16 as soon as it is
+// final instead (the registers are not written back: for callers that store
+// straight to memory, so that the outputs need not stay live).
+template <int DIR, bool OUT, class Out>
+__device__ __forceinline__ void blue2k_impl(double2 *v, int n, const double2 *tab, const double2 *__restrict__ bh,
+                                            double *T, int lane, Out out) {
   const int h = lane >> 5, l = lane & 31;
   const double2 *ch = tab + kR2kCh;
 #pragma unroll
@@ -163,8 +167,20 @@ __device__ __forceinline__ void blue2k(double2 *v, int n, const double2 *tab, co
   for (int q = 16; q < 32; ++q) v[q] = make_double2(0.0, 0.0);
   fft2k_fwd(v, T, tab, lane, [&](int q, double2 &z) { z = cmul(z, bh[l + 32 * q + 1024 * h]); });
   fft2k_inv<true>(v, T, tab, lane, [&](int q, double2 &z) {
-    z = cmul(z, r2k_chirp<DIR>(ch, n, 2 * (l + 32 * q) + h, q));
+    const double2 c = r2k_chirp<DIR>(ch, n, 2 * (l + 32 * q) + h, q);
+    if (OUT) out(q, cmul(z, c));
+    else z = cmul(z, c);
   });
+}
+template <int DIR>
+__device__ __forceinline__ void blue2k(double2 *v, int n, const double2 *tab, const double2 *__restrict__ bh,
+                                       double *T, int lane) {
+  blue2k_impl<DIR, false>(v, n, tab, bh, T, lane, [](int, double2) {});
+}
+template <int DIR, class Out>
+__device__ __forceinline__ void blue2k_out(double2 *v, int n, const double2 *tab, const double2 *__restrict__ bh,
+                                           double *T, int lane, Out out) {
+  blue2k_impl<DIR, true>(v, n, tab, bh, T, lane, out);
 }
 
 }  // namespace so3g

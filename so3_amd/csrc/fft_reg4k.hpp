@@ -13,10 +13,11 @@
 //   A[k] = Z[k] + Z[k + 2048] (wavefront 0),  B[k] = (Z[k] - Z[k + 2048]) conj(v)^k (1),
 //   y[2 i] = DFT2048^+(A)[i],  y[2 i + 1] = DFT2048^+(B)[i].
 // The butterflies pair a register of wavefront 0 with the same register and
-// lane of wavefront 1; they run in two rounds of 16 registers: each
-// wavefront writes its values to its own transpose buffer, the workgroup
+// lane of wavefront 1; they run in four rounds of kR2kRound = 8 registers:
+// each wavefront writes its values to its own transpose buffer, the workgroup
 // synchronises, and each reads its partner's (so every wavefront of the
-// workgroup must run them together).  As in fft_reg2k.hpp each lane then
+// workgroup must run them together).  (Rounds of 16 kept 64 more VGPRs live
+// and spilled.)  As in fft_reg2k.hpp each lane then
 // computes sigma own' + partner' (forward, own' = own v^(e k)) or
 // (sigma own + partner) conj(v)^(e k) (inverse), sigma = 1 - 2 e.
 //
@@ -53,17 +54,18 @@ __device__ __forceinline__ double2 r4k_tw(const double2 *tab, int e, int h, int 
   return cmul(lo, hi);
 }
 
-// one round of the cross-wavefront exchange over registers q0 .. q0 + 15:
-// the partner wavefront's values arrive in p[].  Tp: the partner's buffer.
-// Every wavefront of the workgroup calls it together (two barriers).
-__device__ __forceinline__ void r4k_swap16(const double2 *v, double2 *p, double *T, const double *Tp, int lane) {
+// one round of the cross-wavefront exchange over registers q0 .. q0 +
+// kR2kRound - 1: the partner wavefront's values arrive in p[].  Tp: the
+// partner's buffer.  Every wavefront of the workgroup calls it together (two
+// barriers).
+__device__ __forceinline__ void r4k_swap(const double2 *v, double2 *p, double *T, const double *Tp, int lane) {
   double2 *T2 = (double2 *)T;
   const double2 *P2 = (const double2 *)Tp;
 #pragma unroll
-  for (int i = 0; i < 16; ++i) T2[i * 64 + lane] = v[i];
+  for (int i = 0; i < kR2kRound; ++i) T2[i * 64 + lane] = v[i];
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < 16; ++i) p[i] = P2[i * 64 + lane];
+  for (int i = 0; i < kR2kRound; ++i) p[i] = P2[i * 64 + lane];
   __syncthreads();
 }
 
@@ -75,22 +77,21 @@ __device__ __forceinline__ void fft4k_fwd(double2 *v, double *T, const double *T
   const int h = lane >> 5, l = lane & 31;
   const double sg = e ? -1.0 : 1.0;
   fft2k_fwd(v, T, tab, lane, NoPost());
+  constexpr int R = kR2kRound;
 #pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    double2 p[16];
+  for (int r = 0; r < 32 / R; ++r) {
+    double2 p[R];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      v[16 * r + i] = cmul(v[16 * r + i], r4k_tw(tab, e, h, l, 16 * r + i));
-      if ((i & 7) == 7) r32_fence();
-    }
-    r4k_swap16(v + 16 * r, p, T, Tp, lane);
+    for (int i = 0; i < R; ++i) v[R * r + i] = cmul(v[R * r + i], r4k_tw(tab, e, h, l, R * r + i));
+    r32_fence();
+    r4k_swap(v + R * r, p, T, Tp, lane);
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      double2 &z = v[16 * r + i];
+    for (int i = 0; i < R; ++i) {
+      double2 &z = v[R * r + i];
       z = make_double2(sg * z.x + p[i].x, sg * z.y + p[i].y);
-      post(16 * r + i, z);
-      if ((i & 7) == 7) r32_fence();
+      post(R * r + i, z);
     }
+    r32_fence();
   }
 }
 
@@ -101,16 +102,17 @@ __device__ __forceinline__ void fft4k_inv(double2 *v, double *T, const double *T
                                           int lane, Post post) {
   const int h = lane >> 5, l = lane & 31;
   const double sg = e ? -1.0 : 1.0;
+  constexpr int R = kR2kRound;
 #pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    double2 p[16];
-    r4k_swap16(v + 16 * r, p, T, Tp, lane);
+  for (int r = 0; r < 32 / R; ++r) {
+    double2 p[R];
+    r4k_swap(v + R * r, p, T, Tp, lane);
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-      double2 &z = v[16 * r + i];
-      z = cmulc(make_double2(sg * z.x + p[i].x, sg * z.y + p[i].y), r4k_tw(tab, e, h, l, 16 * r + i));
-      if ((i & 7) == 7) r32_fence();
+    for (int i = 0; i < R; ++i) {
+      double2 &z = v[R * r + i];
+      z = cmulc(make_double2(sg * z.x + p[i].x, sg * z.y + p[i].y), r4k_tw(tab, e, h, l, R * r + i));
     }
+    r32_fence();
   }
   fft2k_inv<HALF>(v, T, tab, lane, post);
 }
@@ -132,9 +134,12 @@ __device__ __forceinline__ double2 r4k_chirp(const double2 *__restrict__ chirp, 
 // entry and are zero on exit.  d: the descriptor (chirp and kernel spectrum
 // of sign DIR in global memory).  The result is exactly the DFT (the two
 // chirp products carry the 1/4096 of the inverse FFT).
-template <int DIR>
-__device__ __forceinline__ void blue4k(double2 *v, int n, const double2 *tab, const FftDesc &d, double *T,
-                                       const double *Tp, int e, int lane) {
+// out(q, X) (blue4k_out) receives output register q < 16 as soon as it is
+// final instead (the registers are not written back: for callers that store
+// straight to memory).
+template <int DIR, bool OUT, class Out>
+__device__ __forceinline__ void blue4k_impl(double2 *v, int n, const double2 *tab, const FftDesc &d, double *T,
+                                            const double *Tp, int e, int lane, Out out) {
   const int h = lane >> 5, l = lane & 31;
   const double2 *__restrict__ bh = DIR < 0 ? d.bhat_f : d.bhat_b;
 #pragma unroll
@@ -146,8 +151,20 @@ __device__ __forceinline__ void blue4k(double2 *v, int n, const double2 *tab, co
   for (int q = 16; q < 32; ++q) v[q] = make_double2(0.0, 0.0);
   fft4k_fwd(v, T, Tp, tab, e, lane, [&](int q, double2 &z) { z = cmul(z, bh[l + 32 * q + 1024 * h + 2048 * e]); });
   fft4k_inv<true>(v, T, Tp, tab, e, lane, [&](int q, double2 &z) {
-    z = cmul(z, r4k_chirp<DIR>(d.chirp, n, 4 * (l + 32 * q) + 2 * h + e));
+    const double2 c = r4k_chirp<DIR>(d.chirp, n, 4 * (l + 32 * q) + 2 * h + e);
+    if (OUT) out(q, cmul(z, c));
+    else z = cmul(z, c);
   });
+}
+template <int DIR>
+__device__ __forceinline__ void blue4k(double2 *v, int n, const double2 *tab, const FftDesc &d, double *T,
+                                       const double *Tp, int e, int lane) {
+  blue4k_impl<DIR, false>(v, n, tab, d, T, Tp, e, lane, [](int, double2) {});
+}
+template <int DIR, class Out>
+__device__ __forceinline__ void blue4k_out(double2 *v, int n, const double2 *tab, const FftDesc &d, double *T,
+                                           const double *Tp, int e, int lane, Out out) {
+  blue4k_impl<DIR, true>(v, n, tab, d, T, Tp, e, lane, out);
 }
 
 }  // namespace so3g

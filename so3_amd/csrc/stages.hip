@@ -252,9 +252,10 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
   __syncthreads();
   const double2 *bh = DIR < 0 ? fd.bhat_f : fd.bhat_b;
   const int64_t groups = (nrows + kRows32Waves - 1) / kRows32Waves;
+  // rows of the mapped layouts number nG nB < 2^31 (32-bit arithmetic)
   auto row_at = [&](int64_t r, bool IN) -> int64_t {
     if (map == kRowsFlat) return r * rs;
-    const int k = (int)(r / g.nB), b = (int)(r % g.nB);
+    const int ri = (int)r, k = ri / g.nB, b = ri - k * g.nB;
     const bool x2 = (map == kRowsX2F) == IN;
     return x2 ? x2_row(g, k, b) * g.ldA : r * g.nA;
   };
@@ -266,17 +267,16 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
     double2 v[32];
 #pragma unroll
     for (int q = 0; q < 16; ++q) v[q] = in[min(2 * (l + 32 * q) + h, n - 1)];  // past n: cleared by the chirp
-    blue2k<DIR>(v, n, tab, bh, T, lane);
-    // store addresses from a fresh opaque lane index: the compiler would
-    // otherwise keep the 16 load addresses live across the transform
+    // outputs stored as the last DFT-32 makes them, addresses from a fresh
+    // opaque lane index (the compiler would otherwise keep the 16 load
+    // addresses live across the transform)
     const int ls = loop_lane();
     double2 *row = dst + row_at(r, false);
     const double sc = 0.5 * scale;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
+    blue2k_out<DIR>(v, n, tab, bh, T, lane, [&](int q, double2 z) {
       const int j = 2 * ((ls & 31) + 32 * q) + (ls >> 5);
-      if (j < n) row[j] = cscale(v[q], sc);
-    }
+      if (j < n) row[j] = cscale(z, sc);
+    });
   }
 }
 
@@ -306,30 +306,30 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(2, 2))) 
   r4k_stage_blue(tab, fd);
   __syncthreads();
   const int64_t groups = (nrows + kPairs - 1) / kPairs;
+  // rows of the mapped layouts number nG nB < 2^31 (32-bit arithmetic)
   auto row_at = [&](int64_t r, bool IN) -> int64_t {
     if (map == kRowsFlat) return r * rs;
-    const int k = (int)(r / g.nB), b = (int)(r % g.nB);
+    const int ri = (int)r, k = ri / g.nB, b = ri - k * g.nB;
     const bool x2 = (map == kRowsX2F) == IN;
     return x2 ? x2_row(g, k, b) * g.ldA : r * g.nA;
   };
   for (int64_t t = xcd_tile(blockIdx.x, gridDim.x); t < groups; t += gridDim.x) {
     const int64_t r = t * kPairs + (wv >> 1);
-    const int64_t rr = r < nrows ? r : nrows - 1;
+    const bool ok = r < nrows;  // a pair past the last row repeats it, unstored
+    const int64_t rr = ok ? r : nrows - 1;
     const int lane = loop_lane(), l = lane & 31, h = lane >> 5;
     const double2 *in = src + row_at(rr, true);
     double2 v[32];
 #pragma unroll
     for (int q = 0; q < 16; ++q) v[q] = in[min(4 * (l + 32 * q) + 2 * h + e, n - 1)];  // past n: cleared by the chirp
-    blue4k<DIR>(v, n, tab, fd, T, Tp, e, lane);
-    if (r >= nrows) continue;
-    // store addresses from a fresh opaque lane index (see k_rows_r2k)
+    // outputs stored as the last DFT-32 makes them, addresses from a fresh
+    // opaque lane index (see k_rows_r2k)
     const int ls = loop_lane();
-    double2 *row = dst + row_at(r, false);
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
+    double2 *row = dst + row_at(rr, false);
+    blue4k_out<DIR>(v, n, tab, fd, T, Tp, e, lane, [&](int q, double2 z) {
       const int j = 4 * ((ls & 31) + 32 * q) + 2 * (ls >> 5) + e;
-      if (j < n) row[j] = cscale(v[q], scale);
-    }
+      if (ok && j < n) row[j] = cscale(z, scale);
+    });
   }
 }
 
