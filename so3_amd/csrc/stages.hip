@@ -1421,10 +1421,13 @@ __global__ __launch_bounds__(64 * kR4kWaves) __attribute__((amdgpu_waves_per_eu(
       }
       if (!ODD && k == L) a = b = make_double2(0.0, 0.0);  // MW_SS: m' = -L
       v[q] = cmul(make_double2(a.x - b.y, a.y + b.x), phase[k]);
+      if ((q & 3) == 3) r32_fence();  // loads issued four registers at a time
     }
     blue4k<+1>(v, Pb, tab, fd, T, Tp, e, lane);
     // Z(b') through the buffer: entry q 64 + lane holds register q; sample
     // j = 2 i + e of this wavefront sits at (i >> 6) 64 + 32 (i & 1) + ((i >> 1) & 31)
+    // (written from the registers after the transform: writing it as the last
+    // DFT-32 makes it and reading Z(b) back measured 2-3% slower)
     double2 *T2 = (double2 *)T;
 #pragma unroll
     for (int q = 0; q < 16; ++q) T2[q * 64 + lane] = v[q];
@@ -1917,16 +1920,20 @@ __global__ __launch_bounds__(64 * kR4kWaves) __attribute__((amdgpu_waves_per_eu(
           else b = make_double2(0.0, 0.0);
         }
         v[q] = make_double2(a.x - b.y, a.y + b.x);
+        if ((q & 3) == 3) r32_fence();  // loads issued four registers at a time
       }
     }
-    blue4k<-1>(v, Pb, tab, fb, T, Tp, e, loop_lane());  // the DFT, registers 0..15
+    {
+      // the DFT into the buffer (entry q 64 + lane holds register q), written
+      // as the last DFT-32 makes it
+      const int lane = loop_lane();
+      blue4k_out<-1>(v, Pb, tab, fb, T, Tp, e, lane, [&](int q, double2 z) { T2[q * 64 + lane] = z; });
+    }
     double2 z1, zm1;
     {
       // remap: u(r) = F~(m'') phase(m''), m'' = r (r <= L-1) or r - 4096
       // (r >= 4096 - (L-1)), zero otherwise; F~(+-1) for the m' = 0 fold
       const int lane = loop_lane(), l = lane & 31, h = lane >> 5;
-#pragma unroll
-      for (int q = 0; q < 16; ++q) T2[q * 64 + lane] = v[q];
       __syncthreads();
       // entry s = r - 4096 + Pb has the parity e ^ (Pb & 1)
       const double2 *S2 = (const double2 *)(ODD ? Tp : T);
@@ -1941,12 +1948,12 @@ __global__ __launch_bounds__(64 * kR4kWaves) __attribute__((amdgpu_waves_per_eu(
       const double2 *B1 = (const double2 *)(e ? T : Tp), *Bm1 = (const double2 *)(((Pb - 1) & 1) == e ? T : Tp);
       z1 = cmul(B1[at(0)], phase[L]);
       zm1 = cmul(Bm1[at((Pb - 1) >> 1)], phase[L - 2]);
-      __syncthreads();  // both wavefronts have read the buffers before the transforms reuse them
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const int r = 4 * (l + 32 * q) + 2 * h + e;
-        v[q] = r <= L - 1 ? cmul(v[q], phase[min(r, L - 1) + L - 1]) : make_double2(0.0, 0.0);
+        v[q] = r <= L - 1 ? cmul(T2[q * 64 + lane], phase[min(r, L - 1) + L - 1]) : make_double2(0.0, 0.0);
       }
+      __syncthreads();  // both wavefronts have read the buffers before the transforms reuse them
 #pragma unroll
       for (int q = 8; q < 24; ++q) v[q] = make_double2(0.0, 0.0);
     }
