@@ -649,6 +649,11 @@ def main():
         "wall_s": wall,
         "roundtrip_rel_err": err,
         "stage_ms": {k: round(v, 4) for k, v in per.items()},
+        # the stage profiler records a pair of events around every stage, so
+        # consecutive kernels no longer overlap at their boundaries: its sum
+        # runs a few percent above the undisturbed step
+        "stage_sum_ms": round(sum(per.values()), 4),
+        "stage_sum_over_step": round(sum(per.values()) / ms_step, 4),
         "roofline": roofline,
         "stage_roofline": stage_roof,
         "path_roofline": {"t_min_ms": t_min * 1e3, "frac": t_min * 1e3 / ms_step,
