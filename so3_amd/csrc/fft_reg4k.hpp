@@ -17,7 +17,13 @@
 // each wavefront writes its values to its own transpose buffer, the workgroup
 // synchronises, and each reads its partner's (so every wavefront of the
 // workgroup must run them together).  (Rounds of 16 kept 64 more VGPRs live
-// and spilled.)  As in fft_reg2k.hpp each lane then
+// and spilled.)  The rounds alternate between the two 8 KB halves of the
+// buffer, so one barrier per round suffices: a wavefront writes a half again
+// only two rounds later, after a barrier that its partner passed once done
+// reading it.  The forward transform's last round is followed by the inverse
+// transform's first (the other half); the inverse transform ends its rounds
+// with one more barrier, because the 2048-point transform after them
+// rewrites the whole buffer.  As in fft_reg2k.hpp each lane then
 // computes sigma own' + partner' (forward, own' = own v^(e k)) or
 // (sigma own + partner) conj(v)^(e k) (inverse), sigma = 1 - 2 e.
 //
@@ -35,6 +41,8 @@ namespace so3g {
 constexpr int kR4kLo = kR2kCh;
 constexpr int kR4kHi = kR2kCh + 64;
 constexpr int kR4kTab = kR2kCh + 96;
+static_assert(2 * kR2kRound * 64 * sizeof(double2) <= kR32T * sizeof(double),
+              "the two halves of the exchange must fit the transpose buffer");
 
 // copies the LDS tables of a Bluestein descriptor with inner length 4096; the
 // caller synchronises
@@ -55,18 +63,18 @@ __device__ __forceinline__ double2 r4k_tw(const double2 *tab, int e, int h, int 
 }
 
 // one round of the cross-wavefront exchange over registers q0 .. q0 +
-// kR2kRound - 1: the partner wavefront's values arrive in p[].  Tp: the
-// partner's buffer.  Every wavefront of the workgroup calls it together (two
-// barriers).
-__device__ __forceinline__ void r4k_swap(const double2 *v, double2 *p, double *T, const double *Tp, int lane) {
-  double2 *T2 = (double2 *)T;
-  const double2 *P2 = (const double2 *)Tp;
+// kR2kRound - 1 through half `half` of the buffers: the partner wavefront's
+// values arrive in p[].  Tp: the partner's buffer.  Every wavefront of the
+// workgroup calls it together (one barrier).
+__device__ __forceinline__ void r4k_swap(const double2 *v, double2 *p, double *T, const double *Tp, int lane,
+                                         int half) {
+  double2 *T2 = (double2 *)T + half * (kR2kRound * 64);
+  const double2 *P2 = (const double2 *)Tp + half * (kR2kRound * 64);
 #pragma unroll
   for (int i = 0; i < kR2kRound; ++i) T2[i * 64 + lane] = v[i];
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < kR2kRound; ++i) p[i] = P2[i * 64 + lane];
-  __syncthreads();
 }
 
 // forward 4096-point transform of the pair, layout E in, layout QQ out.
@@ -84,7 +92,7 @@ __device__ __forceinline__ void fft4k_fwd(double2 *v, double *T, const double *T
 #pragma unroll
     for (int i = 0; i < R; ++i) v[R * r + i] = cmul(v[R * r + i], r4k_tw(tab, e, h, l, R * r + i));
     r32_fence();
-    r4k_swap(v + R * r, p, T, Tp, lane);
+    r4k_swap(v + R * r, p, T, Tp, lane, r & 1);
 #pragma unroll
     for (int i = 0; i < R; ++i) {
       double2 &z = v[R * r + i];
@@ -106,7 +114,7 @@ __device__ __forceinline__ void fft4k_inv(double2 *v, double *T, const double *T
 #pragma unroll
   for (int r = 0; r < 32 / R; ++r) {
     double2 p[R];
-    r4k_swap(v + R * r, p, T, Tp, lane);
+    r4k_swap(v + R * r, p, T, Tp, lane, r & 1);
 #pragma unroll
     for (int i = 0; i < R; ++i) {
       double2 &z = v[R * r + i];
@@ -114,6 +122,7 @@ __device__ __forceinline__ void fft4k_inv(double2 *v, double *T, const double *T
     }
     r32_fence();
   }
+  __syncthreads();  // the partner has read this buffer's last half before it is rewritten
   fft2k_inv<HALF>(v, T, tab, lane, post);
 }
 
