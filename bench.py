@@ -193,6 +193,38 @@ def _batch_legs(A, torch, specs, reps):
     }
 
 
+def batched_plain_only(A, P, torch, table, reps=2):
+    """The 96-item batched workload as plain calls one after another (one
+    warm-up pass, then `reps` timed), tuned by the table in `table` (made by
+    the autotuner and saved there when the file does not exist)."""
+    ps = batched_workload(P)
+    specs, keep = _pair_specs(A, torch, ps, 4)
+    if os.path.exists(table):
+        A.tune_load(table)
+        progress("batched plain: tuning table loaded")
+    else:
+        for p in ps:
+            A.tune(p)
+        A.tune_save(table)
+        progress("batched plain: tuning table made and saved")
+    sid = torch.cuda.current_stream().cuda_stream
+
+    def plain():
+        for sp in specs:
+            A.inverse_device(sp["f"], sp["flmn_in"], sp["p"], A.VIA_SSHT, sid)
+            A.forward_device(sp["flmn_out"], sp["f"], sp["p"], A.VIA_SSHT, sid)
+    plain()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plain()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    del keep
+    return {"workload": "96 uneven fwd+inv pairs (seed 4) as plain calls, tuned", "reps": reps,
+            "plain_ms": ms}
+
+
 def batched_bench(A, P, torch, reps=5):
     """96 uneven fwd+inv pairs: plain calls one after another on one stream
     versus one batch call (lanes or persistent mode, tuned) and its captured
@@ -543,6 +575,10 @@ def main():
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with a 1 GiB device-to-device copy (reference kernel for HBM counters)")
+    ap.add_argument("--batched-plain", metavar="TABLE",
+                    help="only time the 96-item batched workload as plain calls, with the autotuner's "
+                         "table read from TABLE (made and saved there first when absent), so that a "
+                         "profiler run of this mode holds the workload's kernels and not the tuner's")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one GPU's "
@@ -553,6 +589,10 @@ def main():
     import so3_amd.api as A
     import so3_amd.params as P
     from so3_amd import synth
+
+    if args.batched_plain:
+        print(json.dumps(batched_plain_only(A, P, torch, args.batched_plain)), flush=True)
+        return
 
     p = P.SO3Parameters(L=args.L, N=args.N, sampling_scheme=args.sampling, reality=args.real,
                         dl_method=args.dl, n_order=P.SO3_N_ORDER_ZERO_FIRST,
