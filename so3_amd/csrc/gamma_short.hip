@@ -1,5 +1,7 @@
-// gamma_short.hip -- the gamma DFT of short lengths nG = 2N - 1 <= 31
-// (N <= 16) as a direct DFT, one column per thread.
+// gamma_short.hip -- the gamma DFT of short lengths as a direct DFT, one
+// column per thread: nG = 2N - 1 <= 31 (N <= 16) for complex signals, up to
+// 63 (N <= 32) for real ones, whose c2r / r2c forms keep only N - 1 complex
+// (or 2 (N - 1) real) inputs per column in registers.
 //
 // Reference counterparts: inverse gamma, backward DFT over n
 // (so3_core.c:137-158, 216; c2r :564-576); forward gamma, forward DFT over g
@@ -191,18 +193,18 @@ const double2 *short_table(int nG) {
   return d;
 }
 
-template <int NG>
+template <int NG, int REAL>
 void go_short(const Geo &g, const void *src, void *dst, int dir, double scale, hipStream_t st) {
   const double2 *cs = short_table(NG);
   const int64_t S = (int64_t)g.nB * g.nA;
   const unsigned grid = (unsigned)((S + 255) / 256);
   if (dir > 0) {
-    if (g.real)
+    if constexpr (REAL != 0)
       k_inv_gamma_dft<NG, 1><<<grid, 256, 0, st>>>(g, cs, (const double2 *)src, nullptr, (double *)dst, scale);
     else
       k_inv_gamma_dft<NG, 0><<<grid, 256, 0, st>>>(g, cs, (const double2 *)src, (double2 *)dst, nullptr, scale);
   } else {
-    if (g.real)
+    if constexpr (REAL != 0)
       k_fwd_gamma_dft<NG, 1><<<grid, 256, 0, st>>>(g, cs, nullptr, (const double *)src, (double2 *)dst, scale);
     else
       k_fwd_gamma_dft<NG, 0><<<grid, 256, 0, st>>>(g, cs, (const double2 *)src, nullptr, (double2 *)dst, scale);
@@ -211,13 +213,20 @@ void go_short(const Geo &g, const void *src, void *dst, int dir, double scale, h
 }
 
 typedef void (*ShortFn)(const Geo &, const void *, void *, int, double, hipStream_t);
-const ShortFn kShort[16] = {go_short<1>,  go_short<3>,  go_short<5>,  go_short<7>,  go_short<9>,  go_short<11>,
-                            go_short<13>, go_short<15>, go_short<17>, go_short<19>, go_short<21>, go_short<23>,
-                            go_short<25>, go_short<27>, go_short<29>, go_short<31>};
+#define SO3_SHORT16(R, B)                                                                                    \
+  go_short<B + 1, R>, go_short<B + 3, R>, go_short<B + 5, R>, go_short<B + 7, R>, go_short<B + 9, R>,       \
+      go_short<B + 11, R>, go_short<B + 13, R>, go_short<B + 15, R>, go_short<B + 17, R>, go_short<B + 19, R>, \
+      go_short<B + 21, R>, go_short<B + 23, R>, go_short<B + 25, R>, go_short<B + 27, R>, go_short<B + 29, R>, \
+      go_short<B + 31, R>
+const ShortFn kShortC[16] = {SO3_SHORT16(0, 0)};                      // nG = 1 .. 31
+const ShortFn kShortR[32] = {SO3_SHORT16(1, 0), SO3_SHORT16(1, 32)};  // nG = 1 .. 63
+#undef SO3_SHORT16
 
 }  // namespace
 
-bool gamma_short_applies(const Geo &g) { return g.N >= 1 && 2 * g.N - 1 <= kGammaShortMax; }
+bool gamma_short_applies(const Geo &g) {
+  return g.N >= 1 && 2 * g.N - 1 <= (g.real ? kGammaShortMaxReal : kGammaShortMax);
+}
 
 void gamma_short_prepare(const Geo &g) {
   if (gamma_short_applies(g)) (void)short_table(2 * g.N - 1);
@@ -226,7 +235,7 @@ void gamma_short_prepare(const Geo &g) {
 bool launch_gamma_short(const Geo &g, const void *src, void *dst, int dir, double scale, hipStream_t st) {
   if (!gamma_short_applies(g)) return false;
   if ((int64_t)g.nB * g.nA >= ((int64_t)1 << 31)) return false;
-  kShort[g.N - 1](g, src, dst, dir, scale, st);
+  (g.real ? kShortR : kShortC)[g.N - 1](g, src, dst, dir, scale, st);
   return true;
 }
 
