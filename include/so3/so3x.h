@@ -70,8 +70,9 @@ int so3x_prepare(const so3_parameters_t *parameters);
  * workgroup order of the inverse and forward contractions, the length Pw of
  * the forward transform's weight convolution (every 2^a or 3 * 2^a from
  * 4L - 3 up to twice the default, which decides the FFT engine and plan that
- * run it), and for gamma lengths 2N - 1 in [33, 127] the engine of the gamma
- * DFTs (1: the workgroup engine, 2: the 256-point register FFT) -- and keeps
+ * run it), and for gamma lengths 2N - 1 in [65, 127] the engine of the gamma
+ * DFTs (1: the workgroup engine, 2: the 256-point register FFT; shorter
+ * lengths take a direct DFT) -- and keeps
  * the fastest for every later transform with the same (L, N, reality,
  * sampling), single or batched.  It allocates and synchronises; call it
  * outside timed regions.  choice (4 ints, may be NULL) receives (inverse

@@ -260,7 +260,7 @@ def tune(p, method=VIA_SSHT, stream=None):
     """Run the (L, N)-keyed autotuner for p; returns ({"corder_inv", "corder_fwd",
     "pw", "gamma_engine"}, {"contract_inv_ms", "contract_fwd_ms", "beta_fwd_ms",
     "gamma_ms"}).  gamma_engine: 1 workgroup, 2 256-point register FFT, 0 where
-    the gamma length 2N-1 is outside [33, 127] (gamma_ms is then 0)."""
+    the gamma length 2N-1 is outside [65, 127] (gamma_ms is then 0)."""
     ch = (ctypes.c_int * 4)()
     ms = (ctypes.c_double * 4)()
     check(lib().so3x_tune(_s(p), method, ctypes.c_void_p(stream or 0), ch, ms))

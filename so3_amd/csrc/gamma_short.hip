@@ -1,7 +1,8 @@
-// gamma_short.hip -- the gamma DFT of short lengths as a direct DFT, one
-// column per thread: nG = 2N - 1 <= 31 (N <= 16) for complex signals, up to
-// 63 (N <= 32) for real ones, whose c2r / r2c forms keep only N - 1 complex
-// (or 2 (N - 1) real) inputs per column in registers.
+// gamma_short.hip -- the gamma DFT of short lengths as a direct DFT: one
+// column per thread for nG = 2N - 1 <= 31 (N <= 16) and for real signals up
+// to 63 (N <= 32), whose c2r / r2c forms keep only N - 1 complex (or
+// 2 (N - 1) real) inputs per column in registers; complex signals with nG in
+// [33, 63] take one column per pair of lanes (see k_gamma_dft2 below).
 //
 // Reference counterparts: inverse gamma, backward DFT over n
 // (so3_core.c:137-158, 216; c2r :564-576); forward gamma, forward DFT over g
@@ -169,6 +170,86 @@ __global__ __launch_bounds__(256) void k_fwd_gamma_dft(Geo g, const double2 *__r
   }
 }
 
+// Complex gamma for nG in [33, 63] with one column per pair of lanes: the
+// even lane of a pair carries the real parts of the column and the odd lane
+// the imaginary parts.  Each lane forms u_j, v_j of its part and the two sums
+//   even (re):  A = x0.re + sum_j u_j.re c,   C = sum_j v_j.re s
+//   odd  (im):  D = x0.im + sum_j u_j.im c,   B = sum_j v_j.im s
+// with the same twiddle indices in both lanes (uniform: scalar loads); one
+// swap with the partner lane (B to the even lane, C to the odd one) gives
+//   X(k).re = A - DIR B, X(nG-k).re = A + DIR B   (even lane)
+//   X(k).im = D + DIR C, X(nG-k).im = D - DIR C   (odd lane)
+// 2 H^2 multiply-adds and 2 H doubles of inputs per lane (H <= 31), and the
+// two lanes' 8-byte accesses meet in each 16-byte entry.  DIR = +1: X2 -> f
+// (inverse, times scale); -1: f -> X2 (forward, times scale).
+__device__ __forceinline__ double swap_lane_pair(double x) {  // x of lane ^ 1
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, 0xB1, 0xF, 0xF, false);  // quad_perm [1, 0, 3, 2]
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0xB1, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <int NG, int DIR>
+__global__ __launch_bounds__(256) void k_gamma_dft2(Geo g, const double2 *__restrict__ cs,
+                                                    const double2 *__restrict__ src, double2 *__restrict__ dst,
+                                                    double scale) {
+  constexpr int H = (NG - 1) / 2;
+  const int S = g.nB * g.nA;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = tid >> 1, part = tid & 1;
+  // every lane computes (a pair past the last column repeats it) so that the
+  // partner swaps see active lanes; only the stores are predicated
+  const bool ok = c < S;
+  const int cc = ok ? c : S - 1;
+  const int b = cc / g.nA, a = cc - b * g.nA;
+  const double2 *X2c = DIR > 0 ? src : dst;
+  const double *xcol = (const double *)(X2c + x2_row(g, 0, b) * g.ldA + a) + part;
+  const int64_t xs2 = (int64_t)x2_w(g, b) * g.ldA * 2;  // X2 slot step in doubles
+  double u[H + 1], v[H + 1], x0;
+  if (DIR > 0) {
+    x0 = xcol[0];
+#pragma unroll
+    for (int j = 1; j <= H; ++j) {
+      const double y = xcol[j * xs2], z = xcol[(NG - j) * xs2];
+      u[j] = y + z;
+      v[j] = y - z;
+    }
+  } else {
+    const double *fcol = (const double *)(src + cc) + part;
+    const int64_t fs2 = (int64_t)S * 2;
+    x0 = fcol[0];
+#pragma unroll
+    for (int j = 1; j <= H; ++j) {
+      const double y = fcol[j * fs2], z = fcol[(NG - j) * fs2];
+      u[j] = y + z;
+      v[j] = y - z;
+    }
+  }
+  auto store = [&](int k, double val) {
+    if (!ok) return;
+    if (DIR > 0) ((double *)(dst + (int64_t)k * S + cc))[part] = val * scale;
+    else ((double *)(dst + x2_row(g, 0, b) * g.ldA + a))[part + k * xs2] = val * scale;
+  };
+  double s0 = x0;
+#pragma unroll
+  for (int j = 1; j <= H; ++j) s0 += u[j];
+  store(0, s0);
+  const double sg = part == 0 ? -(double)DIR : (double)DIR;
+#pragma unroll
+  for (int k = 1; k <= H; ++k) {
+    double P = x0, Q = 0.0;
+#pragma unroll
+    for (int j = 1; j <= H; ++j) {
+      const double2 t = cs[(j * k) % NG];
+      P = fma(u[j], t.x, P);
+      Q = fma(v[j], t.y, Q);
+    }
+    const double o = swap_lane_pair(Q);
+    store(k, P + sg * o);
+    store(NG - k, P - sg * o);
+  }
+}
+
 // cosines and sines of 2 pi m / nG (m < nG), rounded from long double; one
 // table per (device, nG), kept for the lifetime of the process
 const double2 *short_table(int nG) {
@@ -222,10 +303,29 @@ const ShortFn kShortC[16] = {SO3_SHORT16(0, 0)};                      // nG = 1 
 const ShortFn kShortR[32] = {SO3_SHORT16(1, 0), SO3_SHORT16(1, 32)};  // nG = 1 .. 63
 #undef SO3_SHORT16
 
+// complex nG in [33, 63]: one column per pair of lanes
+template <int NG>
+void go_short2(const Geo &g, const void *src, void *dst, int dir, double scale, hipStream_t st) {
+  const double2 *cs = short_table(NG);
+  const int64_t S = (int64_t)g.nB * g.nA;
+  const unsigned grid = (unsigned)((2 * S + 255) / 256);
+  if (dir > 0)
+    k_gamma_dft2<NG, +1><<<grid, 256, 0, st>>>(g, cs, (const double2 *)src, (double2 *)dst, scale);
+  else
+    k_gamma_dft2<NG, -1><<<grid, 256, 0, st>>>(g, cs, (const double2 *)src, (double2 *)dst, scale);
+  SO3_LAUNCH_CHECK();
+}
+const ShortFn kShortC2[16] = {go_short2<33>, go_short2<35>, go_short2<37>, go_short2<39>, go_short2<41>,
+                              go_short2<43>, go_short2<45>, go_short2<47>, go_short2<49>, go_short2<51>,
+                              go_short2<53>, go_short2<55>, go_short2<57>, go_short2<59>, go_short2<61>,
+                              go_short2<63>};
+
 }  // namespace
 
 bool gamma_short_applies(const Geo &g) {
-  return g.N >= 1 && 2 * g.N - 1 <= (g.real ? kGammaShortMaxReal : kGammaShortMax);
+  // the kernels index columns with 32-bit threads (two per column above 31)
+  const int64_t S = (int64_t)g.nB * g.nA;
+  return g.N >= 1 && 2 * g.N - 1 <= kGammaShortMax && S < ((int64_t)1 << 30);
 }
 
 void gamma_short_prepare(const Geo &g) {
@@ -234,8 +334,9 @@ void gamma_short_prepare(const Geo &g) {
 
 bool launch_gamma_short(const Geo &g, const void *src, void *dst, int dir, double scale, hipStream_t st) {
   if (!gamma_short_applies(g)) return false;
-  if ((int64_t)g.nB * g.nA >= ((int64_t)1 << 31)) return false;
-  (g.real ? kShortR : kShortC)[g.N - 1](g, src, dst, dir, scale, st);
+  if (g.real) kShortR[g.N - 1](g, src, dst, dir, scale, st);
+  else if (g.N <= 16) kShortC[g.N - 1](g, src, dst, dir, scale, st);
+  else kShortC2[g.N - 17](g, src, dst, dir, scale, st);
   return true;
 }
 

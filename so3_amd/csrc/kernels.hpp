@@ -103,19 +103,18 @@ void launch_alpha(const Geo &g, double2 *X2, int dir, double scale, hipStream_t 
 void launch_rows(double2 *X, int64_t nrows, int n, int ld, int dir, double scale, hipStream_t st);
 void launch_inv_gamma(const Geo &g, const double2 *X2, void *f, double scale, hipStream_t st);
 void launch_fwd_gamma(const Geo &g, const void *f, double2 *X2, double scale, hipStream_t st);
-// gamma of short lengths 2N - 1 <= kGammaShortMax (complex) or
-// kGammaShortMaxReal (real) as a direct DFT, one column per thread
-// (gamma_short.hip): dir = +1 X2 -> f (as launch_inv_gamma), -1 f -> X2 (as
+// gamma of short lengths 2N - 1 <= kGammaShortMax as a direct DFT (one
+// column per thread, or per pair of lanes for complex lengths above 31;
+// gamma_short.hip): dir = +1 X2 -> f (as launch_inv_gamma), -1 f -> X2 (as
 // launch_fwd_gamma); false when the length is not short.
 // gamma_short_prepare makes its table ahead of a timed or captured call.
-constexpr int kGammaShortMax = 31;
-constexpr int kGammaShortMaxReal = 63;
+constexpr int kGammaShortMax = 63;
 bool gamma_short_applies(const Geo &g);
 void gamma_short_prepare(const Geo &g);
 bool launch_gamma_short(const Geo &g, const void *src, void *dst, int dir, double scale, hipStream_t st);
 // the engine launch_inv_gamma / launch_fwd_gamma run for g when 2N-1 is in
-// [33, 127] and the length is not short (1 workgroup, 2 256-point register
-// FFT; g.geng overrides the default rule), else 0
+// [33, 127] and the length is not short, i.e. in [65, 127] (1 workgroup, 2
+// 256-point register FFT; g.geng overrides the default rule), else 0
 int gamma_engine(const Geo &g);
 // steerable gamma: dir = +1 X2 -> f (N samples kept), dir = -1 f -> X2
 void launch_gamma_steer(const Geo &g, const void *src, void *dst, int dir, double scale, hipStream_t st);

@@ -2608,7 +2608,7 @@ static void launch_gamma_real(const Geo &g, const FftDesc &fd, double2 *X2, doub
 int gamma_engine(const Geo &g) {
   const int nG = 2 * g.N - 1;
   if (g.steer || nG < 33 || nG > 127 || (int64_t)g.nB * g.nA >= ((int64_t)1 << 30)) return 0;
-  if (gamma_short_applies(g) && (int64_t)g.nB * g.nA < ((int64_t)1 << 31)) return 0;  // the direct DFT
+  if (gamma_short_applies(g)) return 0;  // the direct DFT
   if (g.geng == 1 || g.geng == 2) return g.geng;
   return fft_plan(nG).blue ? 2 : 1;
 }

@@ -1,10 +1,10 @@
 """The gamma DFT at every short length class against the CPU oracle.
 
-Gamma lengths 2N-1 <= 31 (complex) or 63 (real) run as a direct DFT, one
-column per thread (gamma_short.hip); the other lengths in [33, 127] on the
-workgroup engine (smooth lengths, directly) or the 256-point register FFT
-(Bluestein), either one forced through the tuning table in the second test;
-longer ones on the other FFT engines.  Each case runs the inverse of seeded coefficients and the
+Gamma lengths 2N-1 <= 63 run as a direct DFT (gamma_short.hip: one column
+per thread, or per pair of lanes for complex lengths above 31); lengths in
+[65, 127] on the workgroup engine (smooth lengths, directly) or the
+256-point register FFT (Bluestein), either one forced through the tuning
+table in the second test; longer ones on the other FFT engines.  Each case runs the inverse of seeded coefficients and the
 forward of arbitrary samples (which pins the forward DFT on data that is not
 band-limited) through the C ABI, complex and real, MW and MW_SS, against the
 oracle at 1e-12 relative to max|input|.
@@ -42,13 +42,13 @@ def _check(oracle, N, real, ss):
     assert _rel(b_gpu, b_ref) < TOL
 
 
-@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 7, 8, 11, 15, 16, 17, 24, 32, 33, 47, 64])
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 7, 8, 11, 15, 16, 17, 18, 21, 24, 31, 32, 33, 47, 64])
 @pytest.mark.parametrize("real,ss", [(0, 0), (1, 0), (0, 1), (1, 1)])
 def test_gamma_lengths_vs_oracle(oracle, N, real, ss):
     _check(oracle, N, real, ss)
 
 
-@pytest.mark.parametrize("N", [17, 18, 25, 32, 38, 41, 64])
+@pytest.mark.parametrize("N", [25, 33, 38, 41, 53, 64])
 @pytest.mark.parametrize("engine", [1, 2])
 @pytest.mark.parametrize("real", [0, 1])
 def test_gamma_engine_forced_vs_oracle(oracle, tmp_path, N, engine, real):
@@ -57,8 +57,8 @@ def test_gamma_engine_forced_vs_oracle(oracle, tmp_path, N, engine, real):
     A.tune_load(t)
     try:
         p = P.SO3Parameters(L=N + 3, N=N, reality=real, n_order=P.SO3_N_ORDER_ZERO_FIRST)
-        # real signals up to 2N-1 = 63 take the direct DFT whatever the table says
-        assert A.tune_get(p)["gamma_engine"] == (0 if real and 2 * N - 1 <= 63 else engine)
+        # lengths up to 2N-1 = 63 take the direct DFT whatever the table says
+        assert A.tune_get(p)["gamma_engine"] == (0 if 2 * N - 1 <= 63 else engine)
         _check(oracle, N, real, 0)
     finally:
         A.tune_clear()

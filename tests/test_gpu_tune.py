@@ -44,16 +44,16 @@ def test_contraction_order_is_bit_identical(tmp_path, L, N, real, ss):
     A.tune_load(t)
 
 
-@pytest.mark.parametrize("L,N,real,ss", [(40, 20, 0, 0), (120, 8, 1, 1), (200, 30, 0, 1), (90, 40, 1, 0)])
+@pytest.mark.parametrize("L,N,real,ss", [(40, 20, 0, 0), (120, 8, 1, 1), (200, 30, 0, 1), (90, 40, 1, 0),
+                                         (120, 45, 0, 0)])
 def test_tuned_transform_matches_oracle(oracle, L, N, real, ss):
     p = P.SO3Parameters(L=L, N=N, reality=real, sampling_scheme=ss, n_order=Z)
     choice, ms = A.tune(p)
     assert choice["corder_inv"] in (0, 1) and choice["corder_fwd"] in (0, 1)
     assert 4 * L - 3 <= choice["pw"] <= 4096
-    # the gamma engine is a choice for gamma lengths 2N-1 in [33, 127] that do
-    # not take the direct DFT (real signals up to 63) only
-    short = 2 * N - 1 <= (63 if real else 31)
-    assert choice["gamma_engine"] in ((1, 2) if not short and 2 * N - 1 <= 127 else (0,))
+    # the gamma engine is a choice for gamma lengths 2N-1 in [65, 127] only
+    # (shorter ones take the direct DFT)
+    assert choice["gamma_engine"] in ((1, 2) if 65 <= 2 * N - 1 <= 127 else (0,))
     assert (ms["gamma_ms"] > 0) == (choice["gamma_engine"] != 0)
     assert all(v > 0 for k, v in ms.items() if k != "gamma_ms")
     assert A.tune_get(p) == choice
