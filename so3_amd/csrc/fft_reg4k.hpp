@@ -178,6 +178,17 @@ __device__ __forceinline__ void blue4k_pre_out(double2 *v, const double2 *tab, c
   fft4k_fwd(v, T, Tp, tab, e, lane, [&](int q, double2 &z) { z = cmul(z, spre[q]); });
   fft4k_inv<true>(v, T, Tp, tab, e, lane, [&](int q, double2 &z) { out(q, cmul(z, cpre[q])); });
 }
+// blue4k with the tables in registers, as blue4k_pre_out
+template <int DIR>
+__device__ __forceinline__ void blue4k_pre(double2 *v, const double2 *tab, const double2 *cpre, const double2 *spre,
+                                           double *T, const double *Tp, int e, int lane) {
+#pragma unroll
+  for (int q = 0; q < 16; ++q) v[q] = cmul(v[q], cpre[q]);
+#pragma unroll
+  for (int q = 16; q < 32; ++q) v[q] = make_double2(0.0, 0.0);
+  fft4k_fwd(v, T, Tp, tab, e, lane, [&](int q, double2 &z) { z = cmul(z, spre[q]); });
+  fft4k_inv<true>(v, T, Tp, tab, e, lane, [&](int q, double2 &z) { z = cmul(z, cpre[q]); });
+}
 template <int DIR>
 __device__ __forceinline__ void blue4k(double2 *v, int n, const double2 *tab, const FftDesc &d, double *T,
                                        const double *Tp, int e, int lane) {
