@@ -293,17 +293,24 @@ __device__ __forceinline__ void contract_inv_block(unsigned blk, const Geo &g, c
     if (g.clmode == 1) fac *= (2.0 * lk + 1.0) * kInv8Pi2;
     if (!used) fac = 0.0;
     double cv[kXG], dy[kNT];
+    // operands read through a volatile LDS pointer: otherwise the compiler
+    // pairs the reads of two tiles 512 bytes apart into ds_read2st64_b64,
+    // whose two addresses fall in the same banks (1.2 conflict cycles per
+    // active LDS cycle); separate reads of 64 consecutive doubles measured
+    // 0-4% faster, with 25 fewer VGPRs
+    const volatile __attribute__((address_space(3))) double *vb =
+        (const volatile __attribute__((address_space(3))) double *)buf;
 #pragma unroll
-    for (int a = 0; a < kXG; ++a) cv[a] = fac * buf[cvo + 2 * a * St::kRegD];
+    for (int a = 0; a < kXG; ++a) cv[a] = fac * vb[cvo + 2 * a * St::kRegD];
 #pragma unroll
-    for (int t = 0; t < kNT; ++t) dy[t] = buf[1024 + (4 * w + t) * 64 + lane];
+    for (int t = 0; t < kNT; ++t) dy[t] = vb[1024 + (4 * w + t) * 64 + lane];
     // tiles with m' > l or m' >= L were copied from the zero page: the
     // matrix instructions run unconditionally, which keeps the accumulators
     // in place (a branch per tile made the compiler move them every chunk)
 #pragma unroll
     for (int t = 0; t < kNT; ++t) {
 #pragma unroll
-      for (int a = 0; a < kXG; ++a) acc[a][t] = mma(cv[a], buf[(4 * a + t) * 64 + lane] * dy[t], acc[a][t]);
+      for (int a = 0; a < kXG; ++a) acc[a][t] = mma(cv[a], vb[(4 * a + t) * 64 + lane] * dy[t], acc[a][t]);
     }
   }
   if (!yv) return;
@@ -471,16 +478,19 @@ __device__ __forceinline__ void contract_fwd_block(unsigned blk, const Geo &g, c
     const double *buf = sm + (q % kRing) * kBufD;
     const int m0 = 4 * q;
     double cv[kXG], dy[kNT];
+    // separate LDS reads, as in the inverse
+    const volatile __attribute__((address_space(3))) double *vb =
+        (const volatile __attribute__((address_space(3))) double *)buf;
 #pragma unroll
-    for (int a = 0; a < kXG; ++a) cv[a] = used ? buf[cvo + 2 * a * St::kRegD] : 0.0;
+    for (int a = 0; a < kXG; ++a) cv[a] = used ? vb[cvo + 2 * a * St::kRegD] : 0.0;
 #pragma unroll
-    for (int t = 0; t < kNT; ++t) dy[t] = buf[1024 + (4 * w + t) * 64 + lane];
+    for (int t = 0; t < kNT; ++t) dy[t] = vb[1024 + (4 * w + t) * 64 + lane];
     // tiles with m' > l or l >= L were copied from the zero page: the matrix
     // instructions run unconditionally (see the inverse)
 #pragma unroll
     for (int t = 0; t < kNT; ++t) {
 #pragma unroll
-      for (int a = 0; a < kXG; ++a) acc[a][t] = mma(cv[a], buf[(4 * a + t) * 64 + lane] * dy[t], acc[a][t]);
+      for (int a = 0; a < kXG; ++a) acc[a][t] = mma(cv[a], vb[(4 * a + t) * 64 + lane] * dy[t], acc[a][t]);
     }
   }
   if (!yv) return;
