@@ -1590,6 +1590,10 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
     // (r >= 1024 - (L-1)), zero otherwise; F~(+-1) for the m' = 0 fold.
     // Entry s of F~ sits at (s >> 5) 64 + 32 h + (s & 31) of the buffer.
     double2 z1, zm1;
+    // phase(l) of this lane and the 16 lane-uniform block factors after the
+    // table: one load per lane instead of 16
+    const double2 pl = phase[l + L - 1];
+    const double2 *pblk = phase + 2 * L - 1;
     {
       const int d = l + Pb - 1024;  // s = d + 32 q for r = l + 32 q >= 768
       const int gb = (d >> 5) * 64 + 32 * h + (d & 31);
@@ -1613,7 +1617,8 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
         const int r = l + 32 * q;
         const bool ok = r >= 1024 - (L - 1);
         const double2 x = make_double2(gr[q - 24], T[gb + 64 * q]);
-        v[q] = ok ? cmul(x, phase[r - 1024 + L - 1]) : make_double2(0.0, 0.0);
+        // phase(m'' = l + 32 (q - 32)) = phase(l) exp(-i 32 (q - 32) pi/(2L-1))
+        v[q] = ok ? cmul(x, cmul(pl, pblk[q - 24])) : make_double2(0.0, 0.0);
       }
       z1 = cmul(make_double2(r1, T[i1]), phase[L]);
       zm1 = cmul(make_double2(rm1, T[im1]), phase[L - 2]);
@@ -1622,7 +1627,7 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const int r = l + 32 * q;
-        v[q] = r <= L - 1 ? cmul(v[q], phase[min(r, L - 1) + L - 1]) : make_double2(0.0, 0.0);
+        v[q] = r <= L - 1 ? cmul(v[q], cmul(pl, pblk[q + 8])) : make_double2(0.0, 0.0);
       }
 #pragma unroll
       for (int q = 8; q < 24; ++q) v[q] = make_double2(0.0, 0.0);

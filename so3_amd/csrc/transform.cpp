@@ -35,7 +35,7 @@ struct Plan {
   Geo g{};
   int Pw = 0;
   double2 *phase_inv = nullptr;  // Pb entries
-  double2 *phase_fwd = nullptr;  // 2L-1 entries, includes 1/Pb
+  double2 *phase_fwd = nullptr;  // 2L-1 entries, includes 1/Pb; then 16 block factors
   double2 *phase_ai = nullptr;   // 2L-1: exp(-i m'' pi/(2L-1)), adjoint of the inverse
   double2 *phase_af = nullptr;   // 2L-1: exp(+i m' pi/(2L-1)) / Pb, adjoint of the forward
   double2 *wr = nullptr;         // Pw entries
@@ -139,8 +139,10 @@ Plan build_plan(const so3_parameters_t *p, Method method) {
     }
   }
   P.phase_inv = upload(ph);
-  // forward: exp(-i m'' pi/(2L-1)) / Pb for m'' in [-(L-1), L-1]
-  std::vector<double2> pf(2 * L - 1);
+  // forward: exp(-i m'' pi/(2L-1)) / Pb for m'' in [-(L-1), L-1], then
+  // exp(-i 32 j pi/(2L-1)) for j = -8 .. 7 (no 1/Pb): the 1024-point forward
+  // beta forms the factor of m'' = l + 32 j as pf(l) times that entry
+  std::vector<double2> pf(2 * L - 1 + 16);
   for (int mpp = -(L - 1); mpp <= L - 1; ++mpp) {
     if (g.ss) {
       pf[mpp + L - 1] = make_double2(1.0 / g.Pb, 0.0);
@@ -148,6 +150,10 @@ Plan build_plan(const so3_parameters_t *p, Method method) {
       const long double a = -kPiL * mpp / (2.0L * L - 1.0L);
       pf[mpp + L - 1] = make_double2((double)(cosl(a) / g.Pb), (double)(sinl(a) / g.Pb));
     }
+  }
+  for (int j = -8; j < 8; ++j) {
+    const long double a = g.ss ? 0.0L : -kPiL * 32 * j / (2.0L * L - 1.0L);
+    pf[2 * L - 1 + 8 + j] = make_double2((double)cosl(a), (double)sinl(a));
   }
   P.phase_fwd = upload(pf);
   {
