@@ -1799,12 +1799,15 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
       for (int q = 0; q < 16; ++q) T2[q * 64 + lane] = v[q];
       __builtin_amdgcn_wave_barrier();
       r32_fence();
+      // phase(m'' = 2 l + h + 64 j) = phase(2 l + h) times block factor j
+      const double2 pl = phase[2 * l + h + L - 1];
+      const double2 *pblk = phase + 2 * L - 1 + 16;
 #pragma unroll
       for (int q = 24; q < 32; ++q) {
         const int r = 2 * (l + 32 * q) + h;
         const bool ok = r >= 2048 - (L - 1);
         const double2 x = T2[at(r - 2048 + Pb)];  // r - 2048 + Pb >= Pb - 512 >= 1
-        v[q] = ok ? cmul(x, phase[max(r - 2048 + L - 1, 0)]) : make_double2(0.0, 0.0);
+        v[q] = ok ? cmul(x, cmul(pl, pblk[q - 24])) : make_double2(0.0, 0.0);
       }
       z1 = cscale(cmul(T2[at(1)], phase[L]), 0.5);
       zm1 = cscale(cmul(T2[at(Pb - 1)], phase[L - 2]), 0.5);
@@ -1813,7 +1816,7 @@ __global__ __launch_bounds__(64 * kRows32Waves) __attribute__((amdgpu_waves_per_
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const int r = 2 * (l + 32 * q) + h;
-        v[q] = r <= L - 1 ? cmul(v[q], phase[min(r, L - 1) + L - 1]) : make_double2(0.0, 0.0);
+        v[q] = r <= L - 1 ? cmul(v[q], cmul(pl, pblk[q + 8])) : make_double2(0.0, 0.0);
       }
 #pragma unroll
       for (int q = 8; q < 24; ++q) v[q] = make_double2(0.0, 0.0);
@@ -1982,12 +1985,15 @@ __global__ __launch_bounds__(64 * kR4kWaves) __attribute__((amdgpu_waves_per_eu(
       __syncthreads();
       // entry s = r - 4096 + Pb has the parity e ^ (Pb & 1)
       const double2 *S2 = (const double2 *)(ODD ? Tp : T);
+      // phase(m'' = 4 l + 2 h + e + 128 j) = phase(4 l + 2 h + e) times block factor j
+      const double2 pl = phase[4 * l + 2 * h + e + L - 1];
+      const double2 *pblk = phase + 2 * L - 1 + 32;
 #pragma unroll
       for (int q = 24; q < 32; ++q) {
         const int r = 4 * (l + 32 * q) + 2 * h + e;
         const bool ok = r >= 4096 - (L - 1);
         const double2 x = S2[at((r - 4096 + Pb) >> 1)];  // r - 4096 + Pb >= Pb - 1024 >= 1
-        v[q] = ok ? cmul(x, phase[max(r - 4096 + L - 1, 0)]) : make_double2(0.0, 0.0);
+        v[q] = ok ? cmul(x, cmul(pl, pblk[q - 24])) : make_double2(0.0, 0.0);
       }
       // F~(1) in wavefront 1 at i = 0; F~(-1) = F~(Pb - 1) in wavefront (Pb - 1) & 1
       const double2 *B1 = (const double2 *)(e ? T : Tp), *Bm1 = (const double2 *)(((Pb - 1) & 1) == e ? T : Tp);
@@ -1996,7 +2002,7 @@ __global__ __launch_bounds__(64 * kR4kWaves) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const int r = 4 * (l + 32 * q) + 2 * h + e;
-        v[q] = r <= L - 1 ? cmul(T2[q * 64 + lane], phase[min(r, L - 1) + L - 1]) : make_double2(0.0, 0.0);
+        v[q] = r <= L - 1 ? cmul(T2[q * 64 + lane], cmul(pl, pblk[q + 8])) : make_double2(0.0, 0.0);
       }
       __syncthreads();  // both wavefronts have read the buffers before the transforms reuse them
 #pragma unroll

@@ -35,7 +35,7 @@ struct Plan {
   Geo g{};
   int Pw = 0;
   double2 *phase_inv = nullptr;  // Pb entries
-  double2 *phase_fwd = nullptr;  // 2L-1 entries, includes 1/Pb; then 16 block factors
+  double2 *phase_fwd = nullptr;  // 2L-1 entries, includes 1/Pb; then 3 x 16 block factors
   double2 *phase_ai = nullptr;   // 2L-1: exp(-i m'' pi/(2L-1)), adjoint of the inverse
   double2 *phase_af = nullptr;   // 2L-1: exp(+i m' pi/(2L-1)) / Pb, adjoint of the forward
   double2 *wr = nullptr;         // Pw entries
@@ -140,9 +140,10 @@ Plan build_plan(const so3_parameters_t *p, Method method) {
   }
   P.phase_inv = upload(ph);
   // forward: exp(-i m'' pi/(2L-1)) / Pb for m'' in [-(L-1), L-1], then
-  // exp(-i 32 j pi/(2L-1)) for j = -8 .. 7 (no 1/Pb): the 1024-point forward
-  // beta forms the factor of m'' = l + 32 j as pf(l) times that entry
-  std::vector<double2> pf(2 * L - 1 + 16);
+  // exp(-i S j pi/(2L-1)) for S = 32, 64, 128 and j = -8 .. 7 (no 1/Pb): the
+  // 1024-, 2048- and 4096-point forward beta form the factor of m'' = d + S j
+  // (d < S, one per lane) as pf(d) times that entry
+  std::vector<double2> pf(2 * L - 1 + 48);
   for (int mpp = -(L - 1); mpp <= L - 1; ++mpp) {
     if (g.ss) {
       pf[mpp + L - 1] = make_double2(1.0 / g.Pb, 0.0);
@@ -151,10 +152,11 @@ Plan build_plan(const so3_parameters_t *p, Method method) {
       pf[mpp + L - 1] = make_double2((double)(cosl(a) / g.Pb), (double)(sinl(a) / g.Pb));
     }
   }
-  for (int j = -8; j < 8; ++j) {
-    const long double a = g.ss ? 0.0L : -kPiL * 32 * j / (2.0L * L - 1.0L);
-    pf[2 * L - 1 + 8 + j] = make_double2((double)cosl(a), (double)sinl(a));
-  }
+  for (int e = 0; e < 3; ++e)
+    for (int j = -8; j < 8; ++j) {
+      const long double a = g.ss ? 0.0L : -kPiL * (32 << e) * j / (2.0L * L - 1.0L);
+      pf[2 * L - 1 + 16 * e + 8 + j] = make_double2((double)cosl(a), (double)sinl(a));
+    }
   P.phase_fwd = upload(pf);
   {
     std::vector<double2> ai(2 * L - 1), af(2 * L - 1);
